@@ -1,0 +1,292 @@
+"""ctypes binding of the C ABI declared in include/clotho_amd.h.
+
+The Python layer is a thin host-side mirror for tests and benchmarks; all
+compute runs in libclotho_amd.so (HIP kernels for gfx950).  Loading fails
+loudly when the library has not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libclotho_amd.so")
+
+FIT_CONSTANT = 0
+FIT_GAUSSIAN = 1
+FIT_QUADRATIC = 2
+FIT_GAUSSIAN_FIXED = 3
+FIT_MV_STABILIZING = 4
+
+TRAITS_PLAIN_FP32 = 0
+TRAITS_MFMA_BF16 = 1
+TRAITS_EXACT_FP64 = 2
+
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded", -4: "bad state"}
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("replicate", C.c_uint32),
+        ("n_individuals", C.c_uint32),
+        ("trait_count", C.c_uint32),
+        ("fitness_kind", C.c_uint32),
+        ("mu", C.c_double),
+        ("rho", C.c_double),
+        ("bias", C.c_double),
+        ("neutral_p", C.c_double),
+        ("trait_mean", C.c_double),
+        ("trait_sigma", C.c_double),
+        ("fitness_scale", C.c_double),
+        ("env_sigma", C.c_double),
+        ("effect_quantum_log2", C.c_int32),
+        ("max_alleles", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
+    ]
+
+
+class Shape(C.Structure):
+    _fields_ = [
+        ("generation", C.c_uint32),
+        ("n_individuals", C.c_uint32),
+        ("haplotypes", C.c_uint32),
+        ("words", C.c_uint32),
+        ("alleles", C.c_uint32),
+        ("new_mutations", C.c_uint32),
+        ("fixed_count", C.c_uint32),
+        ("lost_count", C.c_uint32),
+        ("free_count", C.c_uint32),
+        ("variable_count", C.c_uint32),
+        ("all_neutral", C.c_uint32),
+        ("fitness_constant", C.c_uint32),
+        ("error", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, neutral_p=0.5,
+                mean=0.0, sigma=1.0, fitness_scale=1.0, env_sigma=0.0, quantum_log2=-24,
+                max_alleles=1 << 20, replicate=0):
+    """Build a Config; defaults follow the reference's parameter defaults
+    (sequence_bias.p 0.5, neutral.p 0.5, traits.mean 0, traits.sigma 1)."""
+    c = Config()
+    c.seed = seed
+    c.replicate = replicate
+    c.n_individuals = n
+    c.trait_count = traits
+    c.fitness_kind = fitness
+    c.mu = mu
+    c.rho = rho
+    c.bias = bias
+    c.neutral_p = neutral_p
+    c.trait_mean = mean
+    c.trait_sigma = sigma
+    c.fitness_scale = fitness_scale
+    c.env_sigma = env_sigma
+    c.effect_quantum_log2 = quantum_log2
+    c.max_alleles = max_alleles
+    return c
+
+
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+EXPORTS = [
+    "clotho_create", "clotho_destroy", "clotho_last_error", "clotho_version",
+    "clotho_poisson_table", "clotho_set_population", "clotho_get_shape",
+    "clotho_download_population", "clotho_download_alleles", "clotho_download_free_space",
+    "clotho_download_traits", "clotho_download_fitness", "clotho_download_gametes",
+    "clotho_run", "clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
+    "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit", "clotho_sync",
+    "clotho_timings", "clotho_set_timing", "clotho_trait_contract",
+]
+
+
+def lib():
+    """Load libclotho_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.clotho_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.clotho_create.restype = C.c_int
+    L.clotho_destroy.argtypes = [vp]
+    L.clotho_last_error.argtypes = [vp]
+    L.clotho_last_error.restype = C.c_char_p
+    L.clotho_version.restype = C.c_char_p
+    L.clotho_poisson_table.argtypes = [C.c_double, f64p, C.c_int]
+    L.clotho_poisson_table.restype = C.c_int
+    L.clotho_set_population.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp]
+    L.clotho_get_shape.argtypes = [vp, vp]
+    L.clotho_download_population.argtypes = [vp, vp, C.c_size_t]
+    L.clotho_download_alleles.argtypes = [vp, vp, vp, vp, C.c_size_t]
+    L.clotho_download_free_space.argtypes = [vp, vp, vp, C.c_size_t]
+    L.clotho_download_traits.argtypes = [vp, vp, C.c_size_t]
+    L.clotho_download_fitness.argtypes = [vp, vp, C.c_size_t]
+    L.clotho_download_gametes.argtypes = [vp, vp, vp, vp, vp, C.c_size_t]
+    L.clotho_run.argtypes = [vp, C.c_uint32, C.c_int]
+    for n in ("clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
+              "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit",
+              "clotho_sync"):
+        getattr(L, n).argtypes = [vp]
+    L.clotho_timings.argtypes = [vp, f64p, C.c_int]
+    L.clotho_set_timing.argtypes = [vp, C.c_int]
+    L.clotho_set_timing.restype = None
+    L.clotho_trait_contract.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, C.c_uint32,
+                                        C.c_int, C.c_int, C.c_int, vp, C.POINTER(C.c_double),
+                                        C.c_int]
+    _lib = L
+    return L
+
+
+class ClothoError(RuntimeError):
+    pass
+
+
+class Simulation:
+    """One context: a population pair and its allele table on one GPU."""
+
+    def __init__(self, cfg: Config, device: int = 0):
+        self.cfg = cfg
+        self.ctx = C.c_void_p()
+        rc = lib().clotho_create(C.addressof(cfg), device, C.byref(self.ctx))
+        if rc != 0:
+            raise ClothoError(f"clotho_create failed ({ERRORS.get(rc, rc)})")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().clotho_last_error(self.ctx)
+            raise ClothoError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.ctx:
+            lib().clotho_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_population(self, rows, positions, effects, fitness=None):
+        rows = np.ascontiguousarray(rows, np.uint64)
+        positions = np.ascontiguousarray(positions, np.float64)
+        effects = np.ascontiguousarray(effects, np.float64)
+        fp = None
+        if fitness is not None:
+            fitness = np.ascontiguousarray(fitness, np.float64)
+            fp = fitness.ctypes.data
+        self._check(lib().clotho_set_population(self.ctx, rows.ctypes.data, rows.shape[1],
+                                                positions.ctypes.data, effects.ctypes.data,
+                                                positions.size, fp), "set_population")
+
+    def run(self, n=1, graph=False):
+        self._check(lib().clotho_run(self.ctx, n, 1 if graph else 0), "run")
+
+    def generation(self):
+        self._check(lib().clotho_generation(self.ctx), "generation")
+
+    def phase(self, name):
+        self._check(getattr(lib(), "clotho_phase_" + name)(self.ctx), name)
+
+    def sync(self):
+        self._check(lib().clotho_sync(self.ctx), "sync")
+
+    def shape(self) -> Shape:
+        s = Shape()
+        self._check(lib().clotho_get_shape(self.ctx, C.addressof(s)), "get_shape")
+        return s
+
+    def rows(self):
+        s = self.shape()
+        out = np.zeros((s.haplotypes, s.words), np.uint64)
+        self._check(lib().clotho_download_population(self.ctx, out.ctypes.data, out.size),
+                    "download_population")
+        return out
+
+    def alleles(self):
+        s = self.shape()
+        t = self.cfg.trait_count
+        pos = np.zeros(max(s.alleles, 1), np.float64)
+        eff = np.zeros(max(s.alleles * t, 1), np.float64)
+        age = np.zeros(max(s.alleles, 1), np.uint32)
+        self._check(lib().clotho_download_alleles(self.ctx, pos.ctypes.data, eff.ctypes.data,
+                                                  age.ctypes.data, s.alleles),
+                    "download_alleles")
+        return pos[: s.alleles], eff[: s.alleles * t].reshape(s.alleles, t), age[: s.alleles]
+
+    def free_space(self):
+        s = self.shape()
+        fx = np.zeros(max(s.fixed_count, 1), np.uint32)
+        ls = np.zeros(max(s.lost_count, 1), np.uint32)
+        self._check(lib().clotho_download_free_space(self.ctx, fx.ctypes.data, ls.ctypes.data,
+                                                     max(s.fixed_count, s.lost_count, 1)),
+                    "download_free_space")
+        return fx[: s.fixed_count], ls[: s.lost_count]
+
+    def traits(self):
+        s = self.shape()
+        out = np.zeros((s.haplotypes, self.cfg.trait_count), np.float64)
+        self._check(lib().clotho_download_traits(self.ctx, out.ctypes.data, out.size),
+                    "download_traits")
+        return out
+
+    def fitness(self):
+        s = self.shape()
+        out = np.zeros(s.n_individuals, np.float64)
+        self._check(lib().clotho_download_fitness(self.ctx, out.ctypes.data, out.size),
+                    "download_fitness")
+        return out
+
+    def gametes(self):
+        s = self.shape()
+        h = s.haplotypes
+        top, bot, ev, mu = (np.zeros(h, np.uint32) for _ in range(4))
+        self._check(lib().clotho_download_gametes(self.ctx, top.ctypes.data, bot.ctypes.data,
+                                                  ev.ctypes.data, mu.ctypes.data, h),
+                    "download_gametes")
+        return top, bot, ev, mu
+
+    def set_timing(self, on=True):
+        lib().clotho_set_timing(self.ctx, 1 if on else 0)
+
+    def timings(self):
+        out = np.zeros(8, np.float64)
+        n = lib().clotho_timings(self.ctx, out, out.size)
+        return n, out
+
+
+def poisson_table(lam, cap=128):
+    out = np.zeros(cap, np.float64)
+    n = lib().clotho_poisson_table(float(lam), out, cap)
+    if n < 0:
+        raise ClothoError("poisson table: lambda out of range")
+    return out[:n].copy()
+
+
+def trait_contract(rows, effects, mode, split_terms=3, repeats=1, device=0):
+    """z = genotypes x effects on the GPU; returns (z [H,T] float64, avg kernel ms)."""
+    rows = np.ascontiguousarray(rows, np.uint64)
+    effects = np.ascontiguousarray(effects, np.float64)
+    h, w = rows.shape
+    s, t = effects.shape
+    z = np.zeros((h, t), np.float64)
+    ms = C.c_double(0.0)
+    rc = lib().clotho_trait_contract(rows.ctypes.data, h, w, effects.ctypes.data, s, t, mode,
+                                     split_terms, repeats, z.ctypes.data, C.byref(ms), device)
+    if rc != 0:
+        raise ClothoError(f"trait_contract failed ({ERRORS.get(rc, rc)})")
+    return z, ms.value

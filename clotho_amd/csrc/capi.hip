@@ -1,0 +1,616 @@
+// capi.hip — C ABI of the offspring pipeline (include/clotho_amd.h): context
+// lifetime, device buffers sized for one population pair, state transfer,
+// generation sequencing (eager launches or a captured hipGraph per parity),
+// and per-phase HIP-event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "device_common.hpp"
+#include "pipeline.hpp"
+
+using namespace clotho;
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);        \
+            return CLOTHO_ERR_HIP;                                                  \
+        }                                                                           \
+    } while (0)
+
+int fail(clotho_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+// Poisson(lambda) inversion table, cdf[k] = P(X <= k), last entry forced to 1.
+int poisson_table(double lambda, double* cdf, int cap) {
+    if (!(lambda >= 0.0) || lambda > 64.0 || cap < 2) return -1;
+    double pk = std::exp(-lambda);
+    double c = pk;
+    int k = 0;
+    cdf[0] = c;
+    while (k + 1 < cap && c < 1.0 - 1e-17 && !(k > lambda && pk < 1e-18)) {
+        ++k;
+        pk = pk * lambda / (double)k;
+        c = c + pk;
+        cdf[k] = c;
+    }
+    cdf[k] = 1.0;
+    return k + 1;
+}
+
+template <typename T>
+int dalloc(clotho_ctx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(c, hipMalloc((void**)p, count * sizeof(T)));
+    return CLOTHO_OK;
+}
+
+void free_buffers(clotho_ctx* c) {
+    Buffers& B = c->b;
+    void* ptrs[] = {B.rows[0],    B.rows[1],   B.traits[0], B.traits[1],  B.fitness[0],
+                    B.fitness[1], B.keys,      B.effects,   B.ages,       B.free_list,
+                    B.fixed_bits, B.mut_count, B.mut_local, B.mut_bsum,   B.top,
+                    B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
+                    B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
+                    B.cdf_rho,    B.cdf_mu,    B.state};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    std::memset(&B, 0, sizeof(B));
+}
+
+void destroy_graphs(clotho_ctx* c) {
+    for (int i = 0; i < 2; ++i)
+        if (c->graphs.exec[i]) {
+            (void)hipGraphExecDestroy(c->graphs.exec[i]);
+            c->graphs.exec[i] = nullptr;
+        }
+    c->graphs.valid = false;
+}
+
+int read_state(clotho_ctx* c, GenState* st) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(st, c->b.state, sizeof(GenState), hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int write_state(clotho_ctx* c, const GenState& st) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->b.state, &st, sizeof(GenState), hipMemcpyHostToDevice));
+    return CLOTHO_OK;
+}
+
+int enqueue_generation(clotho_ctx* c, std::vector<hipEvent_t>* marks) {
+    auto mark = [&](int i) -> hipError_t {
+        if (!marks) return hipSuccess;
+        return hipEventRecord((*marks)[i], c->stream);
+    };
+    HIP_TRY(c, mark(0));
+    HIP_TRY(c, launch_draw(c));
+    HIP_TRY(c, mark(1));
+    HIP_TRY(c, launch_recombine(c));
+    HIP_TRY(c, mark(2));
+    HIP_TRY(c, launch_free_space(c));
+    HIP_TRY(c, mark(3));
+    HIP_TRY(c, launch_fitness(c));
+    HIP_TRY(c, mark(4));
+    HIP_TRY(c, launch_commit(c));
+    HIP_TRY(c, mark(5));
+    return CLOTHO_OK;
+}
+
+int capture_graphs(clotho_ctx* c) {
+    if (c->graphs.valid) return CLOTHO_OK;
+    const int saved = c->cur;
+    for (int k = 0; k < 2; ++k) {
+        c->cur = k;
+        hipGraph_t g = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_generation(c, nullptr);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != CLOTHO_OK) {
+            c->cur = saved;
+            return rc;
+        }
+        if (e != hipSuccess) {
+            c->cur = saved;
+            return fail(c, CLOTHO_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        }
+        e = hipGraphInstantiate(&c->graphs.exec[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->cur = saved;
+            return fail(c, CLOTHO_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+        }
+    }
+    c->cur = saved;
+    c->graphs.valid = true;
+    return CLOTHO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* clotho_version(void) { return "clotho_amd 0.1 (gfx950)"; }
+
+const char* clotho_last_error(const clotho_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int clotho_poisson_table(double lambda, double* cdf, int cap) {
+    return poisson_table(lambda, cdf, cap);
+}
+
+int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
+    if (!cfg || !out) return CLOTHO_ERR_ARG;
+    *out = nullptr;
+    if (cfg->n_individuals == 0 || cfg->n_individuals > (1u << 28)) return CLOTHO_ERR_ARG;
+    if (cfg->trait_count > (uint32_t)kMaxTraits) return CLOTHO_ERR_ARG;
+    if (!(cfg->bias >= 0.0 && cfg->bias <= 1.0)) return CLOTHO_ERR_ARG;
+    if (!(cfg->neutral_p >= 0.0 && cfg->neutral_p <= 1.0)) return CLOTHO_ERR_ARG;
+    if (cfg->fitness_kind > CLOTHO_FIT_MV_STABILIZING) return CLOTHO_ERR_ARG;
+    if (cfg->effect_quantum_log2 > 0 || cfg->effect_quantum_log2 < -60) return CLOTHO_ERR_ARG;
+    if (cfg->max_alleles == 0 || cfg->max_alleles > (1u << 30)) return CLOTHO_ERR_ARG;
+    if ((cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN || cfg->fitness_kind == CLOTHO_FIT_QUADRATIC) &&
+        !(cfg->mu > 0.0))
+        return CLOTHO_ERR_ARG;
+    if ((cfg->fitness_kind == CLOTHO_FIT_QUADRATIC ||
+         cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN_FIXED ||
+         cfg->fitness_kind == CLOTHO_FIT_MV_STABILIZING) &&
+        !(cfg->fitness_scale > 0.0))
+        return CLOTHO_ERR_ARG;
+
+    clotho_ctx* c = new (std::nothrow) clotho_ctx();
+    if (!c) return CLOTHO_ERR_ARG;
+    c->cfg = *cfg;
+    c->device = device;
+    double cdf_rho[kMaxCdf], cdf_mu[kMaxCdf];
+    const int nr = poisson_table(cfg->rho, cdf_rho, kMaxCdf);
+    const int nm = poisson_table(cfg->mu, cdf_mu, kMaxCdf);
+    if (nr < 0 || nm < 0) {
+        delete c;
+        return CLOTHO_ERR_ARG;
+    }
+    auto bail = [&](int rc) {
+        free_buffers(c);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(CLOTHO_ERR_HIP);
+    if (hipGetDeviceProperties(&c->prop, device) != hipSuccess) return bail(CLOTHO_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(CLOTHO_ERR_HIP);
+
+    Params& P = c->p;
+    std::memset(&P, 0, sizeof(P));
+    P.seed = cfg->seed;
+    P.replicate = cfg->replicate;
+    P.n = cfg->n_individuals;
+    P.h = 2 * P.n;
+    P.traits = cfg->trait_count;
+    P.s_cap = ((cfg->max_alleles + 511) / 512) * 512;
+    P.w_cap = P.s_cap / 64;
+    P.n_cdf_rho = (uint32_t)nr;
+    P.n_cdf_mu = (uint32_t)nm;
+    P.ev_cap = (uint32_t)(((std::max(nr - 1, 1)) + 3) / 4 * 4);
+    P.bias = cfg->bias;
+    P.neutral_p = cfg->neutral_p;
+    P.trait_mean = cfg->trait_mean;
+    P.trait_sigma = cfg->trait_sigma;
+    P.env_sigma = cfg->env_sigma;
+    P.eff_scale = std::ldexp(1.0, -cfg->effect_quantum_log2);
+    P.eff_quantum = std::ldexp(1.0, cfg->effect_quantum_log2);
+    P.fitness_kind = cfg->fitness_kind;
+    P.fit_coeff = 1.0;
+    P.fit_inv = 1.0;
+    // ifitness_generator::generate(N): constants for N individuals
+    if (cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN) {
+        const double sigma = std::sqrt(4.0 * (double)P.n * cfg->mu);
+        P.fit_coeff = (1.0 / sigma) * 0.398942280401432677939946059934;
+        P.fit_inv = 1.0 / (2.0 * sigma * sigma);
+    } else if (cfg->fitness_kind == CLOTHO_FIT_QUADRATIC) {
+        const double s = cfg->fitness_scale * std::sqrt(4.0 * (double)P.n * cfg->mu);
+        P.fit_inv = 1.0 / s;
+    } else if (cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN_FIXED ||
+               cfg->fitness_kind == CLOTHO_FIT_MV_STABILIZING) {
+        P.fit_inv = 1.0 / (2.0 * cfg->fitness_scale * cfg->fitness_scale);
+    }
+    P.tiles = (P.w_cap + 63) / 64;
+    {
+        const uint32_t target_blocks = 4096;
+        uint32_t ch = (target_blocks + P.tiles - 1) / P.tiles;
+        ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, P.h / 16));
+        P.chunks = std::max<uint32_t>(1, ch);
+    }
+    P.mut_blocks = (P.h + kScanBlock - 1) / kScanBlock;
+    P.word_blocks = (P.w_cap + kScanBlock - 1) / kScanBlock;
+    const uint32_t sel_blocks = (P.n + kScanBlock - 1) / kScanBlock;
+    const uint32_t T1 = std::max<uint32_t>(1, P.traits);
+
+    Buffers& B = c->b;
+    std::memset(&B, 0, sizeof(B));
+    int rc = CLOTHO_OK;
+    for (int k = 0; k < 2 && rc == CLOTHO_OK; ++k) {
+        rc |= dalloc(c, &B.rows[k], (size_t)P.h * P.w_cap);
+        rc |= dalloc(c, &B.traits[k], (size_t)P.h * T1);
+        rc |= dalloc(c, &B.fitness[k], P.n);
+    }
+    rc |= dalloc(c, &B.keys, P.s_cap);
+    rc |= dalloc(c, &B.effects, (size_t)P.s_cap * T1);
+    rc |= dalloc(c, &B.ages, P.s_cap);
+    rc |= dalloc(c, &B.free_list, P.s_cap + 1);
+    rc |= dalloc(c, &B.fixed_bits, P.w_cap);
+    rc |= dalloc(c, &B.mut_count, P.h);
+    rc |= dalloc(c, &B.mut_local, P.h);
+    rc |= dalloc(c, &B.mut_bsum, P.mut_blocks);
+    rc |= dalloc(c, &B.top, P.h);
+    rc |= dalloc(c, &B.bottom, P.h);
+    rc |= dalloc(c, &B.n_events, P.h);
+    rc |= dalloc(c, &B.events, (size_t)P.h * P.ev_cap);
+    rc |= dalloc(c, &B.var_part, (size_t)(P.chunks + 1) * P.w_cap);
+    rc |= dalloc(c, &B.fix_part, (size_t)P.chunks * P.w_cap);
+    rc |= dalloc(c, &B.word_cnt, P.w_cap);
+    rc |= dalloc(c, &B.word_bsum, 2 * P.word_blocks);
+    rc |= dalloc(c, &B.cum_local, P.n);
+    rc |= dalloc(c, &B.cum_block, sel_blocks);
+    rc |= dalloc(c, &B.fit_stats, 2 * sel_blocks);
+    rc |= dalloc(c, &B.cdf_rho, kMaxCdf);
+    rc |= dalloc(c, &B.cdf_mu, kMaxCdf);
+    rc |= dalloc(c, &B.state, 1);
+    if (rc != CLOTHO_OK) return bail(rc);
+
+    // first generation: empty founders, fitness 1.0 (GeneralFitness::resize)
+    bool ok = hipMemcpy(B.cdf_rho, cdf_rho, sizeof(double) * nr, hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(B.cdf_mu, cdf_mu, sizeof(double) * nm, hipMemcpyHostToDevice) ==
+                  hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k)
+        ok = hipMemset(B.traits[k], 0, sizeof(int64_t) * P.h * T1) == hipSuccess;
+    ok = ok && hipMemset(B.keys, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
+         hipMemset(B.effects, 0, sizeof(int32_t) * P.s_cap * T1) == hipSuccess &&
+         hipMemset(B.ages, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
+         hipMemset(B.fixed_bits, 0, sizeof(uint64_t) * P.w_cap) == hipSuccess;
+    if (ok) {
+        std::vector<double> ones(P.n, 1.0);
+        ok = hipMemcpy(B.fitness[0], ones.data(), 8 * P.n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    GenState st;
+    std::memset(&st, 0, sizeof(st));
+    st.gen = 1;  // Engine::init leaves m_generation at 1 before the first simulate
+    st.n = P.n;
+    st.fit_constant = 1;
+    st.sel_blocks = sel_blocks;
+    ok = ok && hipMemcpy(B.state, &st, sizeof(st), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return bail(CLOTHO_ERR_HIP);
+    c->cur = 0;
+    *out = c;
+    return CLOTHO_OK;
+}
+
+void clotho_destroy(clotho_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    destroy_graphs(ctx);
+    for (hipEvent_t& e : ctx->timing.ev)
+        if (e) (void)hipEventDestroy(e);
+    free_buffers(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int clotho_set_population(clotho_ctx* c, const uint64_t* rows, uint32_t words,
+                          const double* positions, const double* effects, uint32_t alleles,
+                          const double* fitness) {
+    if (!c) return CLOTHO_ERR_ARG;
+    const Params& P = c->p;
+    if (words % 8 || words > P.w_cap || (size_t)words * 64 < alleles || alleles > P.s_cap)
+        return fail(c, CLOTHO_ERR_ARG, "set_population: words/alleles exceed capacity");
+    if ((words && !rows) || (alleles && !positions) || (alleles && P.traits && !effects))
+        return fail(c, CLOTHO_ERR_ARG, "set_population: null buffer");
+    std::vector<uint32_t> keys(P.s_cap, 0);
+    std::vector<int32_t> eq((size_t)P.s_cap * std::max<uint32_t>(1, P.traits), 0);
+    for (uint32_t j = 0; j < alleles; ++j) {
+        const double x = positions[j];
+        const double k = x * 4294967296.0;
+        if (!(x >= 0.0 && x < 1.0) || k != std::floor(k))
+            return fail(c, CLOTHO_ERR_ARG, "set_population: positions must be k*2^-32 in [0,1)");
+        keys[j] = (uint32_t)k;
+        for (uint32_t t = 0; t < P.traits; ++t) {
+            const double w = effects[(size_t)j * P.traits + t] * P.eff_scale;
+            if (w != std::floor(w) || std::fabs(w) > 2147483647.0)
+                return fail(c, CLOTHO_ERR_ARG,
+                            "set_population: effects must be integer multiples of 2^q");
+            eq[(size_t)j * P.traits + t] = (int32_t)w;
+        }
+    }
+    // the bit of a slot >= alleles must be clear in every row
+    if (words && alleles < (size_t)words * 64) {
+        const uint32_t wfull = alleles / 64;
+        for (uint32_t r = 0; r < P.h; ++r) {
+            const uint64_t* row = rows + (size_t)r * words;
+            for (uint32_t w = wfull; w < words; ++w) {
+                const uint32_t lo = w * 64;
+                const uint64_t allowed =
+                    alleles > lo ? (alleles - lo >= 64 ? ~0ull : ((1ull << (alleles - lo)) - 1)) : 0;
+                if (row[w] & ~allowed)
+                    return fail(c, CLOTHO_ERR_ARG, "set_population: bit set beyond allele count");
+            }
+        }
+    }
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const Buffers& B = c->b;
+    const int cur = c->cur;
+    if (words)
+        HIP_TRY(c, hipMemcpy(B.rows[cur], rows, sizeof(uint64_t) * P.h * words,
+                             hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(B.keys, keys.data(), 4 * keys.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(B.effects, eq.data(), 4 * eq.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(B.ages, 0, 4 * P.s_cap));
+    std::vector<double> fit(P.n, 1.0);
+    if (fitness) std::copy(fitness, fitness + P.n, fit.begin());
+    HIP_TRY(c, hipMemcpy(B.fitness[cur], fit.data(), 8 * P.n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(B.traits[cur], 0, sizeof(int64_t) * P.h * std::max<uint32_t>(1, P.traits)));
+    st.alleles = alleles;
+    st.all_size = alleles;
+    st.wc = words;
+    st.wp = words;
+    st.m = 0;
+    st.error = 0;
+    rc = write_state(c, st);
+    if (rc) return rc;
+    HIP_TRY(c, launch_population_analysis(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->has_population = true;
+    return CLOTHO_OK;
+}
+
+int clotho_get_shape(clotho_ctx* c, clotho_shape* out) {
+    if (!c || !out) return CLOTHO_ERR_ARG;
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->generation = st.gen;
+    out->n_individuals = c->p.n;
+    out->haplotypes = c->p.h;
+    out->words = st.wc;
+    out->alleles = st.all_size;
+    out->new_mutations = st.m;
+    out->fixed_count = st.n_fixed;
+    out->lost_count = st.free_count - st.n_fixed;
+    out->free_count = st.free_count;
+    out->variable_count = st.all_size - st.free_count;
+    out->error = st.error;
+    // all-neutral over the allele table, fitness constancy of the current population
+    uint32_t* d_flags = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&d_flags, 64 * sizeof(uint32_t)));
+    hipError_t e = launch_all_neutral(c, d_flags);
+    uint32_t flags[64] = {0};
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost);
+    (void)hipFree(d_flags);
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    uint32_t any = 0;
+    for (uint32_t f : flags) any |= f;
+    out->all_neutral = any ? 0 : 1;
+    std::vector<double> fit(c->p.n);
+    HIP_TRY(c, hipMemcpy(fit.data(), c->b.fitness[c->cur], 8 * fit.size(), hipMemcpyDeviceToHost));
+    bool constant = true;
+    for (double f : fit)
+        if (!(f == fit[0])) {
+            constant = false;
+            break;
+        }
+    out->fitness_constant = constant ? 1 : 0;
+    return CLOTHO_OK;
+}
+
+int clotho_download_population(clotho_ctx* c, uint64_t* rows, size_t cap_words) {
+    if (!c || !rows) return CLOTHO_ERR_ARG;
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const size_t n = (size_t)c->p.h * st.wc;
+    if (cap_words < n) return fail(c, CLOTHO_ERR_ARG, "download_population: buffer too small");
+    if (n)
+        HIP_TRY(c, hipMemcpy(rows, c->b.rows[c->cur], n * 8, hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int clotho_download_alleles(clotho_ctx* c, double* positions, double* effects, uint32_t* ages,
+                            size_t cap) {
+    if (!c) return CLOTHO_ERR_ARG;
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t A = st.all_size, T = c->p.traits;
+    if (cap < A) return fail(c, CLOTHO_ERR_ARG, "download_alleles: buffer too small");
+    if (positions && A) {
+        std::vector<uint32_t> k(A);
+        HIP_TRY(c, hipMemcpy(k.data(), c->b.keys, 4 * A, hipMemcpyDeviceToHost));
+        for (uint32_t j = 0; j < A; ++j) positions[j] = (double)k[j] * 0x1p-32;
+    }
+    if (effects && A && T) {
+        std::vector<int32_t> q((size_t)A * T);
+        HIP_TRY(c, hipMemcpy(q.data(), c->b.effects, 4 * q.size(), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < q.size(); ++j) effects[j] = (double)q[j] * c->p.eff_quantum;
+    }
+    if (ages && A) HIP_TRY(c, hipMemcpy(ages, c->b.ages, 4 * A, hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int clotho_download_free_space(clotho_ctx* c, uint32_t* fixed, uint32_t* lost, size_t cap) {
+    if (!c) return CLOTHO_ERR_ARG;
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t nf = st.n_fixed, nl = st.free_count - st.n_fixed;
+    if (cap < std::max(nf, nl)) return fail(c, CLOTHO_ERR_ARG, "download_free_space: too small");
+    if (fixed && nf)
+        HIP_TRY(c, hipMemcpy(fixed, c->b.free_list, 4 * nf, hipMemcpyDeviceToHost));
+    if (lost && nl)
+        HIP_TRY(c, hipMemcpy(lost, c->b.free_list + nf, 4 * nl, hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int clotho_download_traits(clotho_ctx* c, double* out, size_t cap) {
+    if (!c || !out) return CLOTHO_ERR_ARG;
+    const size_t n = (size_t)c->p.h * c->p.traits;
+    if (cap < n) return fail(c, CLOTHO_ERR_ARG, "download_traits: buffer too small");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!n) return CLOTHO_OK;
+    std::vector<int64_t> q(n);
+    HIP_TRY(c, hipMemcpy(q.data(), c->b.traits[c->cur], 8 * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) out[i] = (double)q[i] * c->p.eff_quantum;
+    return CLOTHO_OK;
+}
+
+int clotho_download_fitness(clotho_ctx* c, double* out, size_t cap) {
+    if (!c || !out) return CLOTHO_ERR_ARG;
+    if (cap < c->p.n) return fail(c, CLOTHO_ERR_ARG, "download_fitness: buffer too small");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->b.fitness[c->cur], 8 * c->p.n, hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int clotho_download_gametes(clotho_ctx* c, uint32_t* top, uint32_t* bottom, uint32_t* events,
+                            uint32_t* mutations, size_t cap) {
+    if (!c) return CLOTHO_ERR_ARG;
+    const size_t h = c->p.h;
+    if (cap < h) return fail(c, CLOTHO_ERR_ARG, "download_gametes: buffer too small");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (top) HIP_TRY(c, hipMemcpy(top, c->b.top, 4 * h, hipMemcpyDeviceToHost));
+    if (bottom) HIP_TRY(c, hipMemcpy(bottom, c->b.bottom, 4 * h, hipMemcpyDeviceToHost));
+    if (events) HIP_TRY(c, hipMemcpy(events, c->b.n_events, 4 * h, hipMemcpyDeviceToHost));
+    if (mutations) HIP_TRY(c, hipMemcpy(mutations, c->b.mut_count, 4 * h, hipMemcpyDeviceToHost));
+    return CLOTHO_OK;
+}
+
+int clotho_phase_draw(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_draw(c));
+    return CLOTHO_OK;
+}
+int clotho_phase_recombine_mutate(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_recombine(c));
+    return CLOTHO_OK;
+}
+int clotho_phase_free_space(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_free_space(c));
+    return CLOTHO_OK;
+}
+int clotho_phase_fitness(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_fitness(c));
+    return CLOTHO_OK;
+}
+int clotho_phase_commit(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_commit(c));
+    c->cur = 1 - c->cur;
+    return CLOTHO_OK;
+}
+int clotho_sync(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return CLOTHO_OK;
+}
+
+void clotho_set_timing(clotho_ctx* c, int enabled) {
+    if (c) c->timing.enabled = enabled != 0;
+}
+
+int clotho_timings(clotho_ctx* c, double* ms, int cap) {
+    if (!c || !ms) return CLOTHO_ERR_ARG;
+    for (int i = 0; i < cap && i < 5; ++i) ms[i] = c->timing.ms[i];
+    return c->timing.generations;
+}
+
+int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    Timing& tm = c->timing;
+    for (double& v : tm.ms) v = 0.0;
+    tm.generations = 0;
+    std::vector<hipEvent_t> marks;
+    if (use_graph) {
+        int rc = capture_graphs(c);
+        if (rc) return rc;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (tm.enabled) {
+            HIP_TRY(c, hipEventCreate(&e0));
+            HIP_TRY(c, hipEventCreate(&e1));
+            HIP_TRY(c, hipEventRecord(e0, c->stream));
+        }
+        for (uint32_t g = 0; g < n; ++g) {
+            HIP_TRY(c, hipGraphLaunch(c->graphs.exec[c->cur], c->stream));
+            c->cur = 1 - c->cur;
+        }
+        if (tm.enabled) {
+            HIP_TRY(c, hipEventRecord(e1, c->stream));
+            HIP_TRY(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+            tm.ms[0] = ms;
+            tm.generations = (int)n;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    } else {
+        std::vector<std::vector<hipEvent_t>> all;
+        for (uint32_t g = 0; g < n; ++g) {
+            std::vector<hipEvent_t>* mk = nullptr;
+            if (tm.enabled) {
+                all.emplace_back(6);
+                for (hipEvent_t& e : all.back()) HIP_TRY(c, hipEventCreate(&e));
+                mk = &all.back();
+            }
+            int rc = enqueue_generation(c, mk);
+            if (rc) return rc;
+            c->cur = 1 - c->cur;
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (tm.enabled) {
+            for (auto& mk : all) {
+                float t = 0.f;
+                HIP_TRY(c, hipEventElapsedTime(&t, mk[0], mk[5]));
+                tm.ms[0] += t;
+                for (int q = 0; q < 4; ++q) {
+                    HIP_TRY(c, hipEventElapsedTime(&t, mk[q], mk[q + 1]));
+                    tm.ms[1 + q] += t;
+                }
+                for (hipEvent_t& e : mk) (void)hipEventDestroy(e);
+            }
+            tm.generations = (int)n;
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    if (st.error)
+        return fail(c, CLOTHO_ERR_CAPACITY,
+                    "allele capacity exceeded at generation " + std::to_string(st.error_gen));
+    c->has_population = true;
+    return CLOTHO_OK;
+}
+
+int clotho_generation(clotho_ctx* c) { return clotho_run(c, 1, 0); }
+
+}  // extern "C"

@@ -1,0 +1,842 @@
+// pipeline.hip — gfx950 kernels of one generation of the offspring pipeline.
+//
+// Generation = draw -> recombine+mutate (+fused free-space partials and the
+// incremental single-trait update) -> free-space reduce/compact -> fitness
+// -> commit, all on the context stream, all shapes read from GenState so a
+// captured hipGraph replays generation after generation.
+//
+// Reference path restated (DS/ = include/clotho/data_spaces/,
+// QS/ = examples/QTLSim/):
+//   QS/engine_parallel_pipeline.hpp:172-239 (simulate), :327-367 (mutations),
+//   :393-447 (child_max_alleles, removeFixedAlleles);
+//   DS/offspring_generator/offspring_generator_population_space_row_alignment.hpp:84-260;
+//   DS/crossover/block_crossover.hpp:46-63; DS/free_space/free_space_details.hpp:359-408;
+//   DS/fitness/general_fitness.hpp:148-177.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "pipeline.hpp"
+
+namespace clotho {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+__device__ inline uint32_t poisson_draw(const double* cdf, int n, double u) {
+    uint32_t k = 0;
+    while ((int)k + 1 < n && !(u < cdf[k])) ++k;
+    return k;
+}
+
+// Inclusive scan over a 256-thread block (four 64-lane waves).
+template <typename T>
+__device__ inline T block_inclusive_scan(T v, T* lds, T& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const T y = __shfl_up(x, d, 64);
+        if (lane >= d) x = x + y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+        if (q < wave) off = off + lds[q];
+        tot = tot + lds[q];
+    }
+    __syncthreads();
+    total = tot;
+    return off + x;
+}
+
+__device__ inline uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------
+// Draw phase
+// ---------------------------------------------------------------------------
+
+// Per-gamete mutation counts, Poisson(mu) by inversion (the reference's
+// Poisson(mu * 2N) total with uniform rows has the same law), and their
+// block-local exclusive scan.
+__global__ void __launch_bounds__(kBlock) k_mut_counts(Params P, const GenState* st,
+                                                       const double* cdf_mu, uint32_t* count,
+                                                       uint32_t* local, uint32_t* bsum) {
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t gen = st->gen;
+    uint32_t c = 0;
+    if (o < P.h) {
+        const Draw4 d = philox_draw(P.seed, gen, S_MUTCOUNT, P.replicate, o, 0);
+        c = poisson_draw(cdf_mu, P.n_cdf_mu, u53(d.x[0], d.x[1]));
+        count[o] = c;
+    }
+    uint32_t total;
+    const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
+    if (o < P.h) local[o] = inc - c;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Per-block max and all-equal flag of the parent fitness (selection set-up).
+__global__ void __launch_bounds__(kBlock) k_fitness_stats(Params P, const double* fit,
+                                                          double* stats) {
+    __shared__ double smax[kBlock];
+    __shared__ int seq[kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const double f0 = fit[0];
+    double m = 0.0;
+    int eq = 1;
+    if (i < P.n) {
+        const double f = fit[i];
+        m = f;
+        eq = (f == f0) ? 1 : 0;
+    }
+    smax[threadIdx.x] = m;
+    seq[threadIdx.x] = eq;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            seq[threadIdx.x] = seq[threadIdx.x] & seq[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[2 * blockIdx.x] = smax[0];
+        stats[2 * blockIdx.x + 1] = seq[0] ? 1.0 : 0.0;
+    }
+}
+
+// Single block: mutation offsets, child allele count / width (child_max_alleles),
+// capacity check, fixed-column effect sums, selection quantisation exponent.
+__global__ void __launch_bounds__(1024) k_prepare(Params P, GenState* st, uint32_t* mut_bsum,
+                                                  const double* stats, const uint32_t* free_list,
+                                                  const int32_t* effects) {
+    __shared__ uint32_t s_carry;
+    __shared__ uint32_t s_wave[16];
+    __shared__ int64_t s_red[1024];
+    __shared__ double s_max[1024];
+    __shared__ int s_eq[1024];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    // exclusive scan of the per-block mutation totals, in place
+    for (uint32_t base = 0; base < P.mut_blocks; base += 1024) {
+        const uint32_t idx = base + tid;
+        const uint32_t v = idx < P.mut_blocks ? mut_bsum[idx] : 0;
+        uint32_t x = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        uint32_t off = s_carry, tot = 0;
+        for (int q = 0; q < 16; ++q) {
+            if (q < wave) off += s_wave[q];
+            tot += s_wave[q];
+        }
+        if (idx < P.mut_blocks) mut_bsum[idx] = off + x - v;
+        __syncthreads();
+        if (tid == 0) s_carry += tot;
+        __syncthreads();
+    }
+    // fixed-column effect sums (removeFixedAlleles drops these effects)
+    const uint32_t nfix = st->n_fixed;
+    for (uint32_t t = 0; t < P.traits && t < (uint32_t)kMaxTraits; ++t) {
+        int64_t acc = 0;
+        for (uint32_t k = tid; k < nfix; k += 1024)
+            acc += (int64_t)effects[(size_t)free_list[k] * P.traits + t];
+        s_red[tid] = acc;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if (tid < s) s_red[tid] += s_red[tid + s];
+            __syncthreads();
+        }
+        if (tid == 0) st->fixed_sum[t] = s_red[0];
+        __syncthreads();
+    }
+    // selection: constant fitness -> uniform; else quantisation exponent
+    double m = 0.0;
+    int eq = 1;
+    for (uint32_t b = tid; b < st->sel_blocks; b += 1024) {
+        m = fmax(m, stats[2 * b]);
+        eq &= stats[2 * b + 1] != 0.0 ? 1 : 0;
+    }
+    s_max[tid] = m;
+    s_eq[tid] = eq;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (tid < s) {
+            s_max[tid] = fmax(s_max[tid], s_max[tid + s]);
+            s_eq[tid] &= s_eq[tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const uint32_t M = s_carry;
+        const uint32_t F = st->free_count;
+        const uint32_t all_size = st->alleles + (M > F ? M - F : 0);
+        st->m = M;
+        uint32_t as = all_size;
+        if (all_size > P.s_cap) {
+            if (!st->error) {
+                st->error = 1;
+                st->error_gen = st->gen;
+            }
+            as = P.s_cap;
+        }
+        st->all_size = as;
+        st->wc = ((as + 511) / 512) * 8;
+        const double mx = s_max[0];
+        int constant = s_eq[0] || !(mx > 0.0);
+        st->fit_constant = constant ? 1u : 0u;
+        if (!constant) {
+            int L = 0;
+            while (((uint64_t)1 << L) < P.n) ++L;
+            st->sel_shift = 52 - L - (det_ilogb(mx) + 1);
+        }
+    }
+}
+
+// New-allele attributes for every mutation of gamete o: slot = free[k] (fixed
+// ascending, then lost ascending) else appended (generate_child_mutations);
+// position uniform (AlleleGenerator), age = generation, per-trait effect
+// neutral ? 0 : quantised Normal(mean, sigma) (TraitSpaceGenerator2).
+__global__ void __launch_bounds__(kBlock) k_mut_attrs(Params P, const GenState* st,
+                                                      const uint32_t* count,
+                                                      const uint32_t* local,
+                                                      const uint32_t* boff,
+                                                      const uint32_t* free_list, uint32_t* keys,
+                                                      int32_t* effects, uint32_t* ages) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= P.h) return;
+    const uint32_t c = count[o];
+    if (!c) return;
+    const uint32_t gen = st->gen, F = st->free_count, A = st->alleles;
+    const uint32_t k0 = local[o] + boff[o / kBlock];
+    for (uint32_t k = k0; k < k0 + c; ++k) {
+        const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+        if (slot >= P.s_cap) continue;
+        const Draw4 d = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 0);
+        keys[slot] = d.x[0];
+        ages[slot] = gen;
+        for (uint32_t t = 0; t < P.traits; ++t) {
+            const Draw4 e = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 1 + t);
+            int32_t q = 0;
+            if (!(u53(e.x[0], e.x[1]) < P.neutral_p)) {
+                const double w = P.trait_mean + P.trait_sigma * normal_ppf(uopen(e.x[2], e.x[3]));
+                double n = rint(w * P.eff_scale);
+                if (n > 2147483647.0) n = 2147483647.0;
+                if (n < -2147483647.0) n = -2147483647.0;
+                q = (int32_t)n;
+            }
+            effects[(size_t)slot * P.traits + t] = q;
+        }
+    }
+}
+
+// Quantised fitness q_i = floor(f_i 2^shift) and its block-local inclusive
+// scan; every partial sum is an integer below 2^52, exact in FP64.
+__global__ void __launch_bounds__(kBlock) k_sel_scan(Params P, const GenState* st,
+                                                     const double* fit, double* cum_local,
+                                                     double* cum_block) {
+    __shared__ double lds[kWaves];
+    if (st->fit_constant) return;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const int sh = st->sel_shift;
+    double q = 0.0;
+    if (i < P.n) {
+        const double f = fit[i];
+        q = floor(det_ldexp(f > 0.0 ? f : 0.0, sh));
+    }
+    double total;
+    const double inc = block_inclusive_scan<double>(q, lds, total);
+    if (i < P.n) cum_local[i] = inc;
+    if (threadIdx.x == 0) cum_block[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_sel_blockscan(GenState* st, double* cum_block) {
+    __shared__ double s_wave[16];
+    __shared__ double s_carry;
+    if (st->fit_constant) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nb = st->sel_blocks;
+    if (tid == 0) s_carry = 0.0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t idx = base + tid;
+        double x = idx < nb ? cum_block[idx] : 0.0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(x, d, 64);
+            if (lane >= d) x = x + y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        double off = s_carry, tot = 0.0;
+        for (int q = 0; q < 16; ++q) {
+            if (q < wave) off = off + s_wave[q];
+            tot = tot + s_wave[q];
+        }
+        if (idx < nb) cum_block[idx] = off + x;
+        __syncthreads();
+        if (tid == 0) s_carry = s_carry + tot;
+        __syncthreads();
+    }
+    if (tid == 0) st->sel_total = (uint64_t)s_carry;
+}
+
+// Parent pick for one uniform: uniform over [0, N) when fitness is constant,
+// else min{i : target < C[i]} with target = floor(k * C[N-1] / 2^53).
+__device__ inline uint32_t pick_parent(const Params& P, const GenState* st, uint64_t k,
+                                       const double* cum_local, const double* cum_block) {
+    if (st->fit_constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n);
+    const double target = (double)__umul64hi(k << 11, st->sel_total);
+    // first block whose inclusive total exceeds target
+    uint32_t lo = 0, hi = st->sel_blocks;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cum_block[mid] > target)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const uint32_t b = lo;
+    const double base = b ? cum_block[b - 1] : 0.0;
+    const double rel = target - base;
+    uint32_t a = b * kBlock, e = min((b + 1) * kBlock, P.n);
+    while (a < e) {
+        const uint32_t mid = (a + e) >> 1;
+        if (cum_local[mid] > rel)
+            e = mid;
+        else
+            a = mid + 1;
+    }
+    return a;
+}
+
+// Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
+// count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
+// (common_crossover::operator(), fill_events), the top/bottom parent rows,
+// and the base of the incremental trait: trait(top) - fixed effects + new
+// mutation effects.
+__global__ void __launch_bounds__(kBlock) k_gametes(
+    Params P, const GenState* st, const double* cdf_rho, const double* cum_local,
+    const double* cum_block, const uint32_t* count, const uint32_t* local, const uint32_t* boff,
+    const uint32_t* free_list, const int32_t* effects, const int64_t* traits_parent,
+    int64_t* traits_child, uint32_t* top_out, uint32_t* bot_out, uint32_t* nev_out,
+    uint32_t* events) {
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= P.h) return;
+    const uint32_t gen = st->gen;
+    const uint32_t i = o >> 1;
+    const Draw4 pd = philox_draw(P.seed, gen, S_PARENTS, P.replicate, i, 0);
+    const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
+    const uint32_t p = pick_parent(P, st, k, cum_local, cum_block);
+    const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
+    const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
+    const bool swap = u53(g.x[2], g.x[3]) < P.bias;
+    const uint32_t top = swap ? 2 * p + 1 : 2 * p;
+    const uint32_t bot = swap ? 2 * p : 2 * p + 1;
+    top_out[o] = top;
+    bot_out[o] = bot;
+    nev_out[o] = ne;
+    uint32_t* ev = events + (size_t)o * P.ev_cap;
+    for (uint32_t e = 0; e < ne; e += 4) {
+        const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
+        for (uint32_t q = 0; q < 4; ++q) ev[e + q] = (e + q < ne) ? d.x[q] : 0u;
+    }
+    const uint32_t c = count[o];
+    const uint32_t k0 = local[o] + boff[o / kBlock];
+    const uint32_t F = st->free_count, A = st->alleles;
+    for (uint32_t t = 0; t < P.traits; ++t) {
+        int64_t v = traits_parent[(size_t)top * P.traits + t] - st->fixed_sum[t];
+        for (uint32_t kk = k0; kk < k0 + c; ++kk) {
+            const uint32_t slot = kk < F ? free_list[kk] : A + (kk - F);
+            if (slot < P.s_cap) v += effects[(size_t)slot * P.traits + t];
+        }
+        traits_child[(size_t)o * P.traits + t] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Recombination + mutation (+ free-space partials, + single-trait delta)
+// ---------------------------------------------------------------------------
+// Grid: x = 64-word column tiles, y = row chunks; 4 waves per block, one
+// 64-bit word per lane.  The tile's 4096 position keys sit in LDS transposed
+// ([bit][word]) so a lane's lookups hit its own bank.  For each child row:
+//   t, b = top/bottom parent words with fixed columns cleared
+//   mask = { het bit j : odd #{event key <= key_j} }   (block_crossover)
+//   child = (t & ~mask) | (b & mask), then XOR of this row's new mutations
+// OR/AND of the child words accumulate in registers (free_space_accumulator),
+// and for one trait the flipped het bits give the exact trait delta.
+__global__ void __launch_bounds__(kBlock) k_recombine(
+    Params P, const GenState* st, const uint64_t* __restrict__ parent,
+    uint64_t* __restrict__ child, const uint32_t* __restrict__ keys,
+    const int32_t* __restrict__ effects, const uint32_t* __restrict__ free_list,
+    const uint64_t* __restrict__ fixed_bits, const uint32_t* __restrict__ top_in,
+    const uint32_t* __restrict__ bot_in, const uint32_t* __restrict__ nev_in,
+    const uint32_t* __restrict__ events, const uint32_t* __restrict__ count,
+    const uint32_t* __restrict__ local, const uint32_t* __restrict__ boff,
+    int64_t* traits_child, uint64_t* var_part, uint64_t* fix_part, int trait_delta) {
+    __shared__ uint32_t s_keys[64 * 64];
+    __shared__ uint64_t s_or[kWaves][64];
+    __shared__ uint64_t s_and[kWaves][64];
+
+    const uint32_t wc = st->wc;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t w0 = tile * 64;
+    if (w0 >= wc) return;  // uniform for the block
+    const uint32_t wp = st->wp;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t w = w0 + lane;
+    const bool valid = w < wc;
+    const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
+
+    // tile keys: lane L of wave v loads bits v, v+4, ... of word w0+L
+    {
+        const uint32_t base = w * 64;
+        for (uint32_t j = wave; j < 64; j += kWaves) {
+            const uint32_t slot = base + j;
+            s_keys[j * 64 + lane] = (slot < A) ? keys[slot] : 0u;
+        }
+    }
+    const uint64_t keep = (valid && w < wp) ? ~fixed_bits[w] : ~(uint64_t)0;
+    __syncthreads();
+
+    const uint32_t rows_per_chunk = (P.h + gridDim.y - 1) / gridDim.y;
+    const uint32_t r0 = blockIdx.y * rows_per_chunk;
+    const uint32_t r1 = min(P.h, r0 + rows_per_chunk);
+    uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
+
+    for (uint32_t o = r0 + wave; o < r1; o += kWaves) {
+        const uint32_t top = wave_uniform(top_in[o]);
+        const uint32_t bot = wave_uniform(bot_in[o]);
+        const uint32_t ne = wave_uniform(nev_in[o]);
+        uint64_t t = 0, b = 0;
+        if (valid && w < wp) {
+            t = parent[(size_t)top * wp + w] & keep;
+            b = parent[(size_t)bot * wp + w] & keep;
+        }
+        uint64_t m = 0;
+        int64_t delta = 0;
+        if (ne) {
+            const uint32_t* ev = events + (size_t)o * P.ev_cap;
+            // events in scalar registers, groups of four; pads are key 0
+            // (always <= key) and their count's parity is removed at the end
+            uint32_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
+            uint32_t e4 = 0, e5 = 0, e6 = 0, e7 = 0;
+            const uint32_t groups = (ne + 3) >> 2;
+            if (groups > 1) {
+                e4 = ev[4];
+                e5 = ev[5];
+                e6 = ev[6];
+                e7 = ev[7];
+            }
+            const uint32_t padpar = (groups * 4 - ne) & 1u;
+            uint64_t hets = t ^ b;
+            while (hets) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(hets);
+                hets &= hets - 1;
+                const uint32_t key = s_keys[j * 64 + lane];
+                uint32_t cnt = (key >= e0) + (key >= e1) + (key >= e2) + (key >= e3);
+                if (groups > 1) cnt += (key >= e4) + (key >= e5) + (key >= e6) + (key >= e7);
+                for (uint32_t gq = 2; gq < groups; ++gq) {
+                    const uint32_t* eg = ev + 4 * gq;
+                    cnt += (key >= eg[0]) + (key >= eg[1]) + (key >= eg[2]) + (key >= eg[3]);
+                }
+                if ((cnt ^ padpar) & 1u) {
+                    m |= (uint64_t)1 << j;
+                    if (trait_delta) {
+                        const int32_t e = effects[(size_t)(w * 64 + j)];
+                        delta += ((b >> j) & 1) ? (int64_t)e : -(int64_t)e;
+                    }
+                }
+            }
+        }
+        uint64_t c = (t & ~m) | (b & m);
+        // new mutations of this row (free columns: zero in every parent row)
+        const uint32_t nm = wave_uniform(count[o]);
+        if (nm) {
+            const uint32_t k0 = wave_uniform(local[o] + boff[o / kBlock]);
+            for (uint32_t k = k0; k < k0 + nm; ++k) {
+                const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+                if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+            }
+        }
+        if (valid) child[(size_t)o * wc + w] = c;
+        acc_or |= c;
+        acc_and &= c;
+        if (trait_delta) {
+            // wave sum of the per-lane deltas; one atomic add per row and tile
+            int64_t s = delta;
+            for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+            if (lane == 0 && s != 0) atomicAdd((unsigned long long*)&traits_child[o],
+                                               (unsigned long long)s);
+        }
+    }
+    s_or[wave][lane] = acc_or;
+    s_and[wave][lane] = acc_and;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        uint64_t vo = s_or[0][lane], va = s_and[0][lane];
+        for (int q = 1; q < kWaves; ++q) {
+            vo |= s_or[q][lane];
+            va &= s_and[q][lane];
+        }
+        var_part[(size_t)blockIdx.y * P.w_cap + w] = vo;
+        fix_part[(size_t)blockIdx.y * P.w_cap + w] = va;
+    }
+}
+
+// Exact trait sums for T > 1 (or a full recompute): one wave per row, set-bit
+// walk of common_phenotype_accumulator with integer accumulation (any order
+// is exact for quantised effects).
+__global__ void __launch_bounds__(kBlock) k_traits_exact(Params P, const GenState* st,
+                                                         const uint64_t* rows,
+                                                         const int32_t* effects,
+                                                         int64_t* traits, uint32_t words) {
+    const uint32_t wv = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (wv >= P.h) return;
+    const uint32_t W = words ? words : st->wc;
+    const uint32_t A = words ? st->alleles : st->all_size;
+    const uint32_t T = P.traits;
+    int64_t acc[kMaxTraits];
+    for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t) acc[t] = 0;
+    const uint64_t* row = rows + (size_t)wv * W;
+    for (uint32_t w = lane; w < W; w += 64) {
+        uint64_t x = row[w];
+        while (x) {
+            const uint32_t j = w * 64 + (uint32_t)__builtin_ctzll(x);
+            x &= x - 1;
+            if (j >= A) continue;
+            for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t)
+                acc[t] += effects[(size_t)j * T + t];
+        }
+    }
+    for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t) {
+        int64_t s = acc[t];
+        for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+        if (lane == 0) traits[(size_t)wv * T + t] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Free space: reduce chunk partials per word, classify, ordered compaction
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_free_reduce(Params P, const GenState* st,
+                                                        const uint64_t* var_part,
+                                                        const uint64_t* fix_part,
+                                                        uint32_t chunks, uint64_t* fixed_bits,
+                                                        uint64_t* lost_bits, uint32_t* word_cnt,
+                                                        uint32_t* word_bsum,
+                                                        uint32_t word_blocks) {
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wc = st->wc, S = st->all_size;
+    uint32_t cf = 0, cl = 0;
+    uint64_t fx = 0, ls = 0;
+    if (w < wc) {
+        uint64_t vo = 0, va = ~(uint64_t)0;
+        for (uint32_t c = 0; c < chunks; ++c) {
+            vo |= var_part[(size_t)c * P.w_cap + w];
+            va &= fix_part[(size_t)c * P.w_cap + w];
+        }
+        const uint32_t lo = w * 64;
+        uint64_t validm = 0;
+        if (S > lo) validm = (S - lo >= 64) ? ~(uint64_t)0 : (((uint64_t)1 << (S - lo)) - 1);
+        fx = va & validm;
+        ls = ~vo & validm;
+        cf = (uint32_t)__popcll(fx);
+        cl = (uint32_t)__popcll(ls);
+        fixed_bits[w] = fx;
+        lost_bits[w] = ls;
+    }
+    uint32_t tf, tl;
+    const uint32_t incf = block_inclusive_scan<uint32_t>(cf, lds, tf);
+    const uint32_t incl = block_inclusive_scan<uint32_t>(cl, lds, tl);
+    if (w < wc) word_cnt[w] = ((incf - cf) & 0xFFFFu) | ((incl - cl) << 16);
+    if (threadIdx.x == 0) {
+        word_bsum[blockIdx.x] = tf;
+        word_bsum[word_blocks + blockIdx.x] = tl;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_free_compact(GenState* st, const uint64_t* fixed_bits,
+                                                         const uint64_t* lost_bits,
+                                                         const uint32_t* word_cnt,
+                                                         const uint32_t* word_bsum,
+                                                         uint32_t word_blocks,
+                                                         uint32_t* free_list) {
+    __shared__ uint32_t s_off[2];
+    __shared__ uint32_t s_red[2][kBlock];
+    const uint32_t wc = st->wc;
+    const uint32_t nb = (wc + kBlock - 1) / kBlock;
+    if (blockIdx.x >= nb) return;
+    // offsets of this block and the fixed total, from the block totals
+    uint32_t pf = 0, tfix = 0, pl = 0;
+    for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
+        const uint32_t a = word_bsum[q], l = word_bsum[word_blocks + q];
+        tfix += a;
+        if (q < blockIdx.x) {
+            pf += a;
+            pl += l;
+        }
+    }
+    s_red[0][threadIdx.x] = pf;
+    s_red[1][threadIdx.x] = pl;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+            s_red[1][threadIdx.x] += s_red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s_off[0] = s_red[0][0];
+        s_off[1] = s_red[1][0];
+    }
+    __syncthreads();
+    s_red[0][threadIdx.x] = tfix;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+        __syncthreads();
+    }
+    const uint32_t nfixed = s_red[0][0];
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    if (w < wc) {
+        const uint32_t wcnt = word_cnt[w];
+        uint32_t pos = s_off[0] + (wcnt & 0xFFFFu);
+        uint64_t fx = fixed_bits[w];
+        while (fx) {
+            free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(fx);
+            fx &= fx - 1;
+        }
+        pos = nfixed + s_off[1] + (wcnt >> 16);
+        uint64_t ls = lost_bits[w];
+        while (ls) {
+            free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(ls);
+            ls &= ls - 1;
+        }
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) {
+        uint32_t tl = 0;
+        for (uint32_t q = 0; q < nb; ++q) tl += word_bsum[word_blocks + q];
+        st->n_fixed = nfixed;
+        st->free_count = nfixed + tl;
+    }
+}
+
+// Column OR/AND straight from a resident population (after an upload).
+__global__ void __launch_bounds__(kBlock) k_column_reduce(Params P, const GenState* st,
+                                                          const uint64_t* rows,
+                                                          uint64_t* var_part,
+                                                          uint64_t* fix_part) {
+    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t wc = st->wc;
+    if (w >= wc) return;
+    const uint32_t rows_per_chunk = (P.h + gridDim.y - 1) / gridDim.y;
+    const uint32_t r0 = blockIdx.y * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
+    uint64_t vo = 0, va = ~(uint64_t)0;
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint64_t x = rows[(size_t)r * wc + w];
+        vo |= x;
+        va &= x;
+    }
+    var_part[(size_t)blockIdx.y * P.w_cap + w] = vo;
+    fix_part[(size_t)blockIdx.y * P.w_cap + w] = va;
+}
+
+// ---------------------------------------------------------------------------
+// Fitness: GeneralFitness over individuals, z = genome(2i) + genome(2i+1)
+// ---------------------------------------------------------------------------
+__device__ inline double fitness_of(const Params& P, const double* z) {
+    const double x = P.traits ? z[0] : 0.0;
+    switch (P.fitness_kind) {
+        case CLOTHO_FIT_GAUSSIAN: {
+            const double ex = (x * x) * P.fit_inv;
+            return P.fit_coeff * det_exp(-ex);
+        }
+        case CLOTHO_FIT_QUADRATIC: {
+            double r = x * P.fit_inv;
+            r = r * r;
+            return r >= 1.0 ? 0.0 : 1.0 - r;
+        }
+        case CLOTHO_FIT_GAUSSIAN_FIXED:
+            return det_exp(-((x * x) * P.fit_inv));
+        case CLOTHO_FIT_MV_STABILIZING: {
+            double ss = 0.0;
+            for (uint32_t t = 0; t < P.traits; ++t) ss = ss + z[t] * z[t];
+            return det_exp(-(ss * P.fit_inv));
+        }
+        default:
+            return 1.0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fitness(Params P, const GenState* st,
+                                                    const int64_t* traits, double* fit) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t gen = st->gen;
+    double z[kMaxTraits];
+    for (uint32_t t = 0; t < P.traits; ++t) {
+        const int64_t g = traits[(size_t)(2 * i) * P.traits + t] +
+                          traits[(size_t)(2 * i + 1) * P.traits + t];
+        z[t] = (double)g * P.eff_quantum;
+        if (P.env_sigma > 0.0) {
+            const Draw4 d = philox_draw(P.seed, gen, S_ENV, P.replicate, i, t);
+            z[t] = z[t] + P.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
+        }
+    }
+    fit[i] = fitness_of(P, z);
+}
+
+__global__ void k_commit(GenState* st) {
+    st->alleles = st->all_size;
+    st->wp = st->wc;
+    st->gen = st->gen + 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_any_nonzero(const int32_t* effects, size_t n,
+                                                        uint32_t* flag_out) {
+    __shared__ int s[kBlock];
+    int any = 0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kBlock)
+        any |= effects[i] != 0;
+    s[threadIdx.x] = any;
+    __syncthreads();
+    for (int q = kBlock / 2; q > 0; q >>= 1) {
+        if ((int)threadIdx.x < q) s[threadIdx.x] |= s[threadIdx.x + q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) flag_out[blockIdx.x] = s[0];
+}
+
+}  // namespace
+
+// ===========================================================================
+// launch sequences
+// ===========================================================================
+#define LAUNCH_CHECK()                              \
+    do {                                            \
+        hipError_t e_ = hipGetLastError();          \
+        if (e_ != hipSuccess) return e_;            \
+    } while (0)
+
+hipError_t launch_draw(clotho_ctx* c) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    const int par = c->cur;  // parent = current population
+    const int chi = 1 - c->cur;
+    const uint32_t sel_blocks = (P.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_mut_counts, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                       B.cdf_mu, B.mut_count, B.mut_local, B.mut_bsum);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fitness_stats, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P,
+                       B.fitness[par], B.fit_stats);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, c->stream, P, B.state, B.mut_bsum,
+                       B.fit_stats, B.free_list, B.effects);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mut_attrs, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                       B.mut_count, B.mut_local, B.mut_bsum, B.free_list, B.keys, B.effects,
+                       B.ages);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sel_scan, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                       B.fitness[par], B.cum_local, B.cum_block);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sel_blockscan, dim3(1), dim3(1024), 0, c->stream, B.state, B.cum_block);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gametes, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                       B.cdf_rho, B.cum_local, B.cum_block, B.mut_count, B.mut_local,
+                       B.mut_bsum, B.free_list, B.effects, B.traits[par], B.traits[chi], B.top,
+                       B.bottom, B.n_events, B.events);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_recombine(clotho_ctx* c) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    const int par = c->cur, chi = 1 - c->cur;
+    const int delta = (P.traits == 1) ? 1 : 0;
+    hipLaunchKernelGGL(k_recombine, dim3(P.tiles, P.chunks), dim3(kBlock), 0, c->stream, P,
+                       B.state, B.rows[par], B.rows[chi], B.keys, B.effects, B.free_list,
+                       B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
+                       B.mut_local, B.mut_bsum, B.traits[chi], B.var_part, B.fix_part, delta);
+    LAUNCH_CHECK();
+    if (P.traits > 1) {
+        const uint32_t blocks = (P.h + kWaves - 1) / kWaves;
+        hipLaunchKernelGGL(k_traits_exact, dim3(blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                           B.rows[chi], B.effects, B.traits[chi], 0u);
+        LAUNCH_CHECK();
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_free_space(clotho_ctx* c) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    uint64_t* lost_bits = B.var_part + (size_t)P.chunks * P.w_cap;  // spare row of partials
+    hipLaunchKernelGGL(k_free_reduce, dim3(P.word_blocks), dim3(kBlock), 0, c->stream, P,
+                       B.state, B.var_part, B.fix_part, P.chunks, B.fixed_bits, lost_bits,
+                       B.word_cnt, B.word_bsum, P.word_blocks);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_free_compact, dim3(P.word_blocks), dim3(kBlock), 0, c->stream, B.state,
+                       B.fixed_bits, lost_bits, B.word_cnt, B.word_bsum, P.word_blocks,
+                       B.free_list);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_fitness(clotho_ctx* c) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    const int chi = 1 - c->cur;
+    const uint32_t sel_blocks = (P.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_fitness, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                       B.traits[chi], B.fitness[chi]);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_commit(clotho_ctx* c) {
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, c->stream, c->b.state);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+// After an upload into population [cur]: traits, free-space classification.
+hipError_t launch_population_analysis(clotho_ctx* c) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    const int cur = c->cur;
+    if (P.traits) {
+        const uint32_t blocks = (P.h + kWaves - 1) / kWaves;
+        hipLaunchKernelGGL(k_traits_exact, dim3(blocks), dim3(kBlock), 0, c->stream, P, B.state,
+                           B.rows[cur], B.effects, B.traits[cur], 0u);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_column_reduce, dim3(P.word_blocks, P.chunks), dim3(kBlock), 0,
+                       c->stream, P, B.state, B.rows[cur], B.var_part, B.fix_part);
+    LAUNCH_CHECK();
+    return launch_free_space(c);
+}
+
+hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {
+    const size_t n = (size_t)c->p.s_cap * c->p.traits;
+    hipLaunchKernelGGL(k_any_nonzero, dim3(64), dim3(kBlock), 0, c->stream, c->b.effects, n,
+                       d_flag);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+}  // namespace clotho

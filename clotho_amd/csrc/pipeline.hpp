@@ -1,0 +1,128 @@
+// pipeline.hpp — device-resident state and launch interface of the
+// per-generation offspring pipeline (one population pair per context).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/clotho_amd.h"
+
+namespace clotho {
+
+constexpr int kMaxCdf = 128;      // Poisson inversion table capacity
+constexpr int kMaxTraits = 32;    // trait capacity of the fused paths
+constexpr int kScanBlock = 256;   // items per block of every two-level scan
+
+// Scalars of the generation, kept in device memory so that a captured
+// hipGraph replays successive generations without host involvement.
+struct GenState {
+    uint32_t gen;          // stream generation counter (reference m_generation)
+    uint32_t n;            // individuals N
+    uint32_t alleles;      // allele-table size of the parent population (A)
+    uint32_t wp;           // parent words per row
+    uint32_t free_count;   // F = fixed + lost of the parent
+    uint32_t n_fixed;      // fixed part of the free list
+    uint32_t m;            // new mutations this generation
+    uint32_t all_size;     // allele-table size of the child
+    uint32_t wc;           // child words per row
+    uint32_t fit_constant; // parent fitness all equal -> uniform selection
+    int32_t sel_shift;     // fitness quantisation exponent
+    uint32_t error;        // sticky: 1 = allele capacity exceeded
+    uint32_t error_gen;    // generation of the first error
+    uint32_t sel_blocks;   // number of fitness scan blocks
+    uint64_t sel_total;    // sum of quantised fitness
+    int64_t fixed_sum[kMaxTraits];  // sum of quantised effects over the fixed columns
+};
+
+// Kernel-side constants that do not change between generations.
+struct Params {
+    uint64_t seed;
+    uint32_t replicate;
+    uint32_t n;            // individuals
+    uint32_t h;            // haplotypes = 2n
+    uint32_t traits;       // T
+    uint32_t s_cap;        // allele capacity
+    uint32_t w_cap;        // word capacity per row
+    uint32_t ev_cap;       // event slots per gamete
+    uint32_t n_cdf_rho, n_cdf_mu;
+    double bias, neutral_p, trait_mean, trait_sigma, env_sigma;
+    double eff_scale;      // 2^-q
+    double eff_quantum;    // 2^q
+    uint32_t fitness_kind;
+    double fit_coeff, fit_inv;
+    uint32_t chunks;       // row chunks of the recombination grid
+    uint32_t tiles;        // 64-word column tiles at capacity
+    uint32_t mut_blocks;   // scan blocks over gametes
+    uint32_t word_blocks;  // scan blocks over words at capacity
+};
+
+// Device buffers.  Index [c] is the population that is current after the
+// last generation ("child"); the other one receives the next generation.
+struct Buffers {
+    uint64_t* rows[2];        // [h][w] dense, row stride = that population's W
+    int64_t* traits[2];       // [h][T] quantised genome trait sums
+    double* fitness[2];       // [n]
+    uint32_t* keys;           // [s_cap] position * 2^32
+    int32_t* effects;         // [s_cap][T] quantised effects
+    uint32_t* ages;           // [s_cap]
+    uint32_t* free_list;      // [s_cap] fixed ascending then lost ascending
+    uint64_t* fixed_bits;     // [w_cap] AND of all rows, idx < all_size
+    uint32_t* mut_count;      // [h]
+    uint32_t* mut_local;      // [h] exclusive scan inside a block
+    uint32_t* mut_bsum;       // [mut_blocks] block totals -> exclusive offsets
+    uint32_t* top;            // [h] parent row used as top strand
+    uint32_t* bottom;         // [h]
+    uint32_t* n_events;       // [h]
+    uint32_t* events;         // [h][ev_cap] event keys
+    uint64_t* var_part;       // [chunks][w_cap] OR partials
+    uint64_t* fix_part;       // [chunks][w_cap] AND partials
+    uint32_t* word_cnt;       // [w_cap] (fixed count << 16 | lost count) per word
+    uint32_t* word_bsum;      // [2*word_blocks] block totals of fixed / lost counts
+    double* cum_local;        // [n] quantised fitness inclusive scan inside a block
+    double* cum_block;        // [sel blocks] inclusive scan of block totals
+    double* fit_stats;        // [2*sel blocks] per-block max, all-equal flag
+    double* cdf_rho;          // [kMaxCdf]
+    double* cdf_mu;           // [kMaxCdf]
+    GenState* state;
+};
+
+struct Timing {
+    bool enabled = false;
+    hipEvent_t ev[6] = {};
+    double ms[5] = {0, 0, 0, 0, 0};
+    int generations = 0;
+};
+
+struct GraphPair {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool valid = false;
+};
+
+}  // namespace clotho
+
+struct clotho_ctx {
+    clotho_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    clotho::Params p;
+    clotho::Buffers b;
+    int cur = 0;                // index of the current (child) population
+    bool has_population = false;
+    clotho::Timing timing;
+    clotho::GraphPair graphs;
+    std::string err;
+    hipDeviceProp_t prop;
+};
+
+namespace clotho {
+// Launch sequences (pipeline.hip).
+hipError_t launch_draw(clotho_ctx* c);
+hipError_t launch_recombine(clotho_ctx* c);
+hipError_t launch_free_space(clotho_ctx* c);
+hipError_t launch_fitness(clotho_ctx* c);
+hipError_t launch_commit(clotho_ctx* c);
+hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
+hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
+}  // namespace clotho

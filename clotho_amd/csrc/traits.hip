@@ -1,0 +1,137 @@
+// traits.hip — genotype x effect contraction over a resident population:
+// z[h][t] = sum_j bit(h, j) * effect[j][t], the population-wide form of
+// common_phenotype_accumulator::operator()
+// (include/clotho/data_spaces/phenotype_evaluator/common_phenotype_accumulator_row_block_alignment.hpp:47-66).
+//
+// Modes: PLAIN_FP32 walks the set bits and sums FP32 effects (the plain
+// popcount-weighted baseline); EXACT_FP64 does the same walk in FP64;
+// MFMA_BF16 expands bits to BF16 and runs the contraction on matrix cores
+// with effects split into BF16 terms (traits_mfma.hip).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/clotho_amd.h"
+#include "traits.hpp"
+
+namespace clotho {
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename Acc, int TMAX>
+__global__ void __launch_bounds__(kBlock) k_traits_walk(const uint64_t* __restrict__ rows,
+                                                        uint32_t h, uint32_t w,
+                                                        const Acc* __restrict__ eff,
+                                                        uint32_t sites, uint32_t traits,
+                                                        Acc* __restrict__ z) {
+    const uint32_t row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (row >= h) return;
+    Acc acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = 0;
+    const uint64_t* r = rows + (size_t)row * w;
+    for (uint32_t k = lane; k < w; k += 64) {
+        uint64_t x = r[k];
+        while (x) {
+            const uint32_t j = k * 64 + (uint32_t)__builtin_ctzll(x);
+            x &= x - 1;
+            if (j >= sites) break;
+            const Acc* e = eff + (size_t)j * traits;
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t)
+                if (t < (int)traits) acc[t] += e[t];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) {
+        if (t >= (int)traits) break;
+        Acc s = acc[t];
+        for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+        if (lane == 0) z[(size_t)row * traits + t] = s;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_traits_walk_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
+                                   const float* eff, uint32_t sites, uint32_t traits, float* z,
+                                   hipStream_t s) {
+    const uint32_t blocks = (h + 3) / 4;
+    if (traits <= 1)
+        hipLaunchKernelGGL((k_traits_walk<float, 1>), dim3(blocks), dim3(kBlock), 0, s, rows, h, w,
+                           eff, sites, traits, z);
+    else if (traits <= 4)
+        hipLaunchKernelGGL((k_traits_walk<float, 4>), dim3(blocks), dim3(kBlock), 0, s, rows, h, w,
+                           eff, sites, traits, z);
+    else
+        hipLaunchKernelGGL((k_traits_walk<float, 16>), dim3(blocks), dim3(kBlock), 0, s, rows, h,
+                           w, eff, sites, traits, z);
+    return hipGetLastError();
+}
+
+hipError_t launch_traits_walk_fp64(const uint64_t* rows, uint32_t h, uint32_t w,
+                                   const double* eff, uint32_t sites, uint32_t traits, double* z,
+                                   hipStream_t s) {
+    const uint32_t blocks = (h + 3) / 4;
+    if (traits <= 1)
+        hipLaunchKernelGGL((k_traits_walk<double, 1>), dim3(blocks), dim3(kBlock), 0, s, rows, h,
+                           w, eff, sites, traits, z);
+    else
+        hipLaunchKernelGGL((k_traits_walk<double, 16>), dim3(blocks), dim3(kBlock), 0, s, rows, h,
+                           w, eff, sites, traits, z);
+    return hipGetLastError();
+}
+
+}  // namespace clotho
+
+using namespace clotho;
+
+extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t w,
+                                     const double* effects, uint32_t sites, uint32_t traits,
+                                     int mode, int split_terms, int repeats, double* z,
+                                     double* avg_ms, int device) {
+    if (!rows || !effects || !z || h == 0 || w == 0 || traits == 0 || traits > 16 ||
+        (size_t)w * 64 < sites || repeats < 1)
+        return CLOTHO_ERR_ARG;
+    if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
+        mode != CLOTHO_TRAITS_MFMA_BF16)
+        return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
+        return CLOTHO_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CLOTHO_ERR_HIP;
+    TraitWorkspace ws;
+    int rc = ws.init(rows, h, w, effects, sites, traits, mode, split_terms, s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double ms_total = 0.0;
+    if (rc == CLOTHO_OK) {
+        // one untimed launch, then `repeats` timed launches
+        if (ws.run(s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = CLOTHO_ERR_HIP;
+    }
+    if (rc == CLOTHO_OK) {
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            rc = CLOTHO_ERR_HIP;
+    }
+    if (rc == CLOTHO_OK) {
+        hipEventRecord(e0, s);
+        for (int r = 0; r < repeats && rc == CLOTHO_OK; ++r)
+            if (ws.run(s) != hipSuccess) rc = CLOTHO_ERR_HIP;
+        hipEventRecord(e1, s);
+        if (rc == CLOTHO_OK && hipEventSynchronize(e1) != hipSuccess) rc = CLOTHO_ERR_HIP;
+        float ms = 0.f;
+        if (rc == CLOTHO_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = CLOTHO_ERR_HIP;
+        ms_total = ms;
+    }
+    if (rc == CLOTHO_OK) rc = ws.download(z, s);
+    if (avg_ms) *avg_ms = ms_total / repeats;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    ws.release();
+    hipStreamDestroy(s);
+    return rc;
+}

@@ -1,0 +1,115 @@
+// traits.hpp — device workspace of the standalone trait contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/clotho_amd.h"
+
+namespace clotho {
+
+hipError_t launch_traits_walk_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
+                                   const float* eff, uint32_t sites, uint32_t traits, float* z,
+                                   hipStream_t s);
+hipError_t launch_traits_walk_fp64(const uint64_t* rows, uint32_t h, uint32_t w,
+                                   const double* eff, uint32_t sites, uint32_t traits, double* z,
+                                   hipStream_t s);
+// MFMA contraction (traits_mfma.hip): effects pre-split into `terms` BF16
+// matrices laid out for the kernel; z is FP32 partials reduced in FP64.
+size_t mfma_effect_elems(uint32_t sites, uint32_t traits, int terms);
+void mfma_pack_effects(const double* effects, uint32_t sites, uint32_t traits, int terms,
+                       uint16_t* packed);
+size_t mfma_partial_elems(uint32_t h, uint32_t w, uint32_t traits);
+hipError_t launch_traits_mfma(const uint64_t* rows, uint32_t h, uint32_t w,
+                              const uint16_t* packed, uint32_t sites, uint32_t traits, int terms,
+                              float* partial, double* z, hipStream_t s);
+
+struct TraitWorkspace {
+    int mode = 0, terms = 0;
+    uint32_t h = 0, w = 0, sites = 0, traits = 0;
+    uint64_t* d_rows = nullptr;
+    float* d_eff32 = nullptr;
+    double* d_eff64 = nullptr;
+    uint16_t* d_packed = nullptr;
+    float* d_z32 = nullptr;
+    double* d_z64 = nullptr;
+    float* d_partial = nullptr;
+
+    int init(const uint64_t* rows, uint32_t h_, uint32_t w_, const double* eff, uint32_t s_,
+             uint32_t t_, int mode_, int terms_, hipStream_t st) {
+        mode = mode_;
+        terms = terms_;
+        h = h_;
+        w = w_;
+        sites = s_;
+        traits = t_;
+        if (hipMalloc((void**)&d_rows, sizeof(uint64_t) * h * w) != hipSuccess)
+            return CLOTHO_ERR_HIP;
+        if (hipMemcpyAsync(d_rows, rows, sizeof(uint64_t) * h * w, hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+            return CLOTHO_ERR_HIP;
+        if (mode == CLOTHO_TRAITS_PLAIN_FP32) {
+            std::vector<float> e((size_t)sites * traits);
+            for (size_t i = 0; i < e.size(); ++i) e[i] = (float)eff[i];
+            if (hipMalloc((void**)&d_eff32, 4 * e.size()) != hipSuccess ||
+                hipMalloc((void**)&d_z32, 4 * (size_t)h * traits) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            if (hipMemcpy(d_eff32, e.data(), 4 * e.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+        } else if (mode == CLOTHO_TRAITS_EXACT_FP64) {
+            if (hipMalloc((void**)&d_eff64, 8 * (size_t)sites * traits) != hipSuccess ||
+                hipMalloc((void**)&d_z64, 8 * (size_t)h * traits) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            if (hipMemcpy(d_eff64, eff, 8 * (size_t)sites * traits, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return CLOTHO_ERR_HIP;
+        } else {
+            const size_t ne = mfma_effect_elems(sites, traits, terms);
+            std::vector<uint16_t> packed(ne);
+            mfma_pack_effects(eff, sites, traits, terms, packed.data());
+            if (hipMalloc((void**)&d_packed, 2 * ne) != hipSuccess ||
+                hipMalloc((void**)&d_partial, 4 * mfma_partial_elems(h, w, traits)) !=
+                    hipSuccess ||
+                hipMalloc((void**)&d_z64, 8 * (size_t)h * traits) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            if (hipMemcpy(d_packed, packed.data(), 2 * ne, hipMemcpyHostToDevice) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+        }
+        return hipStreamSynchronize(st) == hipSuccess ? CLOTHO_OK : CLOTHO_ERR_HIP;
+    }
+
+    hipError_t run(hipStream_t st) {
+        if (mode == CLOTHO_TRAITS_PLAIN_FP32)
+            return launch_traits_walk_fp32(d_rows, h, w, d_eff32, sites, traits, d_z32, st);
+        if (mode == CLOTHO_TRAITS_EXACT_FP64)
+            return launch_traits_walk_fp64(d_rows, h, w, d_eff64, sites, traits, d_z64, st);
+        return launch_traits_mfma(d_rows, h, w, d_packed, sites, traits, terms, d_partial, d_z64,
+                                  st);
+    }
+
+    int download(double* z, hipStream_t st) {
+        const size_t n = (size_t)h * traits;
+        if (hipStreamSynchronize(st) != hipSuccess) return CLOTHO_ERR_HIP;
+        if (mode == CLOTHO_TRAITS_PLAIN_FP32) {
+            std::vector<float> tmp(n);
+            if (hipMemcpy(tmp.data(), d_z32, 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            for (size_t i = 0; i < n; ++i) z[i] = tmp[i];
+        } else {
+            if (hipMemcpy(z, d_z64, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+        }
+        return CLOTHO_OK;
+    }
+
+    void release() {
+        void* ptrs[] = {d_rows, d_eff32, d_eff64, d_packed, d_z32, d_z64, d_partial};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace clotho

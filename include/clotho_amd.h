@@ -1,0 +1,181 @@
+/*
+ * clotho_amd.h — C ABI of the MI355X offspring pipeline.
+ *
+ * One context owns one population pair (parent/child), its allele table, the
+ * free-space classification and the fitness vector, all resident in HBM of one
+ * GPU.  Every entry point is plain C: status codes (0 = ok, <0 = error, text in
+ * clotho_last_error), caller-owned host buffers, context-owned device buffers,
+ * work ordered on the context's single HIP stream, one host thread per context.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference root; DS/ = include/clotho/data_spaces/, QS/ = examples/QTLSim/):
+ *
+ *   clotho_create / clotho_destroy
+ *       Engine<..., parallel_pipeline>::Engine / ~Engine / init
+ *       (QS/engine_parallel_pipeline.hpp:109-170, 306-323)
+ *   clotho_generation / clotho_run
+ *       Engine<..., parallel_pipeline>::simulate
+ *       (QS/engine_parallel_pipeline.hpp:172-239)
+ *   clotho_phase_draw
+ *       mutation_allocator::allocate + generate_child_mutations +
+ *       make_selection_generator + the per-gamete draws of common_crossover
+ *       (DS/mutation/mutation_allocator.hpp:41-47;
+ *        QS/engine_parallel_pipeline.hpp:327-367;
+ *        DS/selection/selection_generator_factory.hpp:33-67;
+ *        DS/crossover/common_crossover_row_block_alignment.hpp:53-69)
+ *   clotho_phase_recombine_mutate
+ *       removeFixedAlleles + offspring_generator::{crossover,mutate,phenotype,fixed}
+ *       (QS/engine_parallel_pipeline.hpp:409-426;
+ *        DS/offspring_generator/offspring_generator_population_space_row_alignment.hpp:84-260;
+ *        DS/crossover/block_crossover.hpp:46-63)
+ *   clotho_phase_free_space
+ *       free_space_accumulator_mt::finalize -> free_space_details::analyze_free_indices
+ *       (DS/free_space/free_space_accumulator.hpp:142-151;
+ *        DS/free_space/free_space_details.hpp:359-408)
+ *   clotho_phase_fitness
+ *       GeneralFitness::operator()(population_space<row_block_alignment>*, s, e)
+ *       (DS/fitness/general_fitness.hpp:148-177) with the ifitness metrics of
+ *       src/clotho/fitness/{normal,quadratic,constant}_fitness_metric.cpp
+ *   clotho_upload_* / clotho_download_*
+ *       population_space::begin_genome/end_genome, AlleleSpace::getPositions,
+ *       trait_space_vector, free_space_details::{fixed,lost}_{begin,end},
+ *       GeneralFitness::begin/end (state access used by the reference's
+ *       state_getter, QS/engine_parallel_pipeline.hpp:488-521)
+ *   clotho_trait_contract
+ *       common_phenotype_accumulator::operator() over a whole population
+ *       (DS/phenotype_evaluator/common_phenotype_accumulator_row_block_alignment.hpp:47-66)
+ *       in three arithmetic modes.
+ */
+#ifndef CLOTHO_AMD_H_
+#define CLOTHO_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLOTHO_OK 0
+#define CLOTHO_ERR_ARG (-1)      /* invalid argument or configuration            */
+#define CLOTHO_ERR_HIP (-2)      /* a HIP runtime call failed                    */
+#define CLOTHO_ERR_CAPACITY (-3) /* population or allele table exceeds capacity  */
+#define CLOTHO_ERR_STATE (-4)    /* call out of order (e.g. no population)      */
+
+/* Fitness metric kinds (fitness.name of the reference's fitness_toolkit). */
+#define CLOTHO_FIT_CONSTANT 0       /* "constant": 1.0                                   */
+#define CLOTHO_FIT_GAUSSIAN 1       /* "gaussian": (1/(s*sqrt(2pi))) exp(-x0^2/(2s^2)),
+                                       s = sqrt(4 N mu), first trait only                */
+#define CLOTHO_FIT_QUADRATIC 2      /* "quadratic": max(0, 1-(x0/s)^2), s = scale*sqrt(4 N mu) */
+#define CLOTHO_FIT_GAUSSIAN_FIXED 3 /* exp(-x0^2/(2 scale^2)); no coefficient            */
+#define CLOTHO_FIT_MV_STABILIZING 4 /* exp(-|x|^2/(2 scale^2)) over all traits           */
+
+/* Trait contraction arithmetic for clotho_trait_contract. */
+#define CLOTHO_TRAITS_PLAIN_FP32 0 /* popcount-walk over set bits, FP32 sums            */
+#define CLOTHO_TRAITS_MFMA_BF16 1  /* bits -> BF16, effects split into BF16 terms, MFMA */
+#define CLOTHO_TRAITS_EXACT_FP64 2 /* set-bit walk, FP64 sums in ascending bit order    */
+
+typedef struct clotho_config {
+    uint64_t seed;            /* random_number.seed (Philox key)                         */
+    uint32_t replicate;       /* replicate id, part of every stream counter              */
+    uint32_t n_individuals;   /* N, diploid individuals (constant population size)       */
+    uint32_t trait_count;     /* traits.N                                                */
+    uint32_t fitness_kind;    /* CLOTHO_FIT_*                                            */
+    double mu;                /* mutation.mu, expected new mutations per gamete          */
+    double rho;               /* recombination.rho, expected crossovers per gamete       */
+    double bias;              /* recombination.sequence_bias.p, strand-swap probability  */
+    double neutral_p;         /* neutral.p, probability that an effect is 0              */
+    double trait_mean;        /* traits.mean                                             */
+    double trait_sigma;       /* traits.sigma                                            */
+    double fitness_scale;     /* quadratic scale, or sigma of the fixed Gaussians        */
+    double env_sigma;         /* s.d. of environmental noise added per individual (0=off)*/
+    int32_t effect_quantum_log2; /* effects are integer multiples of 2^q (e.g. -24)      */
+    uint32_t max_alleles;     /* allele-slot capacity S_cap                              */
+    uint32_t reserved[4];
+} clotho_config;
+
+/* Shape and bookkeeping of the current (child) population after a generation. */
+typedef struct clotho_shape {
+    uint32_t generation;      /* generations completed + 1 (reference's m_generation)   */
+    uint32_t n_individuals;   /* N                                                      */
+    uint32_t haplotypes;      /* H = 2N rows                                            */
+    uint32_t words;           /* W = ceil(alleles/512)*8 uint64 words per row           */
+    uint32_t alleles;         /* allele-table size (getMaxAlleles)                      */
+    uint32_t new_mutations;   /* M of the last generation                               */
+    uint32_t fixed_count;     /* free_space_details::fixed_size                         */
+    uint32_t lost_count;      /* free_space_details::lost_size                          */
+    uint32_t free_count;      /* fixed + lost                                           */
+    uint32_t variable_count;  /* alleles - free                                          */
+    uint32_t all_neutral;     /* trait_space_vector::isAllNeutral                       */
+    uint32_t fitness_constant;/* every fitness value equal (selection is uniform)       */
+    uint32_t error;           /* sticky device-side error flag (capacity)               */
+    uint32_t reserved[3];
+} clotho_shape;
+
+typedef struct clotho_ctx clotho_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out);
+void clotho_destroy(clotho_ctx* ctx);
+const char* clotho_last_error(const clotho_ctx* ctx);
+const char* clotho_version(void);
+
+/* Poisson inversion table used for crossover and mutation counts: cdf[k] =
+ * P(X <= k), last entry forced to 1.0.  Returns the table length (<= cap) or <0. */
+int clotho_poisson_table(double lambda, double* cdf, int cap);
+
+/* ---- state upload / download (host buffers, synchronous) --------------- */
+/* Founders: rows[H][W] with H = 2N, W a multiple of 8; positions[A] in [0,1)
+ * (multiples of 2^-32); effects[A][T] (multiples of 2^q); traits/fitness may
+ * be NULL (then computed on the device; fitness NULL means constant 1.0). */
+int clotho_set_population(clotho_ctx* ctx, const uint64_t* rows, uint32_t words,
+                          const double* positions, const double* effects,
+                          uint32_t alleles, const double* fitness);
+int clotho_get_shape(clotho_ctx* ctx, clotho_shape* out);
+int clotho_download_population(clotho_ctx* ctx, uint64_t* rows, size_t cap_words);
+int clotho_download_alleles(clotho_ctx* ctx, double* positions, double* effects,
+                            uint32_t* ages, size_t cap_alleles);
+int clotho_download_free_space(clotho_ctx* ctx, uint32_t* fixed, uint32_t* lost,
+                               size_t cap);
+int clotho_download_traits(clotho_ctx* ctx, double* haplotype_traits, size_t cap);
+int clotho_download_fitness(clotho_ctx* ctx, double* fitness, size_t cap);
+/* Per offspring gamete o of the last generation: parent haplotype row that was
+ * the top strand, crossover event count, mutation count (any pointer may be NULL). */
+int clotho_download_gametes(clotho_ctx* ctx, uint32_t* top_row, uint32_t* bottom_row,
+                            uint32_t* events, uint32_t* mutations, size_t cap);
+
+/* ---- one generation ---------------------------------------------------- */
+/* Enqueue n generations on the context stream and wait; returns the sticky
+ * device error as CLOTHO_ERR_CAPACITY if one occurred.  use_graph != 0
+ * replays a captured hipGraph of the generation instead of launching kernels. */
+int clotho_run(clotho_ctx* ctx, uint32_t n_generations, int use_graph);
+int clotho_generation(clotho_ctx* ctx);
+
+/* Phase-level entry points, in generation order (clotho_generation = these). */
+int clotho_phase_draw(clotho_ctx* ctx);
+int clotho_phase_recombine_mutate(clotho_ctx* ctx);
+int clotho_phase_free_space(clotho_ctx* ctx);
+int clotho_phase_fitness(clotho_ctx* ctx);
+int clotho_phase_commit(clotho_ctx* ctx);
+int clotho_sync(clotho_ctx* ctx);
+
+/* Kernel timing over the last clotho_run: milliseconds summed over the
+ * generations for [0] whole generations, [1] draw, [2] recombine+mutate,
+ * [3] free space, [4] fitness; returns the number of generations timed. */
+int clotho_timings(clotho_ctx* ctx, double* ms, int cap);
+void clotho_set_timing(clotho_ctx* ctx, int enabled);
+
+/* ---- standalone trait contraction -------------------------------------- */
+/* z[h][t] = sum_j bit(h,j) * effects[j][t] over a device-resident genotype
+ * matrix.  Host-buffer variant: uploads, runs `repeats` times, downloads z
+ * (FP64 for every mode), reports the average kernel milliseconds. */
+int clotho_trait_contract(const uint64_t* rows, uint32_t haplotypes, uint32_t words,
+                          const double* effects, uint32_t sites, uint32_t traits,
+                          int mode, int split_terms, int repeats, double* z,
+                          double* avg_ms, int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOTHO_AMD_H_ */

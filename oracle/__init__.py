@@ -1,0 +1,271 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the CPU oracle.
+
+The oracle (oracle.cpp) restates the reference's per-generation offspring
+pipeline on the CPU over the same counter-based streams the GPU library draws.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "oracle.cpp")
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall"]
+
+
+def build(force: bool = True) -> str:
+    """Compile oracle.cpp with g++ into _build/liboracle.so."""
+    if not force and os.path.exists(LIB):
+        return LIB
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    tmp = LIB + ".tmp"
+    subprocess.check_call(["g++", *CXXFLAGS, "-o", tmp, SRC, "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+    L.or_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                          C.c_uint32, u32p]
+    for name in ("or_det_log", "or_det_exp", "or_det_sqrt", "or_det_recip", "or_normal_ppf"):
+        getattr(L, name).argtypes = [C.c_double]
+        getattr(L, name).restype = C.c_double
+    L.or_poisson_table.argtypes = [C.c_double, f64p, C.c_int]
+    L.or_poisson_table.restype = C.c_int
+    L.or_crossover.argtypes = [u64p, u64p, C.c_uint32, u64p, C.c_uint32, f64p, C.c_uint32,
+                               f64p, C.c_uint32]
+    L.or_free_space.argtypes = [u64p, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
+                                C.POINTER(C.c_uint32), u32p, C.POINTER(C.c_uint32)]
+    L.or_phenotype.argtypes = [u64p, C.c_uint32, C.c_uint32, f64p, C.c_uint32, C.c_uint32,
+                               f64p]
+    L.or_select.argtypes = [f64p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, u32p]
+    L.or_fitness_value.argtypes = [C.c_void_p, C.c_uint32, f64p]
+    L.or_fitness_value.restype = C.c_double
+    L.or_sim_create.argtypes = [C.c_void_p]
+    L.or_sim_create.restype = C.c_void_p
+    L.or_sim_destroy.argtypes = [C.c_void_p]
+    L.or_sim_set_population.argtypes = [C.c_void_p, u64p, C.c_uint32, f64p, f64p, C.c_uint32,
+                                        C.c_void_p]
+    L.or_sim_set_population.restype = C.c_int
+    L.or_sim_set_threads.argtypes = [C.c_void_p, C.c_int]
+    L.or_sim_step.argtypes = [C.c_void_p]
+    L.or_sim_step.restype = C.c_int
+    L.or_sim_shape.argtypes = [C.c_void_p, C.c_void_p]
+    L.or_sim_rows.argtypes = [C.c_void_p, u64p]
+    L.or_sim_alleles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_sim_free_space.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_sim_traits.argtypes = [C.c_void_p, f64p]
+    L.or_sim_fitness.argtypes = [C.c_void_p, f64p]
+    L.or_sim_gametes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_sim_phase_seconds.argtypes = [C.c_void_p, f64p]
+    _lib = L
+    return L
+
+
+# ---------------------------------------------------------------- scalar API
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def draw(seed, generation, stream, replicate, index, sub):
+    out = np.zeros(4, np.uint32)
+    lib().or_draw(seed, generation, stream, replicate, index, sub, out)
+    return out
+
+
+def det_log(x):
+    return lib().or_det_log(float(x))
+
+
+def det_exp(x):
+    return lib().or_det_exp(float(x))
+
+
+def det_sqrt(x):
+    return lib().or_det_sqrt(float(x))
+
+
+def det_recip(x):
+    return lib().or_det_recip(float(x))
+
+
+def normal_ppf(p):
+    return lib().or_normal_ppf(float(p))
+
+
+def poisson_table(lam, cap=128):
+    out = np.zeros(cap, np.float64)
+    n = lib().or_poisson_table(float(lam), out, cap)
+    if n < 0:
+        raise ValueError("poisson table: lambda out of range")
+    return out[:n].copy()
+
+
+def crossover(top, bottom, wc, positions, events):
+    top = np.ascontiguousarray(top, np.uint64)
+    bottom = np.ascontiguousarray(bottom, np.uint64)
+    positions = np.ascontiguousarray(positions, np.float64)
+    events = np.ascontiguousarray(events, np.float64)
+    child = np.zeros(wc, np.uint64)
+    ev = events if events.size else np.zeros(1)
+    lib().or_crossover(top, bottom, top.size, child, wc, positions, positions.size, ev,
+                       events.size)
+    return child
+
+
+def free_space(rows, max_alleles):
+    rows = np.ascontiguousarray(rows, np.uint64)
+    h, w = rows.shape
+    fixed = np.zeros(w * 64 + 1, np.uint32)
+    lost = np.zeros(w * 64 + 1, np.uint32)
+    nf, nl = C.c_uint32(), C.c_uint32()
+    lib().or_free_space(rows, h, w, max_alleles, fixed, C.byref(nf), lost, C.byref(nl))
+    return fixed[: nf.value].copy(), lost[: nl.value].copy()
+
+
+def phenotype(rows, effects):
+    rows = np.ascontiguousarray(rows, np.uint64)
+    effects = np.ascontiguousarray(effects, np.float64)
+    h, w = rows.shape
+    a, t = effects.shape
+    out = np.zeros((h, t), np.float64)
+    lib().or_phenotype(rows, h, w, effects, a, t, out)
+    return out
+
+
+def select(fitness, seed, generation, replicate=0):
+    fitness = np.ascontiguousarray(fitness, np.float64)
+    out = np.zeros(2 * fitness.size, np.uint32)
+    lib().or_select(fitness, fitness.size, seed, generation, replicate, out)
+    return out
+
+
+def fitness_value(cfg, n, z):
+    z = np.ascontiguousarray(z, np.float64)
+    return lib().or_fitness_value(C.addressof(cfg), n, z)
+
+
+# ---------------------------------------------------------------- simulator
+class OracleSim:
+    """Whole-generation oracle over the shared streams (one child population)."""
+
+    def __init__(self, cfg):
+        from clotho_amd.api import Shape  # layout of the C struct, no compute
+
+        self._Shape = Shape
+        self.cfg = cfg
+        self.h = lib().or_sim_create(C.addressof(cfg))
+        if not self.h:
+            raise ValueError("oracle: invalid configuration")
+
+    def close(self):
+        if self.h:
+            lib().or_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_threads(self, t):
+        lib().or_sim_set_threads(self.h, int(t))
+
+    def set_population(self, rows, positions, effects, fitness=None):
+        rows = np.ascontiguousarray(rows, np.uint64)
+        positions = np.ascontiguousarray(positions, np.float64)
+        effects = np.ascontiguousarray(effects, np.float64)
+        fp = None
+        if fitness is not None:
+            fitness = np.ascontiguousarray(fitness, np.float64)
+            fp = fitness.ctypes.data
+        rc = lib().or_sim_set_population(self.h, rows, rows.shape[1], positions,
+                                         effects.reshape(-1), positions.size, fp)
+        if rc != 0:
+            raise ValueError("oracle: bad population")
+
+    def step(self):
+        rc = lib().or_sim_step(self.h)
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed: {rc}")
+
+    def shape(self):
+        s = self._Shape()
+        lib().or_sim_shape(self.h, C.addressof(s))
+        return s
+
+    def rows(self):
+        s = self.shape()
+        out = np.zeros((s.haplotypes, s.words), np.uint64)
+        if out.size:
+            lib().or_sim_rows(self.h, out)
+        return out
+
+    def alleles(self):
+        s = self.shape()
+        pos = np.zeros(max(s.alleles, 1), np.float64)
+        eff = np.zeros(max(s.alleles * self.cfg.trait_count, 1), np.float64)
+        age = np.zeros(max(s.alleles, 1), np.uint32)
+        lib().or_sim_alleles(self.h, pos.ctypes.data, eff.ctypes.data, age.ctypes.data)
+        return (pos[: s.alleles], eff[: s.alleles * self.cfg.trait_count]
+                .reshape(s.alleles, self.cfg.trait_count), age[: s.alleles])
+
+    def free_space(self):
+        s = self.shape()
+        fx = np.zeros(max(s.fixed_count, 1), np.uint32)
+        ls = np.zeros(max(s.lost_count, 1), np.uint32)
+        lib().or_sim_free_space(self.h, fx.ctypes.data, ls.ctypes.data)
+        return fx[: s.fixed_count], ls[: s.lost_count]
+
+    def traits(self):
+        s = self.shape()
+        out = np.zeros((s.haplotypes, self.cfg.trait_count), np.float64)
+        if out.size:
+            lib().or_sim_traits(self.h, out)
+        return out
+
+    def fitness(self):
+        s = self.shape()
+        out = np.zeros(s.n_individuals, np.float64)
+        lib().or_sim_fitness(self.h, out)
+        return out
+
+    def gametes(self):
+        s = self.shape()
+        h = s.haplotypes
+        top = np.zeros(h, np.uint32)
+        bot = np.zeros(h, np.uint32)
+        ev = np.zeros(h, np.uint32)
+        mu = np.zeros(h, np.uint32)
+        lib().or_sim_gametes(self.h, top.ctypes.data, bot.ctypes.data, ev.ctypes.data,
+                             mu.ctypes.data)
+        return top, bot, ev, mu
+
+    def phase_seconds(self):
+        out = np.zeros(7, np.float64)
+        lib().or_sim_phase_seconds(self.h, out)
+        return out

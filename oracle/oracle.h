@@ -1,0 +1,80 @@
+/*
+ * oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the reference's per-generation offspring pipeline
+ * (Engine<parallel_pipeline>::simulate, examples/QTLSim/engine_parallel_pipeline.hpp:172-239,
+ * with offspring_generator<population_space<row_block_alignment<uint64>>>),
+ * fed by the same counter-based event streams the GPU library draws.  It is
+ * the checker for the HIP path: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ */
+#ifndef CLOTHO_ORACLE_H_
+#define CLOTHO_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/clotho_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 block: ctr[4], key[2] -> out[4]. */
+void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+/* One stream draw: key = seed, counter = (index, sub, generation, stream | replicate<<8). */
+void or_draw(uint64_t seed, uint32_t generation, uint32_t stream, uint32_t replicate,
+             uint32_t index, uint32_t sub, uint32_t* out);
+
+/* Deterministic elementary functions (fixed +,-,* sequences). */
+double or_det_log(double x);
+double or_det_exp(double x);
+double or_det_sqrt(double x);
+double or_det_recip(double x);
+double or_normal_ppf(double p);
+int or_poisson_table(double lambda, double* cdf, int cap);
+
+/* Per-phase restatements on caller buffers. */
+/* block_crossover_method over one gamete: child[Wc] from top/bottom rows of
+ * width Wp, sorted or unsorted event positions, allele positions. */
+void or_crossover(const uint64_t* top, const uint64_t* bottom, uint32_t wp, uint64_t* child,
+                  uint32_t wc, const double* positions, uint32_t alleles,
+                  const double* events, uint32_t n_events);
+/* free_space_accumulator_mt + analyze_free_indices over rows[H][W]; returns
+ * fixed count in *n_fixed and lost count in *n_lost. */
+void or_free_space(const uint64_t* rows, uint32_t h, uint32_t w, uint32_t max_alleles,
+                   uint32_t* fixed, uint32_t* n_fixed, uint32_t* lost, uint32_t* n_lost);
+/* common_phenotype_accumulator over rows[H][W]: traits[H][T]. */
+void or_phenotype(const uint64_t* rows, uint32_t h, uint32_t w, const double* effects,
+                  uint32_t alleles, uint32_t t, double* traits);
+/* Parent draw: uniform when fitness is constant, else quantised FP64 CDF search. */
+void or_select(const double* fitness, uint32_t n, uint64_t seed, uint32_t generation,
+               uint32_t replicate, uint32_t* parents /* 2n */);
+double or_fitness_value(const clotho_config* cfg, uint32_t n, const double* z);
+
+/* Whole-simulation oracle. */
+typedef struct or_sim or_sim;
+or_sim* or_sim_create(const clotho_config* cfg);
+void or_sim_destroy(or_sim* s);
+int or_sim_set_population(or_sim* s, const uint64_t* rows, uint32_t words,
+                          const double* positions, const double* effects, uint32_t alleles,
+                          const double* fitness);
+void or_sim_set_threads(or_sim* s, int threads);
+int or_sim_step(or_sim* s);
+void or_sim_shape(const or_sim* s, clotho_shape* out);
+void or_sim_rows(const or_sim* s, uint64_t* out);
+void or_sim_alleles(const or_sim* s, double* positions, double* effects, uint32_t* ages);
+void or_sim_free_space(const or_sim* s, uint32_t* fixed, uint32_t* lost);
+void or_sim_traits(const or_sim* s, double* traits);
+void or_sim_fitness(const or_sim* s, double* fitness);
+void or_sim_gametes(const or_sim* s, uint32_t* top_row, uint32_t* bottom_row,
+                    uint32_t* events, uint32_t* mutations);
+/* Phase wall-clock seconds of the last step: crossover, mutate, phenotype,
+ * fixed, fitness, finalize, draws (mirrors the reference's performance.* keys). */
+void or_sim_phase_seconds(const or_sim* s, double* out7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,356 @@
+"""Pins of the CPU oracle: known-answer vectors, closed forms, brute force on
+tiny inputs, and the reference's own hand-built unit-test constructions.
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong tie rule or index, swapped strands) fails it.  Reference files
+are cited relative to the reference root.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.stats as st
+
+import oracle
+from clotho_amd import api
+
+
+# ----------------------------------------------------------------- Philox
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32 with 10 rounds.
+    kats = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, out in kats:
+        assert list(oracle.philox(ctr, key)) == out
+
+
+def test_stream_counter_layout():
+    # draw(seed, gen, stream, rep, index, sub) = philox(ctr=(index, sub, gen, stream|rep<<8), key=seed)
+    seed = 0x0123456789ABCDEF
+    got = oracle.draw(seed, 17, 3, 5, 1000, 2)
+    want = oracle.philox([1000, 2, 17, 3 | (5 << 8)], [seed & 0xFFFFFFFF, seed >> 32])
+    assert list(got) == list(want)
+
+
+def test_uniform_streams_are_uniform():
+    # position keys (32-bit) and 53-bit uniforms of 4096 draws: KS test
+    keys = np.array([oracle.draw(42, 1, 3, 0, k, 0)[0] for k in range(4096)], np.float64)
+    assert st.kstest(keys / 2**32, "uniform").pvalue > 1e-3
+
+
+# ------------------------------------------------------ deterministic math
+def test_det_log_exp_sqrt_recip_against_libm():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([np.logspace(-300, 300, 3001), rng.uniform(1e-6, 10, 2000)])
+    for x in xs:
+        if x == 1.0:
+            continue
+        assert abs(oracle.det_log(x) - math.log(x)) <= 4e-16 * abs(math.log(x)) + 3e-17
+        assert abs(oracle.det_sqrt(x) - math.sqrt(x)) <= 4e-16 * math.sqrt(x)
+        assert abs(oracle.det_recip(x) - 1 / x) <= 4e-16 / x
+    for y in np.linspace(-700, 700, 4001):
+        assert abs(oracle.det_exp(y) - math.exp(y)) <= 4e-16 * math.exp(y)
+    assert oracle.det_exp(0.0) == 1.0
+    assert oracle.det_exp(-800.0) == 0.0
+    assert oracle.det_log(1.0) == 0.0
+
+
+def test_normal_ppf_matches_published_accuracy():
+    # Acklam's approximation: relative error below 1.15e-9 everywhere.
+    ps = np.concatenate([np.logspace(-16, -1.5, 400), np.linspace(0.02, 0.98, 961),
+                         1 - np.logspace(-16, -1.5, 400)])
+    for p in ps:
+        want = st.norm.ppf(p)
+        got = oracle.normal_ppf(p)
+        assert abs(got - want) <= 1.2e-9 * max(abs(want), 1e-2), p
+    assert oracle.normal_ppf(0.5) == 0.0
+
+
+def test_poisson_table_matches_scipy():
+    for lam in (0.0, 0.5, 1.0, 5.0, 10.0, 20.0, 40.0):
+        cdf = oracle.poisson_table(lam)
+        assert cdf[-1] == 1.0
+        ks = np.arange(len(cdf) - 1)
+        assert np.allclose(cdf[:-1], st.poisson.cdf(ks, lam), rtol=1e-12, atol=1e-15)
+        if lam > 0:
+            assert 1.0 - st.poisson.cdf(len(cdf) - 2, lam) < 1e-15
+    with pytest.raises(ValueError):
+        oracle.poisson_table(100.0)
+
+
+# ------------------------------------------------------------ recombination
+def brute_crossover(top, bottom, wc, positions, events):
+    """child[j] = odd(#{e <= pos_j}) ? bottom[j] : top[j] (zero-filled)."""
+    out = np.zeros(wc, np.uint64)
+    ev = np.asarray(events, np.float64)
+    for w in range(wc):
+        word = 0
+        for j in range(64):
+            s = 64 * w + j
+            t = int(top[w]) >> j & 1 if w < len(top) else 0
+            b = int(bottom[w]) >> j & 1 if w < len(bottom) else 0
+            if t == b:
+                bit = t
+            else:
+                cnt = int(np.sum(ev <= positions[s])) if s < len(positions) else 0
+                bit = b if (cnt & 1) else t
+            word |= bit << j
+        out[w] = word
+    return out
+
+
+def test_crossover_reference_construction():
+    """UT/data_spaces/crossover_test.cpp:53-110 rebuilt with its event injected.
+
+    Parent 1 = rows 2 (top) and 3 (bottom); the test's mt19937(1234) stream
+    produces one event at 0.562272, injected here.  Expected: 77 set
+    (homozygous), 33 unset, 88 set, 99 unset, 22 set, 15 set."""
+    words = 8
+    positions = np.zeros(512, np.float64)
+    positions[88], positions[99], positions[22], positions[15] = 0.75, 0.85, 0.5622, 0.562273
+    rows = np.zeros((4, words), np.uint64)
+
+    def setbit(r, j):
+        rows[r, j // 64] |= np.uint64(1 << (j % 64))
+
+    for r, j in [(0, 28), (0, 110), (1, 28), (2, 22), (2, 77), (2, 99), (3, 77), (3, 88), (3, 15)]:
+        setbit(r, j)
+    child = oracle.crossover(rows[2], rows[3], words, positions, [0.562272])
+
+    def bit(j):
+        return int(child[j // 64]) >> (j % 64) & 1
+
+    assert bit(77) == 1 and bit(33) == 0 and bit(88) == 1
+    assert bit(99) == 0 and bit(22) == 1 and bit(15) == 1
+    # offspring 0 from parent 0 with no event: a copy of the top strand
+    c0 = oracle.crossover(rows[0], rows[1], words, positions, [])
+    assert int(c0[0]) >> 28 & 1 == 1 and np.array_equal(c0, rows[0])
+    # swapping strands (sequence_bias) mirrors the result on het sites
+    sw = oracle.crossover(rows[3], rows[2], words, positions, [0.562272])
+    assert int(sw[99 // 64]) >> (99 % 64) & 1 == 1 and int(sw[0]) >> 22 & 1 == 0
+
+
+def test_crossover_tie_rule_is_upper_bound():
+    # region_classifier uses upper_bound: an event equal to a position counts
+    # (include/clotho/classifiers/region_classifier.hpp:73).
+    positions = np.array([0.25, 0.5, 0.75] + [0.0] * 509)
+    top = np.array([0b001] + [0] * 7, np.uint64)
+    bottom = np.array([0b110] + [0] * 7, np.uint64)
+    child = oracle.crossover(top, bottom, 8, positions, [0.5])
+    assert int(child[0]) & 0b111 == 0b111  # slot 1 (pos 0.5 == event) takes bottom
+    child = oracle.crossover(top, bottom, 8, positions, [0.5000001])
+    assert int(child[0]) & 0b111 == 0b101
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_crossover_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    wp = 8 * int(rng.integers(1, 3))
+    wc = wp + 8 * int(rng.integers(0, 2))
+    positions = np.floor(rng.uniform(0, 1, wc * 64) * 2**32) / 2**32
+    top = rng.integers(0, 2**63, wp, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, wp, dtype=np.uint64)
+    bottom = rng.integers(0, 2**63, wp, dtype=np.uint64)
+    ne = int(rng.integers(0, 9))
+    events = list(positions[rng.integers(0, wp * 64, ne // 2)]) + list(rng.uniform(0, 1, ne - ne // 2))
+    got = oracle.crossover(top, bottom, wc, positions, events)
+    want = brute_crossover(top, bottom, wc, positions, events) if ne else np.concatenate(
+        [top, np.zeros(wc - wp, np.uint64)])
+    assert np.array_equal(got, want)
+
+
+# --------------------------------------------------------------- free space
+def test_free_space_reference_construction():
+    """UT/data_spaces/free_space_test.cpp:30-66: 11 individuals (22 rows) x 200
+    alleles, column 101 set in every row, column 33 in row 2 -> fixed 1,
+    lost 198, free 199."""
+    rows = np.zeros((22, 8), np.uint64)
+    rows[:, 101 // 64] |= np.uint64(1 << (101 % 64))
+    rows[2, 0] |= np.uint64(1 << 33)
+    fixed, lost = oracle.free_space(rows, 200)
+    assert list(fixed) == [101]
+    assert len(lost) == 198 and len(fixed) + len(lost) == 199
+    assert 33 not in lost and 101 not in lost and list(lost) == sorted(lost)
+    assert lost.max() < 200  # the < M filter drops padding columns
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_free_space_brute_force(seed):
+    rng = np.random.default_rng(100 + seed)
+    h, w = int(rng.integers(1, 12)), 8 * int(rng.integers(1, 3))
+    m = int(rng.integers(1, w * 64 + 1))
+    dens = rng.uniform(0, 1, w * 64)
+    bits = (rng.uniform(0, 1, (h, w * 64)) < dens).astype(np.uint8)
+    bits[:, m:] = 0
+    bits[:, rng.integers(0, m, 5)] = 1  # some fixed columns
+    rows = np.packbits(bits, axis=1, bitorder="little").view(np.uint64)
+    fixed, lost = oracle.free_space(rows, m)
+    col_and = bits.all(axis=0)
+    col_or = bits.any(axis=0)
+    assert list(fixed) == [j for j in range(m) if col_and[j]]
+    assert list(lost) == [j for j in range(m) if not col_or[j]]
+
+
+# ---------------------------------------------------------------- phenotype
+def test_phenotype_naive_sum():
+    # UT/data_spaces/row_grouped_trait_accumulator_test.cpp:131-168 style:
+    # compare with a naive per-bit sum (exact for dyadic effects).
+    rng = np.random.default_rng(9)
+    h, w, t = 7, 16, 3
+    bits = (rng.uniform(0, 1, (h, w * 64)) < 0.3).astype(np.uint8)
+    rows = np.packbits(bits, axis=1, bitorder="little").view(np.uint64)
+    eff = np.round(rng.normal(0, 1, (w * 64, t)) * 2**20) / 2**20
+    got = oracle.phenotype(rows, eff)
+    assert np.array_equal(got, bits.astype(np.float64) @ eff)
+
+
+# ---------------------------------------------------------------- selection
+def test_selection_uniform_when_constant():
+    n = 1000
+    par = oracle.select(np.full(n, 0.5), 42, 3)
+    for i in range(0, n, 97):
+        d = oracle.draw(42, 3, 0, 0, i, 0)
+        k0 = ((int(d[1]) << 32) | int(d[0])) >> 11
+        k1 = ((int(d[3]) << 32) | int(d[2])) >> 11
+        assert par[2 * i] == (k0 * n) >> 53 and par[2 * i + 1] == (k1 * n) >> 53
+
+
+def test_selection_proportional_chi_square():
+    n = 50
+    rng = np.random.default_rng(4)
+    f = rng.uniform(0, 1, n)
+    f[7] = 0.0
+    counts = np.zeros(n)
+    for gen in range(1, 81):
+        par = oracle.select(f, 11, gen)
+        counts += np.bincount(par, minlength=n)
+    assert counts[7] == 0
+    exp = counts.sum() * f / f.sum()
+    mask = exp > 0
+    chi = ((counts[mask] - exp[mask]) ** 2 / exp[mask]).sum()
+    assert st.chi2.sf(chi, mask.sum() - 1) > 1e-4
+
+
+def test_selection_inverse_cdf_definition():
+    # idx = min{i : floor(k T / 2^53) < C[i]} on the quantised cumulative table
+    n = 37
+    rng = np.random.default_rng(5)
+    f = rng.uniform(0.1, 2.0, n)
+    par = oracle.select(f, 99, 2)
+    L = max(1, math.ceil(math.log2(n)))
+    sc = 52 - L - (math.frexp(f.max())[1])
+    q = np.floor(np.ldexp(f, sc))
+    C = np.cumsum(q)
+    T = int(C[-1])
+    for i in range(n):
+        d = oracle.draw(99, 2, 0, 0, i, 0)
+        k = ((int(d[1]) << 32) | int(d[0])) >> 11
+        target = (k * T) >> 53
+        assert par[2 * i] == int(np.searchsorted(C, target, side="right"))
+
+
+# ------------------------------------------------------------------ fitness
+def test_fitness_closed_forms():
+    n, mu = 100, 0.01
+    cfg = api.make_config(n, mu, 1.0, fitness=api.FIT_GAUSSIAN)
+    sigma = math.sqrt(4 * n * mu)
+    for x in (0.0, 0.3, -1.7, 5.0):
+        want = math.exp(-x * x / (2 * sigma * sigma)) / (sigma * math.sqrt(2 * math.pi))
+        assert abs(oracle.fitness_value(cfg, n, [x]) - want) <= 1e-15 * max(want, 1e-300)
+    cfg = api.make_config(n, mu, 1.0, fitness=api.FIT_QUADRATIC, fitness_scale=2.0)
+    s = 2.0 * math.sqrt(4 * n * mu)
+    assert oracle.fitness_value(cfg, n, [0.0]) == 1.0
+    assert oracle.fitness_value(cfg, n, [s]) == 0.0 and oracle.fitness_value(cfg, n, [-3 * s]) == 0.0
+    assert abs(oracle.fitness_value(cfg, n, [0.5 * s]) - 0.75) < 1e-15
+    cfg = api.make_config(n, mu, 1.0, traits=3, fitness=api.FIT_MV_STABILIZING, fitness_scale=1.0)
+    z = [0.5, -1.0, 2.0]
+    assert abs(oracle.fitness_value(cfg, n, z) - math.exp(-sum(v * v for v in z) / 2)) < 1e-16
+    cfg = api.make_config(n, mu, 1.0, fitness=api.FIT_CONSTANT)
+    assert oracle.fitness_value(cfg, n, [123.0]) == 1.0
+
+
+# ----------------------------------------------------- whole-generation oracle
+def test_generation_invariants():
+    """One generation: child rows are recombinations of the recorded parents,
+    new mutations land on previously free columns, one per slot, in the order
+    [fixed asc, lost asc, appended]; free lists equal the column AND / ~OR."""
+    cfg = api.make_config(60, 3.0, 2.0, seed=21, traits=1, neutral_p=0.5, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED)
+    sim = oracle.OracleSim(cfg)
+    for _ in range(5):
+        sim.step()
+    parent = sim.rows()
+    pfix, plost = sim.free_space()
+    pos_before, _, _ = sim.alleles()
+    a_before = sim.shape().alleles
+    sim.step()
+    child = sim.rows()
+    shape = sim.shape()
+    pos, eff, age = sim.alleles()
+    top, bot, nev, nmut = sim.gametes()
+    assert shape.new_mutations == nmut.sum()
+    # remove fixed columns from the parent
+    pclear = parent.copy()
+    for j in pfix:
+        pclear[:, j // 64] &= ~np.uint64(1 << (j % 64))
+    free = list(pfix) + list(plost)
+    slots = [int(free[k]) if k < len(free) else a_before + k - len(free) for k in range(int(nmut.sum()))]
+    assert len(set(slots)) == len(slots)
+    assert shape.alleles == a_before + max(0, nmut.sum() - len(free))
+    off = np.concatenate([[0], np.cumsum(nmut)]).astype(np.int64)
+    mutated = np.zeros_like(child)
+    for o in range(len(nmut)):
+        for k in range(off[o], off[o + 1]):
+            s = slots[k]
+            mutated[o, s // 64] |= np.uint64(1 << (s % 64))
+            assert age[s] == shape.generation - 1
+    recomb = child ^ mutated
+    # mutated bits were clear after recombination
+    assert not np.any(recomb & mutated)
+    bits_p = np.unpackbits(pclear.view(np.uint8), axis=1, bitorder="little")
+    bits_c = np.unpackbits(recomb.view(np.uint8), axis=1, bitorder="little")
+    for o in range(len(top)):
+        t, b = bits_p[top[o]], bits_p[bot[o]]
+        homo = t == b
+        w = min(len(t), len(bits_c[o]))
+        assert np.array_equal(bits_c[o][:w][homo[:w]], t[:w][homo[:w]])
+        if nev[o] == 0:
+            assert np.array_equal(bits_c[o][:w], t[:w])
+        assert top[o] // 2 == bot[o] // 2 and top[o] != bot[o]
+    fx, ls = sim.free_space()
+    cb = np.unpackbits(child.view(np.uint8), axis=1, bitorder="little")[:, : shape.alleles]
+    assert list(fx) == list(np.nonzero(cb.all(axis=0))[0])
+    assert list(ls) == list(np.nonzero(~cb.any(axis=0))[0])
+    # traits equal the naive per-bit sum (effects are dyadic, sums exact)
+    assert np.array_equal(sim.traits(), cb.astype(np.float64) @ eff)
+
+
+def test_threads_do_not_change_results():
+    cfg = api.make_config(200, 2.0, 2.0, seed=8, traits=1, neutral_p=0.3, sigma=0.2,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=0.5)
+    a, b = oracle.OracleSim(cfg), oracle.OracleSim(cfg)
+    b.set_threads(4)
+    for _ in range(8):
+        a.step()
+        b.step()
+    assert np.array_equal(a.rows(), b.rows())
+    assert np.array_equal(a.fitness(), b.fitness())
+    assert np.array_equal(a.traits(), b.traits())
+
+
+def test_config0_dynamics_plausible():
+    """Config 0 (neutral, mu = rho = 1, N = 1000): segregating sites after g
+    generations grow like theta * sum 1/i with theta = 4 N mu = 4000 per
+    gamete-pair... bounded checks only (parity is the GPU tests' job)."""
+    cfg = api.make_config(1000, 1.0, 1.0, seed=42, traits=1, neutral_p=1.0)
+    sim = oracle.OracleSim(cfg)
+    for _ in range(40):
+        sim.step()
+    s = sim.shape()
+    assert s.all_neutral == 1 and s.fitness_constant == 1
+    assert 1500 <= s.new_mutations <= 2500
+    assert 8000 < s.variable_count < 20000
+    assert s.words == ((s.alleles + 511) // 512) * 8
