@@ -1,0 +1,202 @@
+"""Offspring-pipeline throughput on one MI355X.
+
+Workload (single population, the configuration the metric is quoted on):
+N = 100,000 diploid (H = 200,000 haplotype rows), 50,176 allele slots
+(W = 784 uint64 words per row), synthetic steady-state standing variation with
+column frequencies from the neutral 1/f spectrum, Poisson(5) crossovers per
+gamete, one additive trait with N(0, 0.1^2) effects on new mutations,
+environmental noise N(0, 1), Gaussian stabilising fitness exp(-z^2/2).  The
+mutation rate is 0.01 per gamete: the rate at which ~50k segregating sites are
+the neutral equilibrium for N = 1e5 (4 N mu sum 1/i ~ 5e4); the literal
+Poisson(10) would add 2e6 new columns per generation (DESIGN.md has the
+arithmetic).
+
+A step is one generation of the pipeline (draws, recombination + mutation,
+free space, traits, fitness, selection) over resident HBM state, replayed as
+a captured hipGraph.  `value` = offspring genomes (haplotype rows) per second.
+The dominant kernel (recombination) is timed with HIP events on the
+library's stream in a second pass; the CPU baseline is the oracle (a C++
+restatement of the reference engine) timed on one host core for one
+generation of the same population, and its output is compared bit for bit.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_IND = 100_000
+ALLELES = 50_176          # W = 784 words
+SEGREGATING = 46_000      # columns with standing variation; the rest start free
+MU, RHO = 0.01, 5.0
+PEAK_HBM_GBPS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload_config(api):
+    return api.make_config(N_IND, MU, RHO, seed=42, traits=1, fitness=api.FIT_GAUSSIAN_FIXED,
+                           neutral_p=0.0, mean=0.0, sigma=0.1, fitness_scale=1.0, env_sigma=1.0,
+                           quantum_log2=-24, max_alleles=ALLELES + 8 * 512)
+
+
+def standing_variation(seed=42):
+    """Per-slot frequencies from P(f) ~ 1/f on [1/H, 1 - 1/H] (inverse
+    transform f = a (b/a)^u), uniform positions k 2^-32, neutral effects."""
+    h = 2 * N_IND
+    rng = np.random.default_rng(seed)
+    a, b = 1.0 / h, 1.0 - 1.0 / h
+    freqs = np.zeros(ALLELES)
+    freqs[:SEGREGATING] = a * (b / a) ** rng.uniform(0, 1, SEGREGATING)
+    rng.shuffle(freqs)
+    positions = np.floor(rng.uniform(0, 1, ALLELES) * 2**32) / 2**32
+    effects = np.zeros((ALLELES, 1))
+    return freqs, positions, effects
+
+
+def traffic_from_profiles():
+    """HBM bytes per recombination launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated by a copy kernel), if any."""
+    p = os.path.join(ROOT, "profiles", "recombine_c1_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return float(json.load(f)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
+    ap.add_argument("--eager", action="store_true", help="time eager launches instead of a graph")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from clotho_amd import api
+
+    cfg = workload_config(api)
+    h = 2 * N_IND
+    copy_gbps = api.stream_copy_gbps(1 << 30, 20)
+    log(f"[bench] measured device copy bandwidth {copy_gbps:.0f} GB/s")
+
+    sim = api.Simulation(cfg)
+    freqs, positions, effects = standing_variation()
+    t0 = time.perf_counter()
+    sim.set_synthetic(freqs, positions, effects, seed=42)
+    s0 = sim.shape()
+    log(f"[bench] synthetic population H={h} W={s0.words} alleles={s0.alleles} "
+        f"free={s0.free_count} ({time.perf_counter() - t0:.2f}s)")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+
+        rows = sim.rows()
+        pos, eff, _ = sim.alleles()
+        orc = oracle.OracleSim(cfg)
+        orc.set_population(rows, pos, eff, None)
+        del rows
+        log("[bench] oracle loaded; timing one generation on one core")
+        t0 = time.perf_counter()
+        orc.step()
+        t_cpu = time.perf_counter() - t0
+        sim.run(1)
+        g_rows = sim.rows()
+        parity = bool(np.array_equal(g_rows, orc.rows()))
+        del g_rows
+        fx, ls = sim.free_space()
+        ofx, ols = orc.free_space()
+        parity = parity and np.array_equal(fx, ofx) and np.array_equal(ls, ols)
+        parity = parity and np.array_equal(sim.traits(), orc.traits())
+        parity = parity and np.array_equal(sim.fitness(), orc.fitness())
+        top, bot, _, _ = sim.gametes()
+        otop, obot, _, _ = orc.gametes()
+        parity = parity and np.array_equal(top, otop) and np.array_equal(bot, obot)
+        cpu = {"value": h / t_cpu, "unit": "offspring genomes/s", "cores": 1, "kind": "port",
+               "sample": f"one full generation (H={h}, W={s0.words}) of the oracle, single thread, "
+                         f"{t_cpu:.1f} s; phases crossover/mutate/phenotype/fixed/fitness/"
+                         f"finalize/draws = "
+                         + "/".join(f"{x:.2f}" for x in orc.phase_seconds()) + " s",
+               "seconds": t_cpu}
+        orc.close()
+        log(f"[bench] cpu generation {t_cpu:.2f}s; parity={parity}")
+
+    graph = not args.eager
+    sim.run(args.warmup, graph=graph)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(args.steps, graph=graph)
+    sim.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    s1 = sim.shape()
+    log(f"[bench] {args.steps} generations in {elapsed * 1e3:.2f} ms; W={s1.words} "
+        f"alleles={s1.alleles} M={s1.new_mutations} free={s1.free_count}")
+
+    # per-phase kernel time on the library stream (eager pass with HIP events)
+    sim.set_timing(True)
+    sim.run(args.steps, graph=False)
+    ngen, ms = sim.timings()
+    sim.set_timing(False)
+    rec_ms = ms[2] / max(ngen, 1)
+    words = s1.words
+    alg_bytes = 24.0 * words * h
+    achieved = alg_bytes / (rec_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    phases = {k: round(float(ms[i]) / max(ngen, 1), 4) for i, k in
+              enumerate(["generation", "draw", "recombine", "traits", "free_space", "fitness"])}
+    log(f"[bench] per-generation ms (eager): {phases}")
+
+    ms_per_step = elapsed / args.steps * 1e3
+    value = h * args.steps / elapsed
+    out = {
+        "metric": "offspring genomes/s/GPU",
+        "value": value,
+        "unit": "offspring genomes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "single population N=1e5 (H=2e5), W=784 (50,176 slots), "
+                               "neutral 1/f standing variation, rho=5, mu=0.01/gamete, "
+                               "1 trait N(0,0.1^2), env N(0,1), fitness exp(-z^2/2)",
+                   "haplotypes": h, "words": words, "rho": RHO, "mu": MU,
+                   "graph": graph},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
+                     "kernel": "k_recombine", "kernel_ms": rec_ms,
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
+        "hbm_copy_gbps_measured": copy_gbps,
+        "phase_ms": phases,
+        "diploid_per_s": value / 2,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

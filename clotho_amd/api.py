@@ -106,8 +106,10 @@ EXPORTS = [
     "clotho_download_population", "clotho_download_alleles", "clotho_download_free_space",
     "clotho_download_traits", "clotho_download_fitness", "clotho_download_gametes",
     "clotho_run", "clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
+    "clotho_phase_traits",
     "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit", "clotho_sync",
     "clotho_timings", "clotho_set_timing", "clotho_trait_contract",
+    "clotho_set_synthetic_population", "clotho_stream_copy", "clotho_trait_contract_synthetic",
 ]
 
 
@@ -129,6 +131,8 @@ def lib():
     L.clotho_poisson_table.argtypes = [C.c_double, f64p, C.c_int]
     L.clotho_poisson_table.restype = C.c_int
     L.clotho_set_population.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, vp]
+    L.clotho_set_synthetic_population.argtypes = [vp, C.c_uint32, vp, vp, vp, C.c_uint64]
+    L.clotho_stream_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.c_int]
     L.clotho_get_shape.argtypes = [vp, vp]
     L.clotho_download_population.argtypes = [vp, vp, C.c_size_t]
     L.clotho_download_alleles.argtypes = [vp, vp, vp, vp, C.c_size_t]
@@ -138,6 +142,7 @@ def lib():
     L.clotho_download_gametes.argtypes = [vp, vp, vp, vp, vp, C.c_size_t]
     L.clotho_run.argtypes = [vp, C.c_uint32, C.c_int]
     for n in ("clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
+              "clotho_phase_traits",
               "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit",
               "clotho_sync"):
         getattr(L, n).argtypes = [vp]
@@ -147,6 +152,9 @@ def lib():
     L.clotho_trait_contract.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, C.c_uint32,
                                         C.c_int, C.c_int, C.c_int, vp, C.POINTER(C.c_double),
                                         C.c_int]
+    L.clotho_trait_contract_synthetic.argtypes = [C.c_uint32, C.c_uint32, vp, C.c_uint64, vp,
+                                                  C.c_uint32, C.c_int, C.c_int, C.c_int, vp,
+                                                  C.POINTER(C.c_double), C.c_int]
     _lib = L
     return L
 
@@ -192,6 +200,16 @@ class Simulation:
         self._check(lib().clotho_set_population(self.ctx, rows.ctypes.data, rows.shape[1],
                                                 positions.ctypes.data, effects.ctypes.data,
                                                 positions.size, fp), "set_population")
+
+    def set_synthetic(self, freqs, positions, effects, seed=42):
+        """Standing variation drawn on the device: row r has slot j with
+        probability freqs[j] (see clotho_set_synthetic_population)."""
+        freqs = np.ascontiguousarray(freqs, np.float64)
+        positions = np.ascontiguousarray(positions, np.float64)
+        effects = np.ascontiguousarray(effects, np.float64)
+        self._check(lib().clotho_set_synthetic_population(
+            self.ctx, freqs.size, freqs.ctypes.data, positions.ctypes.data,
+            effects.ctypes.data, seed), "set_synthetic_population")
 
     def run(self, n=1, graph=False):
         self._check(lib().clotho_run(self.ctx, n, 1 if graph else 0), "run")
@@ -277,6 +295,15 @@ def poisson_table(lam, cap=128):
     return out[:n].copy()
 
 
+def stream_copy_gbps(nbytes=1 << 30, repeats=20, device=0):
+    """Measured device copy bandwidth (read + write bytes per second / 1e9)."""
+    out = C.c_double(0.0)
+    rc = lib().clotho_stream_copy(nbytes, repeats, C.byref(out), device)
+    if rc != 0:
+        raise ClothoError(f"stream_copy failed ({ERRORS.get(rc, rc)})")
+    return out.value
+
+
 def trait_contract(rows, effects, mode, split_terms=3, repeats=1, device=0):
     """z = genotypes x effects on the GPU; returns (z [H,T] float64, avg kernel ms)."""
     rows = np.ascontiguousarray(rows, np.uint64)
@@ -289,4 +316,22 @@ def trait_contract(rows, effects, mode, split_terms=3, repeats=1, device=0):
                                      split_terms, repeats, z.ctypes.data, C.byref(ms), device)
     if rc != 0:
         raise ClothoError(f"trait_contract failed ({ERRORS.get(rc, rc)})")
+    return z, ms.value
+
+
+def trait_contract_synthetic(h, freqs, effects, mode, seed=42, split_terms=3, repeats=1,
+                             device=0):
+    """Contraction over genotypes drawn on the device (see
+    clotho_trait_contract_synthetic); returns (z [H,T] float64, avg kernel ms)."""
+    freqs = np.ascontiguousarray(freqs, np.float64)
+    effects = np.ascontiguousarray(effects, np.float64)
+    s, t = effects.shape
+    assert freqs.size == s
+    z = np.zeros((h, t), np.float64)
+    ms = C.c_double(0.0)
+    rc = lib().clotho_trait_contract_synthetic(h, s, freqs.ctypes.data, seed,
+                                               effects.ctypes.data, t, mode, split_terms,
+                                               repeats, z.ctypes.data, C.byref(ms), device)
+    if rc != 0:
+        raise ClothoError(f"trait_contract_synthetic failed ({ERRORS.get(rc, rc)})")
     return z, ms.value

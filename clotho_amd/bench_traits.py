@@ -1,0 +1,81 @@
+"""Trait contraction at the precision-configuration shape: 200,000 haplotypes
+x 65,536 sites x 16 traits, genotype bits Bernoulli(f_j) with f_j ~ U(0,1)
+(seed 42, drawn on the device), effects from a multivariate normal with unit
+variances and pairwise correlation 0.3 (seed 43).
+
+Reports kernel time of the plain FP32 set-bit walk, the FP64 walk and the
+split-BF16 MFMA contraction (1, 2, 3 BF16 terms, FP32 accumulation per K
+chunk, FP64 chunk reduction), the MFMA/plain speedup, and per-trait error
+against an FP64 reference: normwise max_i |z_i - z*_i| / max_i |z*_i| and the
+median elementwise relative error.  The FP64 reference is the GPU FP64 walk
+over all rows (FP64 sums; the GPU tests pin it against numpy float64 on rows
+rebuilt from the same generator).  One JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def effects_mvn(sites, traits, rho=0.3, seed=43):
+    cov = np.full((traits, traits), rho) + (1 - rho) * np.eye(traits)
+    L = np.linalg.cholesky(cov)
+    z = np.random.default_rng(seed).standard_normal((sites, traits))
+    return z @ L.T
+
+
+def errors(z, ref):
+    norm = np.max(np.abs(z - ref), axis=0) / np.max(np.abs(ref), axis=0)
+    rel = np.abs(z - ref) / np.maximum(np.abs(ref), 1e-300)
+    return norm, np.median(rel, axis=0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--haplotypes", type=int, default=200_000)
+    ap.add_argument("--sites", type=int, default=65_536)
+    ap.add_argument("--traits", type=int, default=16)
+    ap.add_argument("--repeats", type=int, default=10)
+    args = ap.parse_args()
+
+    from clotho_amd import api
+
+    h, s, t = args.haplotypes, args.sites, args.traits
+    freqs = np.random.default_rng(42).uniform(0, 1, s)
+    eff = effects_mvn(s, t)
+    out = {"workload": f"trait contraction H={h} S={s} T={t}, f~U(0,1), MVN effects rho=0.3"}
+
+    z64, ms64 = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_FP64, seed=42,
+                                             repeats=2)
+    out["fp64_walk_ms"] = ms64
+
+    z32, ms32 = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_PLAIN_FP32, seed=42,
+                                             repeats=max(1, args.repeats // 5))
+    n32, m32 = errors(z32, z64)
+    out["plain_fp32_ms"] = ms32
+    out["plain_fp32_err_normwise_max"] = float(n32.max())
+    for terms in (1, 2, 3):
+        zm, msm = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_MFMA_BF16, seed=42,
+                                               split_terms=terms, repeats=args.repeats)
+        nm, mm = errors(zm, z64)
+        key = f"mfma_bf16x{terms}"
+        out[key + "_ms"] = msm
+        out[key + "_speedup_vs_plain"] = ms32 / msm
+        out[key + "_err_normwise_per_trait"] = [float(x) for x in nm]
+        out[key + "_err_median_rel_per_trait"] = [float(x) for x in mm]
+        out[key + "_err_normwise_max"] = float(nm.max())
+        flops = 2.0 * h * s * t
+        out[key + "_alg_tflops"] = flops / (msm * 1e-3) / 1e12
+        out[key + "_exec_tflops"] = terms * flops / (msm * 1e-3) / 1e12
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

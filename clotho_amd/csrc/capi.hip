@@ -13,6 +13,7 @@
 
 #include "device_common.hpp"
 #include "pipeline.hpp"
+#include "traits.hpp"
 
 using namespace clotho;
 
@@ -62,6 +63,7 @@ void free_buffers(clotho_ctx* c) {
     void* ptrs[] = {B.rows[0],    B.rows[1],   B.traits[0], B.traits[1],  B.fitness[0],
                     B.fitness[1], B.keys,      B.effects,   B.ages,       B.free_list,
                     B.fixed_bits, B.mut_count, B.mut_local, B.mut_bsum,   B.top,
+                    B.sorted,     B.pm,        B.digits,
                     B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
                     B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
                     B.cdf_rho,    B.cdf_mu,    B.state};
@@ -101,10 +103,11 @@ int enqueue_generation(clotho_ctx* c, std::vector<hipEvent_t>* marks) {
     HIP_TRY(c, mark(1));
     HIP_TRY(c, launch_recombine(c));
     HIP_TRY(c, mark(2));
-    HIP_TRY(c, launch_free_space(c));
+    HIP_TRY(c, launch_traits(c));
     HIP_TRY(c, mark(3));
-    HIP_TRY(c, launch_fitness(c));
+    HIP_TRY(c, launch_free_space(c));
     HIP_TRY(c, mark(4));
+    HIP_TRY(c, launch_fitness(c));
     HIP_TRY(c, launch_commit(c));
     HIP_TRY(c, mark(5));
     return CLOTHO_OK;
@@ -228,7 +231,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     }
     P.tiles = (P.w_cap + 63) / 64;
     {
-        const uint32_t target_blocks = 4096;
+        const uint32_t target_blocks = 2048;
         uint32_t ch = (target_blocks + P.tiles - 1) / P.tiles;
         ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, P.h / 16));
         P.chunks = std::max<uint32_t>(1, ch);
@@ -251,6 +254,9 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.ages, P.s_cap);
     rc |= dalloc(c, &B.free_list, P.s_cap + 1);
     rc |= dalloc(c, &B.fixed_bits, P.w_cap);
+    rc |= dalloc(c, &B.sorted, (size_t)P.w_cap * 64);
+    rc |= dalloc(c, &B.pm, (size_t)P.w_cap * 65);
+    rc |= dalloc(c, &B.digits, i8_packed_elems(std::max<uint32_t>(1, P.traits), P.w_cap));
     rc |= dalloc(c, &B.mut_count, P.h);
     rc |= dalloc(c, &B.mut_local, P.h);
     rc |= dalloc(c, &B.mut_bsum, P.mut_blocks);
@@ -303,8 +309,6 @@ void clotho_destroy(clotho_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     destroy_graphs(ctx);
-    for (hipEvent_t& e : ctx->timing.ev)
-        if (e) (void)hipEventDestroy(e);
     free_buffers(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -329,7 +333,7 @@ int clotho_set_population(clotho_ctx* c, const uint64_t* rows, uint32_t words,
         keys[j] = (uint32_t)k;
         for (uint32_t t = 0; t < P.traits; ++t) {
             const double w = effects[(size_t)j * P.traits + t] * P.eff_scale;
-            if (w != std::floor(w) || std::fabs(w) > 2147483647.0)
+            if (w != std::floor(w) || std::fabs(w) > kEffectLimit)
                 return fail(c, CLOTHO_ERR_ARG,
                             "set_population: effects must be integer multiples of 2^q");
             eq[(size_t)j * P.traits + t] = (int32_t)w;
@@ -364,6 +368,64 @@ int clotho_set_population(clotho_ctx* c, const uint64_t* rows, uint32_t words,
     if (fitness) std::copy(fitness, fitness + P.n, fit.begin());
     HIP_TRY(c, hipMemcpy(B.fitness[cur], fit.data(), 8 * P.n, hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemset(B.traits[cur], 0, sizeof(int64_t) * P.h * std::max<uint32_t>(1, P.traits)));
+    st.alleles = alleles;
+    st.all_size = alleles;
+    st.wc = words;
+    st.wp = words;
+    st.m = 0;
+    st.error = 0;
+    rc = write_state(c, st);
+    if (rc) return rc;
+    HIP_TRY(c, launch_population_analysis(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->has_population = true;
+    return CLOTHO_OK;
+}
+
+int clotho_set_synthetic_population(clotho_ctx* c, uint32_t alleles, const double* freqs,
+                                    const double* positions, const double* effects,
+                                    uint64_t seed) {
+    if (!c) return CLOTHO_ERR_ARG;
+    const Params& P = c->p;
+    const uint32_t words = ((alleles + 511) / 512) * 8;
+    if (alleles > P.s_cap || words > P.w_cap)
+        return fail(c, CLOTHO_ERR_ARG, "synthetic population exceeds capacity");
+    if (alleles && (!freqs || !positions || (P.traits && !effects)))
+        return fail(c, CLOTHO_ERR_ARG, "synthetic population: null buffer");
+    std::vector<uint32_t> keys(P.s_cap, 0), thr(std::max<uint32_t>(alleles, 1), 0);
+    std::vector<int32_t> eq((size_t)P.s_cap * std::max<uint32_t>(1, P.traits), 0);
+    for (uint32_t j = 0; j < alleles; ++j) {
+        const double x = positions[j], k = x * 4294967296.0;
+        if (!(x >= 0.0 && x < 1.0) || k != std::floor(k))
+            return fail(c, CLOTHO_ERR_ARG, "positions must be k*2^-32 in [0,1)");
+        keys[j] = (uint32_t)k;
+        const double f = std::min(std::max(freqs[j], 0.0), 1.0);
+        thr[j] = f >= 1.0 ? 0xFFFFFFFFu : (uint32_t)std::floor(f * 4294967296.0);
+        for (uint32_t t = 0; t < P.traits; ++t) {
+            const double w = effects[(size_t)j * P.traits + t] * P.eff_scale;
+            if (w != std::floor(w) || std::fabs(w) > kEffectLimit)
+                return fail(c, CLOTHO_ERR_ARG, "effects must be integer multiples of 2^q");
+            eq[(size_t)j * P.traits + t] = (int32_t)w;
+        }
+    }
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const Buffers& B = c->b;
+    uint32_t* d_thr = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&d_thr, 4 * thr.size()));
+    hipError_t e = hipMemcpy(d_thr, thr.data(), 4 * thr.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && words) e = launch_synth_rows(c, seed, d_thr, alleles, words);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_thr);
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    HIP_TRY(c, hipMemcpy(B.keys, keys.data(), 4 * keys.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(B.effects, eq.data(), 4 * eq.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(B.ages, 0, 4 * P.s_cap));
+    std::vector<double> fit(P.n, 1.0);
+    HIP_TRY(c, hipMemcpy(B.fitness[c->cur], fit.data(), 8 * P.n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(B.traits[c->cur], 0,
+                         sizeof(int64_t) * P.h * std::max<uint32_t>(1, P.traits)));
     st.alleles = alleles;
     st.all_size = alleles;
     st.wc = words;
@@ -510,6 +572,11 @@ int clotho_phase_recombine_mutate(clotho_ctx* c) {
     HIP_TRY(c, launch_recombine(c));
     return CLOTHO_OK;
 }
+int clotho_phase_traits(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, launch_traits(c));
+    return CLOTHO_OK;
+}
 int clotho_phase_free_space(clotho_ctx* c) {
     if (!c) return CLOTHO_ERR_ARG;
     HIP_TRY(c, launch_free_space(c));
@@ -538,7 +605,7 @@ void clotho_set_timing(clotho_ctx* c, int enabled) {
 
 int clotho_timings(clotho_ctx* c, double* ms, int cap) {
     if (!c || !ms) return CLOTHO_ERR_ARG;
-    for (int i = 0; i < cap && i < 5; ++i) ms[i] = c->timing.ms[i];
+    for (int i = 0; i < cap && i < 6; ++i) ms[i] = c->timing.ms[i];
     return c->timing.generations;
 }
 
@@ -591,7 +658,7 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
                 float t = 0.f;
                 HIP_TRY(c, hipEventElapsedTime(&t, mk[0], mk[5]));
                 tm.ms[0] += t;
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < 5; ++q) {
                     HIP_TRY(c, hipEventElapsedTime(&t, mk[q], mk[q + 1]));
                     tm.ms[1 + q] += t;
                 }

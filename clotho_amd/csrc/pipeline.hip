@@ -16,6 +16,7 @@
 
 #include "device_common.hpp"
 #include "pipeline.hpp"
+#include "traits.hpp"
 
 namespace clotho {
 namespace {
@@ -110,13 +111,11 @@ __global__ void __launch_bounds__(kBlock) k_fitness_stats(Params P, const double
 }
 
 // Single block: mutation offsets, child allele count / width (child_max_alleles),
-// capacity check, fixed-column effect sums, selection quantisation exponent.
+// capacity check, selection quantisation exponent.
 __global__ void __launch_bounds__(1024) k_prepare(Params P, GenState* st, uint32_t* mut_bsum,
-                                                  const double* stats, const uint32_t* free_list,
-                                                  const int32_t* effects) {
+                                                  const double* stats) {
     __shared__ uint32_t s_carry;
     __shared__ uint32_t s_wave[16];
-    __shared__ int64_t s_red[1024];
     __shared__ double s_max[1024];
     __shared__ int s_eq[1024];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -141,21 +140,6 @@ __global__ void __launch_bounds__(1024) k_prepare(Params P, GenState* st, uint32
         if (idx < P.mut_blocks) mut_bsum[idx] = off + x - v;
         __syncthreads();
         if (tid == 0) s_carry += tot;
-        __syncthreads();
-    }
-    // fixed-column effect sums (removeFixedAlleles drops these effects)
-    const uint32_t nfix = st->n_fixed;
-    for (uint32_t t = 0; t < P.traits && t < (uint32_t)kMaxTraits; ++t) {
-        int64_t acc = 0;
-        for (uint32_t k = tid; k < nfix; k += 1024)
-            acc += (int64_t)effects[(size_t)free_list[k] * P.traits + t];
-        s_red[tid] = acc;
-        __syncthreads();
-        for (int s = 512; s > 0; s >>= 1) {
-            if (tid < s) s_red[tid] += s_red[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) st->fixed_sum[t] = s_red[0];
         __syncthreads();
     }
     // selection: constant fitness -> uniform; else quantisation exponent
@@ -229,8 +213,8 @@ __global__ void __launch_bounds__(kBlock) k_mut_attrs(Params P, const GenState* 
             if (!(u53(e.x[0], e.x[1]) < P.neutral_p)) {
                 const double w = P.trait_mean + P.trait_sigma * normal_ppf(uopen(e.x[2], e.x[3]));
                 double n = rint(w * P.eff_scale);
-                if (n > 2147483647.0) n = 2147483647.0;
-                if (n < -2147483647.0) n = -2147483647.0;
+                if (n > kEffectLimit) n = kEffectLimit;
+                if (n < -kEffectLimit) n = -kEffectLimit;
                 q = (int32_t)n;
             }
             effects[(size_t)slot * P.traits + t] = q;
@@ -319,15 +303,13 @@ __device__ inline uint32_t pick_parent(const Params& P, const GenState* st, uint
 
 // Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
 // count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
-// (common_crossover::operator(), fill_events), the top/bottom parent rows,
-// and the base of the incremental trait: trait(top) - fixed effects + new
-// mutation effects.
-__global__ void __launch_bounds__(kBlock) k_gametes(
-    Params P, const GenState* st, const double* cdf_rho, const double* cum_local,
-    const double* cum_block, const uint32_t* count, const uint32_t* local, const uint32_t* boff,
-    const uint32_t* free_list, const int32_t* effects, const int64_t* traits_parent,
-    int64_t* traits_child, uint32_t* top_out, uint32_t* bot_out, uint32_t* nev_out,
-    uint32_t* events) {
+// (common_crossover::operator(), fill_events), and the top/bottom parent rows.
+__global__ void __launch_bounds__(kBlock) k_gametes(Params P, const GenState* st,
+                                                    const double* cdf_rho,
+                                                    const double* cum_local,
+                                                    const double* cum_block, uint32_t* top_out,
+                                                    uint32_t* bot_out, uint32_t* nev_out,
+                                                    uint32_t* events) {
     const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
     if (o >= P.h) return;
     const uint32_t gen = st->gen;
@@ -338,73 +320,98 @@ __global__ void __launch_bounds__(kBlock) k_gametes(
     const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
     const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
     const bool swap = u53(g.x[2], g.x[3]) < P.bias;
-    const uint32_t top = swap ? 2 * p + 1 : 2 * p;
-    const uint32_t bot = swap ? 2 * p : 2 * p + 1;
-    top_out[o] = top;
-    bot_out[o] = bot;
+    top_out[o] = swap ? 2 * p + 1 : 2 * p;
+    bot_out[o] = swap ? 2 * p : 2 * p + 1;
     nev_out[o] = ne;
     uint32_t* ev = events + (size_t)o * P.ev_cap;
     for (uint32_t e = 0; e < ne; e += 4) {
         const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
-        for (uint32_t q = 0; q < 4; ++q) ev[e + q] = (e + q < ne) ? d.x[q] : 0u;
-    }
-    const uint32_t c = count[o];
-    const uint32_t k0 = local[o] + boff[o / kBlock];
-    const uint32_t F = st->free_count, A = st->alleles;
-    for (uint32_t t = 0; t < P.traits; ++t) {
-        int64_t v = traits_parent[(size_t)top * P.traits + t] - st->fixed_sum[t];
-        for (uint32_t kk = k0; kk < k0 + c; ++kk) {
-            const uint32_t slot = kk < F ? free_list[kk] : A + (kk - F);
-            if (slot < P.s_cap) v += effects[(size_t)slot * P.traits + t];
-        }
-        traits_child[(size_t)o * P.traits + t] = v;
+        for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
     }
 }
 
 // ---------------------------------------------------------------------------
-// Recombination + mutation (+ free-space partials, + single-trait delta)
+// Recombination + mutation (+ free-space partials)
 // ---------------------------------------------------------------------------
-// Grid: x = 64-word column tiles, y = row chunks; 4 waves per block, one
-// 64-bit word per lane.  The tile's 4096 position keys sit in LDS transposed
-// ([bit][word]) so a lane's lookups hit its own bank.  For each child row:
-//   t, b = top/bottom parent words with fixed columns cleared
-//   mask = { het bit j : odd #{event key <= key_j} }   (block_crossover)
+// Per-word order statistics, once per generation: the 64 position keys of
+// word w sorted ascending (sorted[w][k]) and prefix masks pm[w][c] = bits of
+// the c smallest keys.  For a gamete, {j in w : key_j >= e} = ~pm[w][c(e)]
+// with c(e) = #{k : sorted[w][k] < e}, so the crossover mask of the word,
+// {j : odd #{events e <= key_j}} (block_crossover + region_classifier's
+// upper_bound), is the XOR over events of ~pm[w][c(e)].
+__global__ void __launch_bounds__(kBlock) k_word_sort(Params P, const GenState* st,
+                                                      const uint32_t* keys, uint32_t* sorted,
+                                                      uint64_t* pm) {
+    __shared__ uint32_t s_key[kWaves][64];
+    __shared__ uint32_t s_perm[kWaves][64];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * kWaves + wave;
+    const uint32_t wc = st->wc, A = st->alleles;
+    if (w >= wc) return;  // uniform per wave; no block barrier follows
+    const uint32_t slot = w * 64 + lane;
+    const uint32_t key = slot < A ? keys[slot] : 0u;
+    s_key[wave][lane] = key;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < 64; ++k) {
+        const uint32_t v = s_key[wave][k];
+        rank += (v < key || (v == key && k < lane)) ? 1u : 0u;
+    }
+    s_perm[wave][rank] = lane;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t j = s_perm[wave][lane];  // slot bit of the lane-th smallest key
+    sorted[(size_t)w * 64 + lane] = s_key[wave][j];
+    uint64_t x = (uint64_t)1 << j;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d, 64);
+        if ((int)lane >= d) x |= y;
+    }
+    pm[(size_t)w * 65 + lane + 1] = x;
+    if (lane == 0) pm[(size_t)w * 65] = 0;
+}
+
+// Grid: x = 64-word column tiles, y = row chunks; kRecWaves waves per block,
+// one 64-bit word per lane.  The tile's sorted keys ([k][lane]) and prefix
+// masks ([c][lane]) sit in LDS, so the per-event lower_bound is a 7-step
+// binary search on the lane's own bank column.  Parent words of the next
+// row are loaded while the current row is classified.  For each child row:
+//   t, b  = top/bottom parent words with fixed columns cleared
+//   mask  = XOR_e ~pm[c(e)]
 //   child = (t & ~mask) | (b & mask), then XOR of this row's new mutations
-// OR/AND of the child words accumulate in registers (free_space_accumulator),
-// and for one trait the flipped het bits give the exact trait delta.
-__global__ void __launch_bounds__(kBlock) k_recombine(
+// OR/AND of the child words accumulate in registers (free_space_accumulator).
+constexpr int kRecWaves = 8;
+constexpr int kRecBlock = 64 * kRecWaves;
+
+__global__ void __launch_bounds__(kRecBlock) k_recombine(
     Params P, const GenState* st, const uint64_t* __restrict__ parent,
-    uint64_t* __restrict__ child, const uint32_t* __restrict__ keys,
-    const int32_t* __restrict__ effects, const uint32_t* __restrict__ free_list,
+    uint64_t* __restrict__ child, const uint32_t* __restrict__ sorted,
+    const uint64_t* __restrict__ pm, const uint32_t* __restrict__ free_list,
     const uint64_t* __restrict__ fixed_bits, const uint32_t* __restrict__ top_in,
     const uint32_t* __restrict__ bot_in, const uint32_t* __restrict__ nev_in,
     const uint32_t* __restrict__ events, const uint32_t* __restrict__ count,
     const uint32_t* __restrict__ local, const uint32_t* __restrict__ boff,
-    int64_t* traits_child, uint64_t* var_part, uint64_t* fix_part, int trait_delta) {
-    __shared__ uint32_t s_keys[64 * 64];
-    __shared__ uint64_t s_or[kWaves][64];
-    __shared__ uint64_t s_and[kWaves][64];
+    uint64_t* var_part, uint64_t* fix_part) {
+    __shared__ uint32_t s_sorted[64 * 64];
+    __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
 
     const uint32_t wc = st->wc;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t w0 = tile * 64;
+    const uint32_t w0 = blockIdx.x * 64;
     if (w0 >= wc) return;  // uniform for the block
     const uint32_t wp = st->wp;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t w = w0 + lane;
     const bool valid = w < wc;
+    const bool inpar = valid && w < wp;
     const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
 
-    // tile keys: lane L of wave v loads bits v, v+4, ... of word w0+L
-    {
-        const uint32_t base = w * 64;
-        for (uint32_t j = wave; j < 64; j += kWaves) {
-            const uint32_t slot = base + j;
-            s_keys[j * 64 + lane] = (slot < A) ? keys[slot] : 0u;
-        }
-    }
-    const uint64_t keep = (valid && w < wp) ? ~fixed_bits[w] : ~(uint64_t)0;
+    for (uint32_t k = wave; k < 64; k += kRecWaves)
+        s_sorted[k * 64 + lane] = valid ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
+    for (uint32_t c = wave; c < 65; c += kRecWaves)
+        s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
+    const uint64_t keep = inpar ? ~fixed_bits[w] : ~(uint64_t)0;
     __syncthreads();
 
     const uint32_t rows_per_chunk = (P.h + gridDim.y - 1) / gridDim.y;
@@ -412,56 +419,40 @@ __global__ void __launch_bounds__(kBlock) k_recombine(
     const uint32_t r1 = min(P.h, r0 + rows_per_chunk);
     uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-    for (uint32_t o = r0 + wave; o < r1; o += kWaves) {
-        const uint32_t top = wave_uniform(top_in[o]);
-        const uint32_t bot = wave_uniform(bot_in[o]);
-        const uint32_t ne = wave_uniform(nev_in[o]);
-        uint64_t t = 0, b = 0;
-        if (valid && w < wp) {
-            t = parent[(size_t)top * wp + w] & keep;
-            b = parent[(size_t)bot * wp + w] & keep;
+    uint32_t o = r0 + wave;
+    uint64_t t = 0, b = 0;
+    if (o < r1 && inpar) {
+        t = parent[(size_t)top_in[o] * wp + w];
+        b = parent[(size_t)bot_in[o] * wp + w];
+    }
+    for (; o < r1; o += kRecWaves) {
+        // prefetch the next row's parent words
+        const uint32_t on = o + kRecWaves;
+        uint64_t tn = 0, bn = 0;
+        if (on < r1 && inpar) {
+            tn = parent[(size_t)top_in[on] * wp + w];
+            bn = parent[(size_t)bot_in[on] * wp + w];
         }
+        t &= keep;
+        b &= keep;
+        const uint32_t ne = nev_in[o];
         uint64_t m = 0;
-        int64_t delta = 0;
-        if (ne) {
-            const uint32_t* ev = events + (size_t)o * P.ev_cap;
-            // events in scalar registers, groups of four; pads are key 0
-            // (always <= key) and their count's parity is removed at the end
-            uint32_t e0 = ev[0], e1 = ev[1], e2 = ev[2], e3 = ev[3];
-            uint32_t e4 = 0, e5 = 0, e6 = 0, e7 = 0;
-            const uint32_t groups = (ne + 3) >> 2;
-            if (groups > 1) {
-                e4 = ev[4];
-                e5 = ev[5];
-                e6 = ev[6];
-                e7 = ev[7];
-            }
-            const uint32_t padpar = (groups * 4 - ne) & 1u;
-            uint64_t hets = t ^ b;
-            while (hets) {
-                const uint32_t j = (uint32_t)__builtin_ctzll(hets);
-                hets &= hets - 1;
-                const uint32_t key = s_keys[j * 64 + lane];
-                uint32_t cnt = (key >= e0) + (key >= e1) + (key >= e2) + (key >= e3);
-                if (groups > 1) cnt += (key >= e4) + (key >= e5) + (key >= e6) + (key >= e7);
-                for (uint32_t gq = 2; gq < groups; ++gq) {
-                    const uint32_t* eg = ev + 4 * gq;
-                    cnt += (key >= eg[0]) + (key >= eg[1]) + (key >= eg[2]) + (key >= eg[3]);
-                }
-                if ((cnt ^ padpar) & 1u) {
-                    m |= (uint64_t)1 << j;
-                    if (trait_delta) {
-                        const int32_t e = effects[(size_t)(w * 64 + j)];
-                        delta += ((b >> j) & 1) ? (int64_t)e : -(int64_t)e;
-                    }
-                }
-            }
+        const uint32_t* ev = events + (size_t)o * P.ev_cap;
+        for (uint32_t e = 0; e < ne; ++e) {
+            const uint32_t key = ev[e];
+            // lower_bound over the 63 smallest keys, then the 64th
+            uint32_t pos = 0;
+#pragma unroll
+            for (uint32_t sstep = 32; sstep >= 1; sstep >>= 1)
+                pos += (s_sorted[(pos + sstep - 1) * 64 + lane] < key) ? sstep : 0u;
+            pos += (s_sorted[pos * 64 + lane] < key) ? 1u : 0u;
+            m ^= ~s_pm[pos * 64 + lane];
         }
         uint64_t c = (t & ~m) | (b & m);
         // new mutations of this row (free columns: zero in every parent row)
-        const uint32_t nm = wave_uniform(count[o]);
+        const uint32_t nm = count[o];
         if (nm) {
-            const uint32_t k0 = wave_uniform(local[o] + boff[o / kBlock]);
+            const uint32_t k0 = local[o] + boff[o / kBlock];
             for (uint32_t k = k0; k < k0 + nm; ++k) {
                 const uint32_t slot = k < F ? free_list[k] : A + (k - F);
                 if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
@@ -470,58 +461,23 @@ __global__ void __launch_bounds__(kBlock) k_recombine(
         if (valid) child[(size_t)o * wc + w] = c;
         acc_or |= c;
         acc_and &= c;
-        if (trait_delta) {
-            // wave sum of the per-lane deltas; one atomic add per row and tile
-            int64_t s = delta;
-            for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-            if (lane == 0 && s != 0) atomicAdd((unsigned long long*)&traits_child[o],
-                                               (unsigned long long)s);
-        }
+        t = tn;
+        b = bn;
     }
-    s_or[wave][lane] = acc_or;
-    s_and[wave][lane] = acc_and;
+    __syncthreads();  // every wave is past its last prefix-mask read
+    uint64_t* s_or = s_pm;
+    uint64_t* s_and = s_pm + kRecWaves * 64;
+    s_or[wave * 64 + lane] = acc_or;
+    s_and[wave * 64 + lane] = acc_and;
     __syncthreads();
     if (wave == 0 && valid) {
-        uint64_t vo = s_or[0][lane], va = s_and[0][lane];
-        for (int q = 1; q < kWaves; ++q) {
-            vo |= s_or[q][lane];
-            va &= s_and[q][lane];
+        uint64_t vo = s_or[lane], va = s_and[lane];
+        for (int q = 1; q < kRecWaves; ++q) {
+            vo |= s_or[q * 64 + lane];
+            va &= s_and[q * 64 + lane];
         }
         var_part[(size_t)blockIdx.y * P.w_cap + w] = vo;
         fix_part[(size_t)blockIdx.y * P.w_cap + w] = va;
-    }
-}
-
-// Exact trait sums for T > 1 (or a full recompute): one wave per row, set-bit
-// walk of common_phenotype_accumulator with integer accumulation (any order
-// is exact for quantised effects).
-__global__ void __launch_bounds__(kBlock) k_traits_exact(Params P, const GenState* st,
-                                                         const uint64_t* rows,
-                                                         const int32_t* effects,
-                                                         int64_t* traits, uint32_t words) {
-    const uint32_t wv = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const uint32_t lane = threadIdx.x & 63;
-    if (wv >= P.h) return;
-    const uint32_t W = words ? words : st->wc;
-    const uint32_t A = words ? st->alleles : st->all_size;
-    const uint32_t T = P.traits;
-    int64_t acc[kMaxTraits];
-    for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t) acc[t] = 0;
-    const uint64_t* row = rows + (size_t)wv * W;
-    for (uint32_t w = lane; w < W; w += 64) {
-        uint64_t x = row[w];
-        while (x) {
-            const uint32_t j = w * 64 + (uint32_t)__builtin_ctzll(x);
-            x &= x - 1;
-            if (j >= A) continue;
-            for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t)
-                acc[t] += effects[(size_t)j * T + t];
-        }
-    }
-    for (uint32_t t = 0; t < T && t < (uint32_t)kMaxTraits; ++t) {
-        int64_t s = acc[t];
-        for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-        if (lane == 0) traits[(size_t)wv * T + t] = s;
     }
 }
 
@@ -698,6 +654,26 @@ __global__ void __launch_bounds__(kBlock) k_fitness(Params P, const GenState* st
     fit[i] = fitness_of(P, z);
 }
 
+
+// Synthetic standing variation: bit j of row r set when Philox word
+// (seed, gen 0, stream 7, replicate, r, j/4)[j%4] < thr[j].
+__global__ void __launch_bounds__(kBlock) k_synth_rows(Params P, uint64_t seed,
+                                                       const uint32_t* thr, uint32_t alleles,
+                                                       uint32_t words, uint64_t* rows) {
+    const size_t id = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (size_t)P.h * words) return;
+    const uint32_t r = (uint32_t)(id / words), w = (uint32_t)(id % words);
+    uint64_t x = 0;
+    for (uint32_t q = 0; q < 16; ++q) {
+        const uint32_t j0 = w * 64 + q * 4;
+        if (j0 >= alleles) break;
+        const Draw4 d = philox_draw(seed, 0, 7, P.replicate, r, j0 / 4);
+        for (uint32_t k = 0; k < 4; ++k)
+            if (j0 + k < alleles && d.x[k] < thr[j0 + k]) x |= (uint64_t)1 << (q * 4 + k);
+    }
+    rows[id] = x;
+}
+
 __global__ void k_commit(GenState* st) {
     st->alleles = st->all_size;
     st->wp = st->wc;
@@ -735,7 +711,6 @@ hipError_t launch_draw(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
     const int par = c->cur;  // parent = current population
-    const int chi = 1 - c->cur;
     const uint32_t sel_blocks = (P.n + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_mut_counts, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
                        B.cdf_mu, B.mut_count, B.mut_local, B.mut_bsum);
@@ -744,7 +719,7 @@ hipError_t launch_draw(clotho_ctx* c) {
                        B.fitness[par], B.fit_stats);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, c->stream, P, B.state, B.mut_bsum,
-                       B.fit_stats, B.free_list, B.effects);
+                       B.fit_stats);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_mut_attrs, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
                        B.mut_count, B.mut_local, B.mut_bsum, B.free_list, B.keys, B.effects,
@@ -756,31 +731,41 @@ hipError_t launch_draw(clotho_ctx* c) {
     hipLaunchKernelGGL(k_sel_blockscan, dim3(1), dim3(1024), 0, c->stream, B.state, B.cum_block);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gametes, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.cdf_rho, B.cum_local, B.cum_block, B.mut_count, B.mut_local,
-                       B.mut_bsum, B.free_list, B.effects, B.traits[par], B.traits[chi], B.top,
-                       B.bottom, B.n_events, B.events);
+                       B.cdf_rho, B.cum_local, B.cum_block, B.top, B.bottom, B.n_events,
+                       B.events);
     LAUNCH_CHECK();
     return hipSuccess;
+}
+
+// Exact genome traits of population [which] (width st->wc, alleles st->all_size).
+static hipError_t launch_traits_of(clotho_ctx* c, int which) {
+    const Params& P = c->p;
+    Buffers& B = c->b;
+    if (!P.traits) return hipSuccess;
+    hipError_t e = launch_pack_i8(B.effects, P.traits, &B.state->wc, &B.state->all_size, P.w_cap,
+                                  B.digits, c->stream);
+    if (e != hipSuccess) return e;
+    return launch_traits_i8(B.rows[which], P.h, &B.state->wc, P.traits, B.digits,
+                            B.traits[which], c->stream);
 }
 
 hipError_t launch_recombine(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
     const int par = c->cur, chi = 1 - c->cur;
-    const int delta = (P.traits == 1) ? 1 : 0;
-    hipLaunchKernelGGL(k_recombine, dim3(P.tiles, P.chunks), dim3(kBlock), 0, c->stream, P,
-                       B.state, B.rows[par], B.rows[chi], B.keys, B.effects, B.free_list,
-                       B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
-                       B.mut_local, B.mut_bsum, B.traits[chi], B.var_part, B.fix_part, delta);
+    (void)chi;
+    hipLaunchKernelGGL(k_word_sort, dim3((P.w_cap + kWaves - 1) / kWaves), dim3(kBlock), 0,
+                       c->stream, P, B.state, B.keys, B.sorted, B.pm);
     LAUNCH_CHECK();
-    if (P.traits > 1) {
-        const uint32_t blocks = (P.h + kWaves - 1) / kWaves;
-        hipLaunchKernelGGL(k_traits_exact, dim3(blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                           B.rows[chi], B.effects, B.traits[chi], 0u);
-        LAUNCH_CHECK();
-    }
+    hipLaunchKernelGGL(k_recombine, dim3(P.tiles, P.chunks), dim3(kRecBlock), 0, c->stream, P,
+                       B.state, B.rows[par], B.rows[chi], B.sorted, B.pm, B.free_list,
+                       B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
+                       B.mut_local, B.mut_bsum, B.var_part, B.fix_part);
+    LAUNCH_CHECK();
     return hipSuccess;
 }
+
+hipError_t launch_traits(clotho_ctx* c) { return launch_traits_of(c, 1 - c->cur); }
 
 hipError_t launch_free_space(clotho_ctx* c) {
     const Params& P = c->p;
@@ -819,11 +804,9 @@ hipError_t launch_population_analysis(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
     const int cur = c->cur;
-    if (P.traits) {
-        const uint32_t blocks = (P.h + kWaves - 1) / kWaves;
-        hipLaunchKernelGGL(k_traits_exact, dim3(blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                           B.rows[cur], B.effects, B.traits[cur], 0u);
-        LAUNCH_CHECK();
+    {
+        const hipError_t e = launch_traits_of(c, cur);
+        if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_column_reduce, dim3(P.word_blocks, P.chunks), dim3(kBlock), 0,
                        c->stream, P, B.state, B.rows[cur], B.var_part, B.fix_part);
@@ -835,6 +818,16 @@ hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {
     const size_t n = (size_t)c->p.s_cap * c->p.traits;
     hipLaunchKernelGGL(k_any_nonzero, dim3(64), dim3(kBlock), 0, c->stream, c->b.effects, n,
                        d_flag);
+    LAUNCH_CHECK();
+    return hipSuccess;
+}
+
+hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
+                             uint32_t words) {
+    const size_t n = (size_t)c->p.h * words;
+    const uint32_t blocks = (uint32_t)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_synth_rows, dim3(blocks), dim3(kBlock), 0, c->stream, c->p, seed, d_thr,
+                       alleles, words, c->b.rows[c->cur]);
     LAUNCH_CHECK();
     return hipSuccess;
 }

@@ -14,6 +14,8 @@ namespace clotho {
 constexpr int kMaxCdf = 128;      // Poisson inversion table capacity
 constexpr int kMaxTraits = 32;    // trait capacity of the fused paths
 constexpr int kScanBlock = 256;   // items per block of every two-level scan
+// |quantised effect| bound: four balanced base-256 digits represent it exactly
+constexpr double kEffectLimit = 1073741823.0;  // 2^30 - 1
 
 // Scalars of the generation, kept in device memory so that a captured
 // hipGraph replays successive generations without host involvement.
@@ -33,7 +35,6 @@ struct GenState {
     uint32_t error_gen;    // generation of the first error
     uint32_t sel_blocks;   // number of fitness scan blocks
     uint64_t sel_total;    // sum of quantised fitness
-    int64_t fixed_sum[kMaxTraits];  // sum of quantised effects over the fixed columns
 };
 
 // Kernel-side constants that do not change between generations.
@@ -69,6 +70,9 @@ struct Buffers {
     uint32_t* ages;           // [s_cap]
     uint32_t* free_list;      // [s_cap] fixed ascending then lost ascending
     uint64_t* fixed_bits;     // [w_cap] AND of all rows, idx < all_size
+    uint32_t* sorted;         // [w_cap][64] position keys of each word, ascending
+    uint64_t* pm;             // [w_cap][65] bits of the c smallest keys of each word
+    uint4* digits;            // [ceil(T/4)][w_cap][64] int8 digit planes of the effects
     uint32_t* mut_count;      // [h]
     uint32_t* mut_local;      // [h] exclusive scan inside a block
     uint32_t* mut_bsum;       // [mut_blocks] block totals -> exclusive offsets
@@ -90,8 +94,7 @@ struct Buffers {
 
 struct Timing {
     bool enabled = false;
-    hipEvent_t ev[6] = {};
-    double ms[5] = {0, 0, 0, 0, 0};
+    double ms[6] = {0, 0, 0, 0, 0, 0};
     int generations = 0;
 };
 
@@ -120,9 +123,12 @@ namespace clotho {
 // Launch sequences (pipeline.hip).
 hipError_t launch_draw(clotho_ctx* c);
 hipError_t launch_recombine(clotho_ctx* c);
+hipError_t launch_traits(clotho_ctx* c);
 hipError_t launch_free_space(clotho_ctx* c);
 hipError_t launch_fitness(clotho_ctx* c);
 hipError_t launch_commit(clotho_ctx* c);
 hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
+hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
+                             uint32_t words);
 }  // namespace clotho

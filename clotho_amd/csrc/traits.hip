@@ -9,11 +9,14 @@
 // with effects split into BF16 terms (traits_mfma.hip).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/clotho_amd.h"
+#include "device_common.hpp"
 #include "traits.hpp"
 
 namespace clotho {
@@ -86,6 +89,25 @@ hipError_t launch_traits_walk_fp64(const uint64_t* rows, uint32_t h, uint32_t w,
     return hipGetLastError();
 }
 
+// Genotype bits drawn on the device: bit j of row r set when Philox word
+// (seed, generation 0, stream 7, replicate 0, index r, sub j/4)[j%4] < thr[j].
+__global__ void __launch_bounds__(kBlock) k_synth_genotypes(uint64_t seed, const uint32_t* thr,
+                                                            uint32_t sites, uint32_t h,
+                                                            uint32_t words, uint64_t* rows) {
+    const size_t id = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (size_t)h * words) return;
+    const uint32_t r = (uint32_t)(id / words), w = (uint32_t)(id % words);
+    uint64_t x = 0;
+    for (uint32_t q = 0; q < 16; ++q) {
+        const uint32_t j0 = w * 64 + q * 4;
+        if (j0 >= sites) break;
+        const Draw4 d = philox_draw(seed, 0, 7, 0, r, j0 / 4);
+        for (uint32_t k = 0; k < 4; ++k)
+            if (j0 + k < sites && d.x[k] < thr[j0 + k]) x |= (uint64_t)1 << (q * 4 + k);
+    }
+    rows[id] = x;
+}
+
 }  // namespace clotho
 
 using namespace clotho;
@@ -118,10 +140,10 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
             rc = CLOTHO_ERR_HIP;
     }
     if (rc == CLOTHO_OK) {
-        hipEventRecord(e0, s);
+        (void)hipEventRecord(e0, s);
         for (int r = 0; r < repeats && rc == CLOTHO_OK; ++r)
             if (ws.run(s) != hipSuccess) rc = CLOTHO_ERR_HIP;
-        hipEventRecord(e1, s);
+        (void)hipEventRecord(e1, s);
         if (rc == CLOTHO_OK && hipEventSynchronize(e1) != hipSuccess) rc = CLOTHO_ERR_HIP;
         float ms = 0.f;
         if (rc == CLOTHO_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = CLOTHO_ERR_HIP;
@@ -129,9 +151,71 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
     }
     if (rc == CLOTHO_OK) rc = ws.download(z, s);
     if (avg_ms) *avg_ms = ms_total / repeats;
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     ws.release();
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+extern "C" int clotho_trait_contract_synthetic(uint32_t h, uint32_t sites, const double* freqs,
+                                               uint64_t seed, const double* effects,
+                                               uint32_t traits, int mode, int split_terms,
+                                               int repeats, double* z, double* avg_ms,
+                                               int device) {
+    if (!freqs || !effects || !z || h == 0 || sites == 0 || traits == 0 || traits > 16 ||
+        repeats < 1)
+        return CLOTHO_ERR_ARG;
+    if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
+        mode != CLOTHO_TRAITS_MFMA_BF16)
+        return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
+        return CLOTHO_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
+    const uint32_t words = ((sites + 511) / 512) * 8;
+    std::vector<uint32_t> thr(sites);
+    for (uint32_t j = 0; j < sites; ++j) {
+        const double f = std::min(std::max(freqs[j], 0.0), 1.0);
+        thr[j] = f >= 1.0 ? 0xFFFFFFFFu : (uint32_t)std::floor(f * 4294967296.0);
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CLOTHO_ERR_HIP;
+    uint32_t* d_thr = nullptr;
+    int rc = CLOTHO_OK;
+    TraitWorkspace ws;
+    if (hipMalloc((void**)&d_thr, 4 * (size_t)sites) != hipSuccess ||
+        hipMemcpy(d_thr, thr.data(), 4 * (size_t)sites, hipMemcpyHostToDevice) != hipSuccess)
+        rc = CLOTHO_ERR_HIP;
+    if (rc == CLOTHO_OK) rc = ws.init(nullptr, h, words, effects, sites, traits, mode, split_terms, s);
+    if (rc == CLOTHO_OK) {
+        const size_t n = (size_t)h * words;
+        hipLaunchKernelGGL(k_synth_genotypes, dim3((uint32_t)((n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, seed, d_thr, sites, h, words, ws.d_rows);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = CLOTHO_ERR_HIP;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == CLOTHO_OK && (ws.run(s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess))
+        rc = CLOTHO_ERR_HIP;
+    if (rc == CLOTHO_OK &&
+        (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = CLOTHO_ERR_HIP;
+    if (rc == CLOTHO_OK) {
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < repeats && rc == CLOTHO_OK; ++r)
+            if (ws.run(s) != hipSuccess) rc = CLOTHO_ERR_HIP;
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (rc == CLOTHO_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = CLOTHO_ERR_HIP;
+        if (avg_ms) *avg_ms = ms / repeats;
+    }
+    if (rc == CLOTHO_OK) rc = ws.download(z, s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (d_thr) (void)hipFree(d_thr);
+    ws.release();
+    (void)hipStreamDestroy(s);
     return rc;
 }

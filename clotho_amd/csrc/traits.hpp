@@ -17,15 +17,24 @@ hipError_t launch_traits_walk_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
 hipError_t launch_traits_walk_fp64(const uint64_t* rows, uint32_t h, uint32_t w,
                                    const double* eff, uint32_t sites, uint32_t traits, double* z,
                                    hipStream_t s);
+// Exact traits of the pipeline (traits_mfma.hip): int8 digit planes of the
+// quantised effects, int32 MFMA accumulation, int64 recombination.
+uint32_t i8_ntiles(uint32_t traits);
+size_t i8_packed_elems(uint32_t traits, uint32_t w_cap);  // uint4 elements
+hipError_t launch_pack_i8(const int32_t* eff, uint32_t traits, const uint32_t* p_words,
+                          const uint32_t* p_alleles, uint32_t w_cap, uint4* packed,
+                          hipStream_t s);
+hipError_t launch_traits_i8(const uint64_t* rows, uint32_t h, const uint32_t* p_words,
+                            uint32_t traits, const uint4* packed, int64_t* out, hipStream_t s);
 // MFMA contraction (traits_mfma.hip): effects pre-split into `terms` BF16
-// matrices laid out for the kernel; z is FP32 partials reduced in FP64.
-size_t mfma_effect_elems(uint32_t sites, uint32_t traits, int terms);
-void mfma_pack_effects(const double* effects, uint32_t sites, uint32_t traits, int terms,
-                       uint16_t* packed);
+// matrices laid out for the kernel; FP64 partials per K chunk reduced in FP64.
+size_t mfma_effect_elems(uint32_t words, int terms);
+void mfma_pack_effects(const double* effects, uint32_t sites, uint32_t traits, uint32_t words,
+                       int terms, uint16_t* packed);
 size_t mfma_partial_elems(uint32_t h, uint32_t w, uint32_t traits);
 hipError_t launch_traits_mfma(const uint64_t* rows, uint32_t h, uint32_t w,
                               const uint16_t* packed, uint32_t sites, uint32_t traits, int terms,
-                              float* partial, double* z, hipStream_t s);
+                              double* partial, double* z, hipStream_t s);
 
 struct TraitWorkspace {
     int mode = 0, terms = 0;
@@ -36,7 +45,7 @@ struct TraitWorkspace {
     uint16_t* d_packed = nullptr;
     float* d_z32 = nullptr;
     double* d_z64 = nullptr;
-    float* d_partial = nullptr;
+    double* d_partial = nullptr;
 
     int init(const uint64_t* rows, uint32_t h_, uint32_t w_, const double* eff, uint32_t s_,
              uint32_t t_, int mode_, int terms_, hipStream_t st) {
@@ -48,8 +57,8 @@ struct TraitWorkspace {
         traits = t_;
         if (hipMalloc((void**)&d_rows, sizeof(uint64_t) * h * w) != hipSuccess)
             return CLOTHO_ERR_HIP;
-        if (hipMemcpyAsync(d_rows, rows, sizeof(uint64_t) * h * w, hipMemcpyHostToDevice, st) !=
-            hipSuccess)
+        if (rows && hipMemcpyAsync(d_rows, rows, sizeof(uint64_t) * h * w,
+                                   hipMemcpyHostToDevice, st) != hipSuccess)
             return CLOTHO_ERR_HIP;
         if (mode == CLOTHO_TRAITS_PLAIN_FP32) {
             std::vector<float> e((size_t)sites * traits);
@@ -67,11 +76,11 @@ struct TraitWorkspace {
                 hipSuccess)
                 return CLOTHO_ERR_HIP;
         } else {
-            const size_t ne = mfma_effect_elems(sites, traits, terms);
+            const size_t ne = mfma_effect_elems(w, terms);
             std::vector<uint16_t> packed(ne);
-            mfma_pack_effects(eff, sites, traits, terms, packed.data());
+            mfma_pack_effects(eff, sites, traits, w, terms, packed.data());
             if (hipMalloc((void**)&d_packed, 2 * ne) != hipSuccess ||
-                hipMalloc((void**)&d_partial, 4 * mfma_partial_elems(h, w, traits)) !=
+                hipMalloc((void**)&d_partial, 8 * mfma_partial_elems(h, w, traits)) !=
                     hipSuccess ||
                 hipMalloc((void**)&d_z64, 8 * (size_t)h * traits) != hipSuccess)
                 return CLOTHO_ERR_HIP;

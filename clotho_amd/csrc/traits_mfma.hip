@@ -1,18 +1,394 @@
-// traits_mfma.hip — placeholder until the matrix-core contraction lands.
+// traits_mfma.hip — genotype x effect contraction on CDNA4 matrix cores.
+//
+// z[h][t] = sum_j bit(h, j) * effect[j][t] over bit-packed rows (the
+// population-wide common_phenotype_accumulator,
+// include/clotho/data_spaces/phenotype_evaluator/common_phenotype_accumulator_row_block_alignment.hpp:47-66).
+//
+// Exact path (v_mfma_i32_16x16x64_i8): quantised int32 effects are written as
+// four balanced base-256 digits d_p in [-128, 127]; genotype bits expand to
+// int8 0/1; each 16x16x64 product accumulates exactly in int32 and the digit
+// planes recombine in int64, so the sums equal the FP64 sequential sums of
+// the reference bit for bit (effects are multiples of 2^q).  One N tile holds
+// 4 traits x 4 digit planes.
+//
+// Split-BF16 path (v_mfma_f32_16x16x32_bf16): bits expand to BF16 0/1 (exact),
+// effects split into 1-3 BF16 terms hi + mid + lo, FP32 accumulation per
+// 64-word K chunk, chunks reduced in FP64.
+//
+// Operand layouts (16x16xK, wave64): lane l supplies A row (l % 16) and B
+// column (l % 16) for K elements [K/4 * (l / 16), K/4 * (l / 16 + 1)); the
+// accumulator lane l holds column (l % 16), rows 4 * (l / 16) + i.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstring>
+
+#include "pipeline.hpp"
 #include "traits.hpp"
 
 namespace clotho {
+namespace {
 
-size_t mfma_effect_elems(uint32_t sites, uint32_t traits, int terms) {
-    return (size_t)sites * traits * terms;
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// four genotype bits -> four int8 0/1 bytes
+__device__ inline uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
+
+// two genotype bits -> two BF16 0/1 halves (1.0 = 0x3F80)
+__device__ inline uint32_t bits2_bf16(uint32_t two) {
+    return ((two * 0x8001u) & 0x10001u) * 0x3F80u;
 }
-void mfma_pack_effects(const double*, uint32_t, uint32_t, int, uint16_t*) {}
-size_t mfma_partial_elems(uint32_t h, uint32_t, uint32_t traits) { return (size_t)h * traits; }
-hipError_t launch_traits_mfma(const uint64_t*, uint32_t, uint32_t, const uint16_t*, uint32_t,
-                              uint32_t, int, float*, double*, hipStream_t) {
-    return hipErrorNotSupported;
+
+__device__ inline int32_t balanced_digit(int32_t q, int p) {
+    int32_t d = 0;
+    for (int i = 0; i <= p; ++i) {
+        d = ((q + 128) & 255) - 128;
+        q = (q - d) >> 8;
+    }
+    return d;
+}
+
+// K-block order.  Rows are read 16 words (one 128-byte line) at a time per
+// row: lane group g = lane / 16 loads words 4g..4g+3 of the group with two
+// 16-byte loads, and K block q (of 16 per group) takes 16-bit slice q % 4 of
+// the lane's word q / 4.  The site order inside a K block is free as long as
+// the packed effects follow it, so no lane ever needs another lane's bits.
+constexpr uint32_t kGroupWords = 16;
+
+__device__ inline uint32_t i8_site(uint32_t G, uint32_t q, uint32_t g, uint32_t i) {
+    return 64 * (kGroupWords * G + 4 * g + q / 4) + 16 * (q & 3) + i;
+}
+
+// B fragments of the exact path: packed[((nt * NG + G) * 16 + q) * 64 + lane]
+// holds the 16 int8 digits for column n = lane % 16 (trait 4 nt + n / 4,
+// digit plane n % 4) and sites i8_site(G, q, lane / 16, i).
+__global__ void __launch_bounds__(256) k_pack_i8(const int32_t* __restrict__ eff, uint32_t T,
+                                                 uint32_t ntiles, const uint32_t* p_words,
+                                                 const uint32_t* p_alleles, uint32_t ng_cap,
+                                                 uint4* __restrict__ packed) {
+    const uint32_t W = *p_words, A = *p_alleles;
+    const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
+    const size_t id = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t lane = (uint32_t)(id & 63);
+    const uint32_t q = (uint32_t)((id >> 6) & 15);
+    const size_t gn = id >> 10;
+    const uint32_t G = (uint32_t)(gn % ng_cap), nt = (uint32_t)(gn / ng_cap);
+    if (nt >= ntiles || G >= NG) return;
+    const uint32_t n = lane & 15, g = lane >> 4;
+    const uint32_t t = 4 * nt + n / 4, p = n & 3;
+    uint32_t d[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t s = i8_site(G, q, g, i);
+        const int32_t v = (s < A && t < T) ? eff[(size_t)s * T + t] : 0;
+        d[i >> 2] |= ((uint32_t)balanced_digit(v, (int)p) & 0xFFu) << (8 * (i & 3));
+    }
+    packed[(((size_t)nt * NG + G) * 16 + q) * 64 + lane] = make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
+template <int RT, int NT>
+__global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ rows, uint32_t h,
+                                                   const uint32_t* p_words, uint32_t T,
+                                                   const uint4* __restrict__ packed,
+                                                   int64_t* __restrict__ traits) {
+    const uint32_t W = *p_words;
+    const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t m = lane & 15, g = lane >> 4;
+    const uint32_t base = (blockIdx.x * 4 + wave) * (16 * RT);
+    v4i acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[r][nt] = v4i{0, 0, 0, 0};
+    const uint64_t* rp[RT];
+    bool rv[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const uint32_t row = base + 16 * r + m;
+        rv[r] = row < h;
+        rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
+    }
+    for (uint32_t G = 0; G < NG; ++G) {
+        const uint32_t w = kGroupWords * G + 4 * g;
+        uint4 x[RT][2];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            if (rv[r] && w < W) {
+                const uint4* src = reinterpret_cast<const uint4*>(rp[r] + w);
+                x[r][0] = src[0];
+                x[r][1] = src[1];
+            } else {
+                x[r][0] = make_uint4(0, 0, 0, 0);
+                x[r][1] = make_uint4(0, 0, 0, 0);
+            }
+        }
+        const uint4* bp = packed + (size_t)G * 16 * 64 + lane;
+#pragma unroll
+        for (uint32_t q = 0; q < 16; ++q) {
+            v4i a[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const uint4 v = x[r][q >> 3];
+                const uint32_t dw = ((q >> 1) & 3) == 0 ? v.x
+                                  : ((q >> 1) & 3) == 1 ? v.y
+                                  : ((q >> 1) & 3) == 2 ? v.z : v.w;
+                const uint32_t b16 = (q & 1) ? (dw >> 16) : (dw & 0xFFFFu);
+                a[r] = v4i{(int)spread4(b16 & 15), (int)spread4((b16 >> 4) & 15),
+                           (int)spread4((b16 >> 8) & 15), (int)spread4(b16 >> 12)};
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const uint4 bq = bp[((size_t)nt * NG * 16 + q) * 64];
+                const v4i bv = v4i{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+                    acc[r][nt] =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
+            }
+        }
+    }
+    // recombine the digit planes: column n = 4 t_local + p held by lane m = n
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t c0 = acc[r][nt][i];
+                const int64_t c1 = __shfl_down((int)acc[r][nt][i], 1, 64);
+                const int64_t c2 = __shfl_down((int)acc[r][nt][i], 2, 64);
+                const int64_t c3 = __shfl_down((int)acc[r][nt][i], 3, 64);
+                const uint32_t row = base + 16 * r + 4 * g + i;
+                const uint32_t t = 4 * nt + m / 4;
+                if ((m & 3) == 0 && row < h && t < T)
+                    traits[(size_t)row * T + t] = c0 + (c1 << 8) + (c2 << 16) + (c3 << 24);
+            }
+        }
+    }
+}
+
+// Split-BF16 contraction, K chunked: block = 4 waves x RT row tiles, one
+// chunk of kChunkGroups 16-word groups.  The FP32 MFMA accumulator is
+// flushed into FP64 registers every kFlushBlocks K blocks (256 sites), so
+// FP32 rounding acts only on short partial sums; FP64 partial[chunk][row][t].
+// K block q (of 32 per group) takes byte q % 8 of the lane's word q / 8, i.e.
+// sites bf_site(G, q, lane / 16, i).
+constexpr uint32_t kChunkGroups = 16;
+constexpr uint32_t kFlushBlocks = 8;
+
+__host__ __device__ inline uint32_t bf_site(uint32_t G, uint32_t q, uint32_t g, uint32_t i) {
+    return 64 * (16 * G + 4 * g + q / 8) + 8 * (q & 7) + i;
+}
+
+template <int RT, int TERMS>
+__global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict__ rows,
+                                                     uint32_t h, uint32_t W, uint32_t T,
+                                                     const uint4* __restrict__ packed,
+                                                     double* __restrict__ partial) {
+    const uint32_t NG = (W + 15) / 16;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t m = lane & 15, g = lane >> 4;
+    const uint32_t base = (blockIdx.x * 4 + wave) * (16 * RT);
+    const uint32_t G0 = blockIdx.y * kChunkGroups;
+    const uint32_t G1 = min(NG, G0 + kChunkGroups);
+    double acc64[RT][4];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc64[r][i] = 0.0;
+    const uint64_t* rp[RT];
+    bool rv[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const uint32_t row = base + 16 * r + m;
+        rv[r] = row < h;
+        rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
+    }
+    for (uint32_t G = G0; G < G1; ++G) {
+        const uint32_t w = 16 * G + 4 * g;
+        uint4 x[RT][2];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            if (rv[r] && w < W) {
+                const uint4* src = reinterpret_cast<const uint4*>(rp[r] + w);
+                x[r][0] = src[0];
+                x[r][1] = src[1];
+            } else {
+                x[r][0] = make_uint4(0, 0, 0, 0);
+                x[r][1] = make_uint4(0, 0, 0, 0);
+            }
+        }
+        const uint4* bp = packed + (size_t)G * 32 * TERMS * 64 + lane;
+#pragma unroll
+        for (uint32_t qb = 0; qb < 32; qb += kFlushBlocks) {
+            v4f acc[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t q = qb; q < qb + kFlushBlocks; ++q) {
+                v8bf a[RT];
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    const uint4 v = x[r][q >> 4];
+                    const uint32_t sel = (q >> 2) & 3;
+                    const uint32_t dw = sel == 0 ? v.x : sel == 1 ? v.y : sel == 2 ? v.z : v.w;
+                    const uint32_t b8 = (dw >> (8 * (q & 3))) & 0xFFu;
+                    uint4 u = make_uint4(bits2_bf16(b8 & 3), bits2_bf16((b8 >> 2) & 3),
+                                         bits2_bf16((b8 >> 4) & 3), bits2_bf16(b8 >> 6));
+                    __builtin_memcpy(&a[r], &u, 16);
+                }
+#pragma unroll
+                for (int tm = 0; tm < TERMS; ++tm) {
+                    const uint4 bq = bp[((size_t)q * TERMS + tm) * 64];
+                    v8bf bv;
+                    __builtin_memcpy(&bv, &bq, 16);
+#pragma unroll
+                    for (int r = 0; r < RT; ++r)
+                        acc[r] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bv, acc[r], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc64[r][i] += (double)acc[r][i];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t row = base + 16 * r + 4 * g + i;
+            if (row < h && m < T)
+                partial[((size_t)blockIdx.y * h + row) * T + m] = acc64[r][i];
+        }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_chunks(const double* __restrict__ partial,
+                                                       uint32_t chunks, size_t n,
+                                                       double* __restrict__ z) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) s += partial[(size_t)c * n + i];
+    z[i] = s;
+}
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+float bf16_to_f(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+// ---- exact int8 path (used by the pipeline) -------------------------------
+uint32_t i8_ntiles(uint32_t traits) { return (traits + 3) / 4; }
+
+size_t i8_packed_elems(uint32_t traits, uint32_t w_cap) {
+    const size_t ng = (w_cap + kGroupWords - 1) / kGroupWords;
+    return (size_t)i8_ntiles(traits) * ng * 16 * 64;
+}
+
+hipError_t launch_pack_i8(const int32_t* eff, uint32_t traits, const uint32_t* p_words,
+                          const uint32_t* p_alleles, uint32_t w_cap, uint4* packed,
+                          hipStream_t s) {
+    const uint32_t nt = i8_ntiles(traits);
+    const uint32_t ng_cap = (w_cap + kGroupWords - 1) / kGroupWords;
+    const size_t n = (size_t)nt * ng_cap * 16 * 64;
+    hipLaunchKernelGGL(k_pack_i8, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, eff, traits,
+                       nt, p_words, p_alleles, ng_cap, packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_traits_i8(const uint64_t* rows, uint32_t h, const uint32_t* p_words,
+                            uint32_t traits, const uint4* packed, int64_t* out, hipStream_t s) {
+    const uint32_t nt = i8_ntiles(traits);
+    if (nt <= 1) {
+        const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
+        hipLaunchKernelGGL((k_traits_i8<2, 1>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+                           traits, packed, out);
+    } else if (nt <= 2) {
+        const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
+        hipLaunchKernelGGL((k_traits_i8<2, 2>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+                           traits, packed, out);
+    } else if (nt <= 4) {
+        const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
+        hipLaunchKernelGGL((k_traits_i8<2, 4>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+                           traits, packed, out);
+    } else {
+        const uint32_t blocks = (h + 4 * 16 - 1) / (4 * 16);
+        hipLaunchKernelGGL((k_traits_i8<1, 8>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+                           traits, packed, out);
+    }
+    return hipGetLastError();
+}
+
+// ---- split-BF16 path (standalone contraction) -----------------------------
+// The packed effects cover whole 16-word groups of the padded row width
+// (sites >= `sites` are 0): packed[((G * 32 + q) * terms + term) * 64 + lane]
+// = 8 BF16 of column lane % 16 (trait) at sites bf_site(G, q, lane / 16, i).
+size_t mfma_effect_elems(uint32_t words, int terms) {
+    const size_t ng = (words + 15) / 16;
+    return ng * 32 * (size_t)terms * 64 * 8;  // bf16 elements
+}
+
+void mfma_pack_effects(const double* effects, uint32_t sites, uint32_t traits, uint32_t words,
+                       int terms, uint16_t* packed) {
+    const uint32_t ng = (words + 15) / 16;
+    for (uint32_t G = 0; G < ng; ++G)
+        for (uint32_t q = 0; q < 32; ++q)
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                const uint32_t n = lane & 15, g = lane >> 4;
+                for (uint32_t i = 0; i < 8; ++i) {
+                    const size_t site = bf_site(G, q, g, i);
+                    double x = (site < sites && n < traits) ? effects[site * traits + n] : 0.0;
+                    for (int tm = 0; tm < terms; ++tm) {
+                        const uint16_t hb = bf16_rne((float)x);
+                        packed[((((size_t)G * 32 + q) * terms + tm) * 64 + lane) * 8 + i] = hb;
+                        x -= (double)bf16_to_f(hb);
+                    }
+                }
+            }
+}
+
+size_t mfma_partial_elems(uint32_t h, uint32_t w, uint32_t traits) {
+    const size_t chunks = ((w + 15) / 16 + kChunkGroups - 1) / kChunkGroups;
+    return chunks * h * traits;
+}
+
+hipError_t launch_traits_mfma(const uint64_t* rows, uint32_t h, uint32_t w,
+                              const uint16_t* packed, uint32_t sites, uint32_t traits, int terms,
+                              double* partial, double* z, hipStream_t s) {
+    (void)sites;
+    const uint32_t chunks = ((w + 15) / 16 + kChunkGroups - 1) / kChunkGroups;
+    const dim3 grid((h + 4 * 32 - 1) / (4 * 32), chunks);
+    const uint4* pk = reinterpret_cast<const uint4*>(packed);
+    if (terms == 1)
+        hipLaunchKernelGGL((k_traits_bf16<2, 1>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+                           partial);
+    else if (terms == 2)
+        hipLaunchKernelGGL((k_traits_bf16<2, 2>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+                           partial);
+    else
+        hipLaunchKernelGGL((k_traits_bf16<2, 3>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+                           partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t n = (size_t)h * traits;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                       partial, chunks, n, z);
+    return hipGetLastError();
 }
 
 }  // namespace clotho

@@ -28,6 +28,11 @@
  *       (QS/engine_parallel_pipeline.hpp:409-426;
  *        DS/offspring_generator/offspring_generator_population_space_row_alignment.hpp:84-260;
  *        DS/crossover/block_crossover.hpp:46-63)
+ *   clotho_phase_traits
+ *       offspring_generator::phenotype -> common_phenotype_accumulator::operator()
+ *       + population_space::updateGenomeWeights
+ *       (DS/offspring_generator/offspring_generator_population_space_row_alignment.hpp:234-251;
+ *        DS/phenotype_evaluator/common_phenotype_accumulator_row_block_alignment.hpp:47-66)
  *   clotho_phase_free_space
  *       free_space_accumulator_mt::finalize -> free_space_details::analyze_free_indices
  *       (DS/free_space/free_space_accumulator.hpp:142-151;
@@ -131,6 +136,14 @@ int clotho_poisson_table(double lambda, double* cdf, int cap);
 int clotho_set_population(clotho_ctx* ctx, const uint64_t* rows, uint32_t words,
                           const double* positions, const double* effects,
                           uint32_t alleles, const double* fitness);
+/* Synthetic standing variation, drawn on the device (no reference counterpart:
+ * the reference starts from empty founders).  Bit j of row r is set when the
+ * 32-bit Philox word (seed, generation 0, stream 7, replicate, index r,
+ * sub j/4)[j%4] is below floor(freqs[j] * 2^32).  Positions/effects as in
+ * clotho_set_population; fitness starts at 1.0. */
+int clotho_set_synthetic_population(clotho_ctx* ctx, uint32_t alleles, const double* freqs,
+                                    const double* positions, const double* effects,
+                                    uint64_t seed);
 int clotho_get_shape(clotho_ctx* ctx, clotho_shape* out);
 int clotho_download_population(clotho_ctx* ctx, uint64_t* rows, size_t cap_words);
 int clotho_download_alleles(clotho_ctx* ctx, double* positions, double* effects,
@@ -154,6 +167,7 @@ int clotho_generation(clotho_ctx* ctx);
 /* Phase-level entry points, in generation order (clotho_generation = these). */
 int clotho_phase_draw(clotho_ctx* ctx);
 int clotho_phase_recombine_mutate(clotho_ctx* ctx);
+int clotho_phase_traits(clotho_ctx* ctx);
 int clotho_phase_free_space(clotho_ctx* ctx);
 int clotho_phase_fitness(clotho_ctx* ctx);
 int clotho_phase_commit(clotho_ctx* ctx);
@@ -161,7 +175,8 @@ int clotho_sync(clotho_ctx* ctx);
 
 /* Kernel timing over the last clotho_run: milliseconds summed over the
  * generations for [0] whole generations, [1] draw, [2] recombine+mutate,
- * [3] free space, [4] fitness; returns the number of generations timed. */
+ * [3] traits, [4] free space, [5] fitness + commit; returns the number of
+ * generations timed (eager runs; a graph run fills [0] only). */
 int clotho_timings(clotho_ctx* ctx, double* ms, int cap);
 void clotho_set_timing(clotho_ctx* ctx, int enabled);
 
@@ -173,6 +188,22 @@ int clotho_trait_contract(const uint64_t* rows, uint32_t haplotypes, uint32_t wo
                           const double* effects, uint32_t sites, uint32_t traits,
                           int mode, int split_terms, int repeats, double* z,
                           double* avg_ms, int device);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* STREAM-style device-to-device copy of `bytes` (read + write counted);
+ * no reference counterpart: it measures the HBM bandwidth the roofline is
+ * quoted against and calibrates hardware byte counters. */
+int clotho_stream_copy(size_t bytes, int repeats, double* gbps, int device);
+
+/* Same contraction over a genotype matrix drawn on the device: bit j of row
+ * r is set when Philox word (seed, generation 0, stream 7, replicate 0,
+ * index r, sub j/4)[j%4] < floor(freqs[j] * 2^32); rows are
+ * ceil(sites/512)*8 words wide.  No reference counterpart (synthetic
+ * workloads for measuring the contraction at scale). */
+int clotho_trait_contract_synthetic(uint32_t haplotypes, uint32_t sites, const double* freqs,
+                                    uint64_t seed, const double* effects, uint32_t traits,
+                                    int mode, int split_terms, int repeats, double* z,
+                                    double* avg_ms, int device);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,8 @@ def lib():
         getattr(L, name).restype = C.c_double
     L.or_poisson_table.argtypes = [C.c_double, f64p, C.c_int]
     L.or_poisson_table.restype = C.c_int
+    L.or_synth_rows.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                C.c_uint32, u64p]
     L.or_crossover.argtypes = [u64p, u64p, C.c_uint32, u64p, C.c_uint32, f64p, C.c_uint32,
                                f64p, C.c_uint32]
     L.or_free_space.argtypes = [u64p, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
@@ -122,6 +124,25 @@ def poisson_table(lam, cap=128):
     if n < 0:
         raise ValueError("poisson table: lambda out of range")
     return out[:n].copy()
+
+
+def freq_thresholds(freqs):
+    """floor(f * 2^32) per slot, saturated at 2^32 - 1 (the device rule)."""
+    f = np.clip(np.asarray(freqs, np.float64), 0.0, 1.0)
+    thr = np.floor(f * 2.0**32)
+    thr = np.where(f >= 1.0, 2.0**32 - 1, np.minimum(thr, 2.0**32 - 1))
+    return np.ascontiguousarray(thr.astype(np.uint32))
+
+
+def synth_rows(seed, freqs, r0, r1, replicate=0, words=None):
+    """Rows [r0, r1) of the device-drawn synthetic genotype matrix."""
+    thr = freq_thresholds(freqs)
+    sites = thr.size
+    if words is None:
+        words = ((sites + 511) // 512) * 8
+    out = np.zeros((r1 - r0, words), np.uint64)
+    lib().or_synth_rows(seed, replicate, thr, sites, words, r0, r1, out)
+    return out
 
 
 def crossover(top, bottom, wc, positions, events):

@@ -240,10 +240,11 @@ inline uint32_t poisson_draw(const std::vector<double>& cdf, double u) {
 
 inline int ctz64(uint64_t x) { return __builtin_ctzll(x); }
 
-// Effects are integer multiples of 2^q: round-half-even of (mean + sigma z) 2^-q.
+// Effects are integer multiples of 2^q: round-half-even of (mean + sigma z) 2^-q,
+// clamped to |n| <= 2^30 - 1.
 inline double quantize_effect(double w, double scale, double quantum) {
     double n = std::nearbyint(w * scale);
-    const double lim = 2147483647.0;
+    const double lim = 1073741823.0;
     if (n > lim) n = lim;
     if (n < -lim) n = -lim;
     return n * quantum;
@@ -296,6 +297,22 @@ extern "C" double or_det_recip(double x) { return det_recip(x); }
 extern "C" double or_normal_ppf(double p) { return normal_ppf(p); }
 extern "C" int or_poisson_table(double lambda, double* cdf, int cap) {
     return poisson_table(lambda, cdf, cap);
+}
+
+// Synthetic genotype rows [r0, r1): bit j of row r set when Philox word
+// (seed, generation 0, stream 7, replicate, index r, sub j/4)[j%4] < thr[j].
+extern "C" void or_synth_rows(uint64_t seed, uint32_t replicate, const uint32_t* thr,
+                              uint32_t sites, uint32_t words, uint32_t r0, uint32_t r1,
+                              uint64_t* out) {
+    for (uint32_t r = r0; r < r1; ++r) {
+        uint64_t* row = out + (size_t)(r - r0) * words;
+        std::memset(row, 0, 8 * (size_t)words);
+        for (uint32_t j0 = 0; j0 < sites; j0 += 4) {
+            Draw d = draw(seed, 0, 7, replicate, r, j0 / 4);
+            for (uint32_t k = 0; k < 4 && j0 + k < sites; ++k)
+                if (d.x[k] < thr[j0 + k]) row[(j0 + k) / 64] |= (uint64_t)1 << ((j0 + k) % 64);
+        }
+    }
 }
 
 // block_crossover_method::operator() (DS/crossover/block_crossover_method.hpp:36-70)
@@ -561,6 +578,12 @@ extern "C" int or_sim_set_population(or_sim* s, const uint64_t* rows, uint32_t w
     s->pos.assign(positions, positions + alleles);
     s->eff.assign(effects, effects + (size_t)alleles * s->T);
     s->age.assign(alleles, 0);
+    s->all_neutral = true;
+    for (double w : s->eff)
+        if (w != 0.0) {
+            s->all_neutral = false;
+            break;
+        }
     s->traits.assign((size_t)s->H * s->T, 0.0);
     or_phenotype(s->child.data(), s->H, words, s->eff.data(), alleles, s->T, s->traits.data());
     if (fitness) {

@@ -34,6 +34,10 @@ double or_det_recip(double x);
 double or_normal_ppf(double p);
 int or_poisson_table(double lambda, double* cdf, int cap);
 
+/* Synthetic genotype rows [r0, r1) of the device generator (stream 7). */
+void or_synth_rows(uint64_t seed, uint32_t replicate, const uint32_t* thr, uint32_t sites,
+                   uint32_t words, uint32_t r0, uint32_t r1, uint64_t* out);
+
 /* Per-phase restatements on caller buffers. */
 /* block_crossover_method over one gamete: child[Wc] from top/bottom rows of
  * width Wp, sorted or unsorted event positions, allele positions. */

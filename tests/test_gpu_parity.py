@@ -43,3 +43,70 @@ def test_graph_replay_matches_eager():
     cfg = api.make_config(500, 2.0, 2.0, seed=3, traits=1, neutral_p=0.5, sigma=0.2,
                           fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 14)
     run_pair(cfg, 12, check_every=3, graph=True)
+
+
+def _uploaded_population(n, alleles, traits, seed):
+    rng = np.random.default_rng(seed)
+    words = ((alleles + 511) // 512) * 8
+    freqs = rng.uniform(0, 1, alleles) ** 3
+    bits = (rng.uniform(0, 1, (2 * n, words * 64)) < np.pad(freqs, (0, words * 64 - alleles))[None, :])
+    bits[:, rng.integers(0, alleles, 3)] = 1  # fixed columns
+    rows = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64)
+    pos = np.floor(rng.uniform(0, 1, alleles) * 2**32) / 2**32
+    eff = np.round(rng.normal(0, 0.2, (alleles, traits)) * 2**24) / 2**24
+    fit = rng.uniform(0.1, 1.0, n)
+    return rows, pos, eff, fit
+
+
+def test_uploaded_population_with_fixed_columns():
+    cfg = api.make_config(150, 3.0, 4.0, seed=13, traits=2, neutral_p=0.4, sigma=0.2,
+                          fitness=api.FIT_MV_STABILIZING, fitness_scale=3.0, max_alleles=1 << 13)
+    run_pair(cfg, 6, check_every=1, population=_uploaded_population(150, 900, 2, 1))
+
+
+def test_many_events_and_strand_bias_edges():
+    for bias in (0.0, 1.0):
+        cfg = api.make_config(120, 1.5, 25.0, seed=17, traits=1, neutral_p=0.5, sigma=0.3,
+                              bias=bias, fitness=api.FIT_QUADRATIC, fitness_scale=2.0,
+                              max_alleles=1 << 13)
+        run_pair(cfg, 8, check_every=2)
+
+
+def test_no_mutation_no_recombination():
+    rows, pos, eff, fit = _uploaded_population(100, 600, 1, 2)
+    for mu, rho in ((0.0, 3.0), (2.0, 0.0)):
+        cfg = api.make_config(100, mu, rho, seed=19, traits=1, neutral_p=0.0, sigma=0.1,
+                              fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 12)
+        run_pair(cfg, 5, check_every=1, population=(rows, pos, eff, fit))
+
+
+def test_single_individual():
+    cfg = api.make_config(1, 2.0, 2.0, seed=23, traits=1, neutral_p=0.5, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 12)
+    run_pair(cfg, 10, check_every=5)
+
+
+def test_capacity_overflow_is_an_error():
+    cfg = api.make_config(500, 5.0, 1.0, seed=29, traits=1, neutral_p=1.0, max_alleles=2048)
+    sim = api.Simulation(cfg)
+    with pytest.raises(api.ClothoError, match="capacity"):
+        sim.run(3)
+
+
+def test_synthetic_population_bits():
+    import oracle
+
+    cfg = api.make_config(64, 1.0, 1.0, seed=31, traits=1, max_alleles=1 << 12)
+    sim = api.Simulation(cfg)
+    rng = np.random.default_rng(4)
+    freqs = rng.uniform(0, 1, 700)
+    pos = np.floor(rng.uniform(0, 1, 700) * 2**32) / 2**32
+    sim.set_synthetic(freqs, pos, np.zeros((700, 1)), seed=77)
+    rows = sim.rows()
+    thr = np.floor(freqs * 2**32)
+    for r in (0, 5, 127):
+        bits = np.unpackbits(rows[r].view(np.uint8), bitorder="little")
+        for j in range(0, 700, 37):
+            d = oracle.draw(77, 0, 7, 0, r, j // 4)
+            assert bits[j] == (d[j % 4] < thr[j])
+        assert not bits[700:].any()
