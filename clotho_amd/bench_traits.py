@@ -7,7 +7,8 @@ Reports kernel time of the plain FP32 set-bit walk, the FP64 walk and the
 split-BF16 MFMA contraction (1, 2, 3 BF16 terms, FP32 accumulation per K
 chunk, FP64 chunk reduction), the MFMA/plain speedup, and per-trait error
 against an FP64 reference: normwise max_i |z_i - z*_i| / max_i |z*_i| and the
-median elementwise relative error.  The FP64 reference is the GPU FP64 walk
+median elementwise relative error.  "plain" is the dense bit x effect FP32
+kernel (lane = haplotype); "walk" is the sparse set-bit walk.  The FP64 reference is the GPU FP64 walk
 over all rows (FP64 sums; the GPU tests pin it against numpy float64 on rows
 rebuilt from the same generator).  One JSON line on stdout.
 """
@@ -61,6 +62,10 @@ def main():
     n32, m32 = errors(z32, z64)
     out["plain_fp32_ms"] = ms32
     out["plain_fp32_err_normwise_max"] = float(n32.max())
+    zw, msw = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_WALK_FP32, seed=42,
+                                           repeats=1)
+    out["walk_fp32_ms"] = msw
+    out["walk_fp32_err_normwise_max"] = float(errors(zw, z64)[0].max())
     for terms in (1, 2, 3):
         zm, msm = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_MFMA_BF16, seed=42,
                                                split_terms=terms, repeats=args.repeats)

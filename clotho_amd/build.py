@@ -39,7 +39,24 @@ def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     subprocess.check_call(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    build_host_demo(verbose)
     return OUT
+
+
+DEMO = os.path.join(HERE, "lib", "engine_demo")
+
+
+def build_host_demo(verbose: bool = False) -> str:
+    """The C++ adapter driver: plain g++ against the C ABI library."""
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
+           os.path.join(HERE, "host", "engine_demo.cpp"), "-o", DEMO + ".tmp",
+           "-L", os.path.dirname(OUT), "-lclotho_amd", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(DEMO + ".tmp", DEMO)
+    return DEMO
 
 
 if __name__ == "__main__":

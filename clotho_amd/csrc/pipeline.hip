@@ -53,8 +53,6 @@ __device__ inline T block_inclusive_scan(T v, T* lds, T& total) {
     return off + x;
 }
 
-__device__ inline uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
 // ---------------------------------------------------------------------------
 // Draw phase
 // ---------------------------------------------------------------------------

@@ -58,7 +58,59 @@ __global__ void __launch_bounds__(kBlock) k_traits_walk(const uint64_t* __restri
     }
 }
 
+// Dense popcount-weighted FP32 contraction: lane = haplotype row, effects of
+// a 64-site block staged in LDS and broadcast, z_t += bit * w_t for every
+// site (the reference's per-sequence bit x effect accumulation,
+// CU/phenotype/translation_kernels.hpp:123-181, generalised to T traits).
+template <int TMAX>
+__global__ void __launch_bounds__(kBlock) k_traits_dense(const uint64_t* __restrict__ rows,
+                                                         uint32_t h, uint32_t w,
+                                                         const float* __restrict__ eff,
+                                                         uint32_t sites, uint32_t traits,
+                                                         float* __restrict__ z) {
+    __shared__ float s_eff[64 * TMAX];
+    const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = row < h;
+    const uint64_t* r = rows + (size_t)(valid ? row : 0) * w;
+    float acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = 0.f;
+    for (uint32_t k = 0; k < w; ++k) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 64 * TMAX; i += kBlock) {
+            const uint32_t j = k * 64 + i / TMAX, t = i % TMAX;
+            s_eff[i] = (j < sites && t < traits) ? eff[(size_t)j * traits + t] : 0.f;
+        }
+        __syncthreads();
+        const uint64_t x = valid ? r[k] : 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const float bit = (float)((uint32_t)(x >> j) & 1u);
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t) acc[t] += bit * s_eff[j * TMAX + t];
+        }
+    }
+    if (valid)
+        for (uint32_t t = 0; t < traits && t < (uint32_t)TMAX; ++t)
+            z[(size_t)row * traits + t] = acc[t];
+}
+
 }  // namespace
+
+hipError_t launch_traits_dense_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
+                                    const float* eff, uint32_t sites, uint32_t traits, float* z,
+                                    hipStream_t s) {
+    const uint32_t blocks = (h + kBlock - 1) / kBlock;
+    if (traits <= 1)
+        hipLaunchKernelGGL((k_traits_dense<1>), dim3(blocks), dim3(kBlock), 0, s, rows, h, w, eff,
+                           sites, traits, z);
+    else if (traits <= 4)
+        hipLaunchKernelGGL((k_traits_dense<4>), dim3(blocks), dim3(kBlock), 0, s, rows, h, w, eff,
+                           sites, traits, z);
+    else
+        hipLaunchKernelGGL((k_traits_dense<16>), dim3(blocks), dim3(kBlock), 0, s, rows, h, w, eff,
+                           sites, traits, z);
+    return hipGetLastError();
+}
 
 hipError_t launch_traits_walk_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
                                    const float* eff, uint32_t sites, uint32_t traits, float* z,
@@ -120,7 +172,7 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
         (size_t)w * 64 < sites || repeats < 1)
         return CLOTHO_ERR_ARG;
     if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
-        mode != CLOTHO_TRAITS_MFMA_BF16)
+        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32)
         return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;
@@ -167,7 +219,7 @@ extern "C" int clotho_trait_contract_synthetic(uint32_t h, uint32_t sites, const
         repeats < 1)
         return CLOTHO_ERR_ARG;
     if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
-        mode != CLOTHO_TRAITS_MFMA_BF16)
+        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32)
         return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;

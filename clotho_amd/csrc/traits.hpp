@@ -11,6 +11,9 @@
 
 namespace clotho {
 
+hipError_t launch_traits_dense_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
+                                    const float* eff, uint32_t sites, uint32_t traits, float* z,
+                                    hipStream_t s);
 hipError_t launch_traits_walk_fp32(const uint64_t* rows, uint32_t h, uint32_t w,
                                    const float* eff, uint32_t sites, uint32_t traits, float* z,
                                    hipStream_t s);
@@ -60,7 +63,7 @@ struct TraitWorkspace {
         if (rows && hipMemcpyAsync(d_rows, rows, sizeof(uint64_t) * h * w,
                                    hipMemcpyHostToDevice, st) != hipSuccess)
             return CLOTHO_ERR_HIP;
-        if (mode == CLOTHO_TRAITS_PLAIN_FP32) {
+        if (mode == CLOTHO_TRAITS_PLAIN_FP32 || mode == CLOTHO_TRAITS_WALK_FP32) {
             std::vector<float> e((size_t)sites * traits);
             for (size_t i = 0; i < e.size(); ++i) e[i] = (float)eff[i];
             if (hipMalloc((void**)&d_eff32, 4 * e.size()) != hipSuccess ||
@@ -92,6 +95,8 @@ struct TraitWorkspace {
 
     hipError_t run(hipStream_t st) {
         if (mode == CLOTHO_TRAITS_PLAIN_FP32)
+            return launch_traits_dense_fp32(d_rows, h, w, d_eff32, sites, traits, d_z32, st);
+        if (mode == CLOTHO_TRAITS_WALK_FP32)
             return launch_traits_walk_fp32(d_rows, h, w, d_eff32, sites, traits, d_z32, st);
         if (mode == CLOTHO_TRAITS_EXACT_FP64)
             return launch_traits_walk_fp64(d_rows, h, w, d_eff64, sites, traits, d_z64, st);
@@ -102,7 +107,7 @@ struct TraitWorkspace {
     int download(double* z, hipStream_t st) {
         const size_t n = (size_t)h * traits;
         if (hipStreamSynchronize(st) != hipSuccess) return CLOTHO_ERR_HIP;
-        if (mode == CLOTHO_TRAITS_PLAIN_FP32) {
+        if (mode == CLOTHO_TRAITS_PLAIN_FP32 || mode == CLOTHO_TRAITS_WALK_FP32) {
             std::vector<float> tmp(n);
             if (hipMemcpy(tmp.data(), d_z32, 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
                 return CLOTHO_ERR_HIP;

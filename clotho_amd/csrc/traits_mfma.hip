@@ -194,6 +194,14 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
     const uint32_t base = (blockIdx.x * 4 + wave) * (16 * RT);
     const uint32_t G0 = blockIdx.y * kChunkGroups;
     const uint32_t G1 = min(NG, G0 + kChunkGroups);
+    // byte -> 8 BF16 (0.0 / 1.0) lookup, 4 KB of LDS
+    __shared__ uint4 s_lut[256];
+    {
+        const uint32_t b8 = threadIdx.x;
+        s_lut[b8] = make_uint4(bits2_bf16(b8 & 3), bits2_bf16((b8 >> 2) & 3),
+                               bits2_bf16((b8 >> 4) & 3), bits2_bf16(b8 >> 6));
+    }
+    __syncthreads();
     double acc64[RT][4];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
@@ -236,8 +244,7 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
                     const uint32_t sel = (q >> 2) & 3;
                     const uint32_t dw = sel == 0 ? v.x : sel == 1 ? v.y : sel == 2 ? v.z : v.w;
                     const uint32_t b8 = (dw >> (8 * (q & 3))) & 0xFFu;
-                    uint4 u = make_uint4(bits2_bf16(b8 & 3), bits2_bf16((b8 >> 2) & 3),
-                                         bits2_bf16((b8 >> 4) & 3), bits2_bf16(b8 >> 6));
+                    const uint4 u = s_lut[b8];
                     __builtin_memcpy(&a[r], &u, 16);
                 }
 #pragma unroll
@@ -372,16 +379,16 @@ hipError_t launch_traits_mfma(const uint64_t* rows, uint32_t h, uint32_t w,
                               double* partial, double* z, hipStream_t s) {
     (void)sites;
     const uint32_t chunks = ((w + 15) / 16 + kChunkGroups - 1) / kChunkGroups;
-    const dim3 grid((h + 4 * 32 - 1) / (4 * 32), chunks);
+    const dim3 grid((h + 4 * 64 - 1) / (4 * 64), chunks);
     const uint4* pk = reinterpret_cast<const uint4*>(packed);
     if (terms == 1)
-        hipLaunchKernelGGL((k_traits_bf16<2, 1>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+        hipLaunchKernelGGL((k_traits_bf16<4, 1>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
                            partial);
     else if (terms == 2)
-        hipLaunchKernelGGL((k_traits_bf16<2, 2>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+        hipLaunchKernelGGL((k_traits_bf16<4, 2>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
                            partial);
     else
-        hipLaunchKernelGGL((k_traits_bf16<2, 3>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
+        hipLaunchKernelGGL((k_traits_bf16<4, 3>), grid, dim3(256), 0, s, rows, h, w, traits, pk,
                            partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

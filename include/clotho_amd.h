@@ -76,9 +76,10 @@ extern "C" {
 #define CLOTHO_FIT_MV_STABILIZING 4 /* exp(-|x|^2/(2 scale^2)) over all traits           */
 
 /* Trait contraction arithmetic for clotho_trait_contract. */
-#define CLOTHO_TRAITS_PLAIN_FP32 0 /* popcount-walk over set bits, FP32 sums            */
+#define CLOTHO_TRAITS_PLAIN_FP32 0 /* dense bit x effect FMAs, FP32 (lane = haplotype)   */
 #define CLOTHO_TRAITS_MFMA_BF16 1  /* bits -> BF16, effects split into BF16 terms, MFMA */
-#define CLOTHO_TRAITS_EXACT_FP64 2 /* set-bit walk, FP64 sums in ascending bit order    */
+#define CLOTHO_TRAITS_EXACT_FP64 2 /* set-bit walk, FP64 sums                          */
+#define CLOTHO_TRAITS_WALK_FP32 3  /* set-bit walk, FP32 sums                          */
 
 typedef struct clotho_config {
     uint64_t seed;            /* random_number.seed (Philox key)                         */

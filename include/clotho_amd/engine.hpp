@@ -1,0 +1,509 @@
+// clotho_amd/engine.hpp — C++17 host adapters over the C ABI (clotho_amd.h)
+// that keep the reference's names and call shapes for the offspring path, so
+// a caller written against Engine<..., parallel_pipeline> switches by type.
+//
+// Reference families mirrored (DS/ = include/clotho/data_spaces/,
+// QS/ = examples/QTLSim/):
+//   property_tree          boost::property_tree::ptree keys used by the path
+//                          (src/clotho/utility/clotho_strings.cpp:16-47)
+//   ifitness, ifitness_generator, fitness_toolkit
+//                          include/clotho/fitness/ifitness.hpp:21-38,
+//                          include/clotho/fitness/ifitness_generator.hpp:22-44,
+//                          src/clotho/fitness/fitness_toolkit.cpp:50-78
+//   mutation_allocator     DS/mutation/mutation_allocator.hpp:25-55
+//   common_crossover       DS/crossover/common_crossover_row_block_alignment.hpp:29-100
+//   free_space_accumulator_mt + free_space_details accessors
+//                          DS/free_space/free_space_accumulator.hpp:53-197,
+//                          DS/free_space/free_space_details.hpp:193-277
+//   GeneralFitness         DS/fitness/general_fitness.hpp:30-284
+//   Engine<..., mi355x_pipeline>
+//                          QS/engine_parallel_pipeline.hpp:59-483
+//
+// The per-gamete and per-thread granularity of the reference collapses into
+// whole-generation device phases: each adapter delegates to the phase entry
+// point that covers its work.  Errors from the C ABI throw std::runtime_error
+// (the reference asserts).
+#ifndef CLOTHO_AMD_ENGINE_HPP_
+#define CLOTHO_AMD_ENGINE_HPP_
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <ostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../clotho_amd.h"
+
+namespace clotho_amd {
+
+// ---------------------------------------------------------------------------
+// configuration: the dotted keys the reference reads from its JSON property
+// tree ("configuration." prefix optional), with its defaults.
+// ---------------------------------------------------------------------------
+class property_tree {
+public:
+    void put(const std::string& key, const std::string& value) { m_kv[strip(key)] = value; }
+    void put(const std::string& key, double value) {
+        std::ostringstream os;
+        os.precision(17);
+        os << value;
+        m_kv[strip(key)] = os.str();
+    }
+    bool has(const std::string& key) const { return m_kv.count(strip(key)) != 0; }
+    std::string get(const std::string& key, const std::string& dflt) const {
+        auto it = m_kv.find(strip(key));
+        return it == m_kv.end() ? dflt : it->second;
+    }
+    double get(const std::string& key, double dflt) const {
+        auto it = m_kv.find(strip(key));
+        return it == m_kv.end() ? dflt : std::stod(it->second);
+    }
+    const std::map<std::string, std::string>& items() const { return m_kv; }
+
+private:
+    static std::string strip(const std::string& k) {
+        const std::string p = "configuration.";
+        return k.compare(0, p.size(), p) == 0 ? k.substr(p.size()) : k;
+    }
+    std::map<std::string, std::string> m_kv;
+};
+
+inline void check(int rc, const clotho_ctx* ctx, const char* what) {
+    if (rc != CLOTHO_OK) {
+        std::ostringstream os;
+        os << what << " failed (" << rc << ")";
+        if (ctx) os << ": " << clotho_last_error(ctx);
+        throw std::runtime_error(os.str());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fitness plug-ins: ifitness / ifitness_generator / fitness_toolkit
+// ---------------------------------------------------------------------------
+struct ifitness {
+    typedef double result_type;
+    virtual double operator()(const std::vector<double>& pheno) = 0;
+    virtual double operator()(double* first, double* last) = 0;
+    virtual const std::string name() const = 0;
+    virtual void log(std::ostream& out) const = 0;
+    virtual ~ifitness() {}
+};
+
+struct ifitness_generator {
+    virtual std::shared_ptr<ifitness_generator> create(property_tree& config) const = 0;
+    virtual std::shared_ptr<ifitness> generate(size_t N) = 0;
+    virtual const std::string name() const = 0;
+    virtual void log(std::ostream& out) const = 0;
+    // the device metric this generator runs as
+    virtual uint32_t device_kind() const = 0;
+    virtual double device_scale() const { return 1.0; }
+    virtual ~ifitness_generator() {}
+};
+
+namespace detail {
+// Host-side evaluation of a device metric (for callers that score a single
+// phenotype); the device evaluates the same formula per individual.
+class metric : public ifitness {
+public:
+    metric(std::string n, std::function<double(const double*, const double*)> f)
+        : m_name(std::move(n)), m_f(std::move(f)) {}
+    double operator()(const std::vector<double>& p) override {
+        return m_f(p.data(), p.data() + p.size());
+    }
+    double operator()(double* first, double* last) override { return m_f(first, last); }
+    const std::string name() const override { return m_name; }
+    void log(std::ostream& out) const override { out << "{" << m_name << "}"; }
+
+private:
+    std::string m_name;
+    std::function<double(const double*, const double*)> m_f;
+};
+
+class generator : public ifitness_generator {
+public:
+    typedef std::function<std::shared_ptr<ifitness>(const property_tree&, size_t)> factory;
+    generator(std::string n, uint32_t kind, factory f, property_tree cfg = property_tree())
+        : m_name(std::move(n)), m_kind(kind), m_factory(std::move(f)), m_cfg(std::move(cfg)) {}
+    std::shared_ptr<ifitness_generator> create(property_tree& config) const override {
+        return std::make_shared<generator>(m_name, m_kind, m_factory, config);
+    }
+    std::shared_ptr<ifitness> generate(size_t N) override { return m_factory(m_cfg, N); }
+    const std::string name() const override { return m_name; }
+    void log(std::ostream& out) const override { out << "{" << m_name << "_gen}"; }
+    uint32_t device_kind() const override { return m_kind; }
+    double device_scale() const override { return m_cfg.get("fitness.params.scale", 1.0); }
+
+private:
+    std::string m_name;
+    uint32_t m_kind;
+    factory m_factory;
+    property_tree m_cfg;
+};
+}  // namespace detail
+
+// String-keyed registry of fitness generators, selected by "fitness.name".
+class fitness_toolkit {
+public:
+    static fitness_toolkit* getInstance() {
+        static fitness_toolkit inst;
+        return &inst;
+    }
+    void register_tool(std::shared_ptr<ifitness_generator> gen) { m_tools[gen->name()] = gen; }
+    std::shared_ptr<ifitness_generator> get_tool(property_tree& config) const {
+        const std::string n = config.get("fitness.name", std::string(""));
+        auto it = m_tools.find(n);
+        if (it == m_tools.end()) return nullptr;
+        return it->second->create(config);
+    }
+    std::vector<std::string> tool_names() const {
+        std::vector<std::string> v;
+        for (auto& kv : m_tools) v.push_back(kv.first);
+        return v;
+    }
+
+private:
+    fitness_toolkit() {
+        using detail::generator;
+        using detail::metric;
+        // "gaussian": sigma = sqrt(4 N mu), (1/(sigma sqrt(2 pi))) exp(-x^2/(2 sigma^2)),
+        // first trait (src/clotho/fitness/normal_fitness_generator.cpp:36-52)
+        register_tool(std::make_shared<generator>(
+            "gaussian", CLOTHO_FIT_GAUSSIAN, [](const property_tree& c, size_t N) {
+                const double s = std::sqrt(4.0 * (double)N * c.get("mutation.mu", 1e-4));
+                return std::make_shared<metric>("gaussian", [s](const double* a, const double* b) {
+                    const double x = a == b ? 0.0 : *a;
+                    return (1.0 / s) * 0.398942280401432677939946059934 *
+                           std::exp(-(x * x) / (2.0 * s * s));
+                });
+            }));
+        // "quadratic": s = scale sqrt(4 N mu), max(0, 1 - (x/s)^2)
+        // (src/clotho/fitness/quadratic_fitness_generator.cpp:42-54)
+        register_tool(std::make_shared<generator>(
+            "quadratic", CLOTHO_FIT_QUADRATIC, [](const property_tree& c, size_t N) {
+                const double s = c.get("fitness.params.scale", 1.0) *
+                                 std::sqrt(4.0 * (double)N * c.get("mutation.mu", 1e-4));
+                return std::make_shared<metric>("quadratic", [s](const double* a, const double* b) {
+                    const double x = a == b ? 0.0 : *a;
+                    const double r = (x / s) * (x / s);
+                    return r >= 1.0 ? 0.0 : 1.0 - r;
+                });
+            }));
+        register_tool(std::make_shared<generator>(
+            "constant", CLOTHO_FIT_CONSTANT, [](const property_tree&, size_t) {
+                return std::make_shared<metric>("constant",
+                                                [](const double*, const double*) { return 1.0; });
+            }));
+        // fixed-sigma Gaussian exp(-x^2/(2 scale^2)) and multivariate stabilising
+        // exp(-|x|^2/(2 scale^2)): new generators behind the same interface
+        register_tool(std::make_shared<generator>(
+            "gaussian_fixed", CLOTHO_FIT_GAUSSIAN_FIXED, [](const property_tree& c, size_t) {
+                const double s = c.get("fitness.params.scale", 1.0);
+                return std::make_shared<metric>(
+                    "gaussian_fixed", [s](const double* a, const double* b) {
+                        const double x = a == b ? 0.0 : *a;
+                        return std::exp(-(x * x) / (2.0 * s * s));
+                    });
+            }));
+        register_tool(std::make_shared<generator>(
+            "mv_stabilizing", CLOTHO_FIT_MV_STABILIZING, [](const property_tree& c, size_t) {
+                const double s = c.get("fitness.params.scale", 1.0);
+                return std::make_shared<metric>(
+                    "mv_stabilizing", [s](const double* a, const double* b) {
+                        double ss = 0.0;
+                        for (const double* p = a; p != b; ++p) ss += *p * *p;
+                        return std::exp(-ss / (2.0 * s * s));
+                    });
+            }));
+    }
+    std::map<std::string, std::shared_ptr<ifitness_generator>> m_tools;
+};
+
+// ---------------------------------------------------------------------------
+// device population: owns the C ABI context
+// ---------------------------------------------------------------------------
+// Population size from population_growth: the reference starts from
+// pN = growth(0, gen) (QS/engine_parallel_pipeline.hpp:150-153); "linear" is
+// pN' = A pN + B (src/clotho/genetics/linear_population_growth.cpp:23-25), so
+// A = 0 gives the constant size B, the only schedule the device pipeline runs.
+inline uint32_t population_size(const property_tree& config) {
+    if (config.has("population_growth.size"))
+        return (uint32_t)config.get("population_growth.size", 0.0);
+    const std::string name = config.get("population_growth.name", std::string("linear"));
+    if (name != "linear")
+        throw std::runtime_error("population_growth." + name + " is not supported (constant N only)");
+    const double A = config.get("population_growth.params.A", config.get("population_growth.A", 1.0));
+    const double B = config.get("population_growth.params.B", config.get("population_growth.B", 0.0));
+    if (A != 0.0 || !(B >= 1.0))
+        throw std::runtime_error("population_growth.linear must have A = 0 and B >= 1 (constant N)");
+    return (uint32_t)B;
+}
+
+inline int device_of(const property_tree& config) { return (int)config.get("device", 0.0); }
+
+inline clotho_config make_config(property_tree& config) {
+    clotho_config c{};
+    c.seed = (uint64_t)config.get("random_number.seed", 0.0);
+    c.replicate = (uint32_t)config.get("replicate", 0.0);
+    c.n_individuals = population_size(config);
+    c.trait_count = (uint32_t)config.get("traits.N", 1.0);
+    c.mu = config.get("mutation.mu", 1e-4);
+    c.rho = config.get("recombination.rho", 1e-4);
+    c.bias = config.get("recombination.sequence_bias.p", 0.5);
+    c.neutral_p = config.get("neutral.p", 0.5);
+    c.trait_mean = config.get("traits.mean", 0.0);
+    c.trait_sigma = config.get("traits.sigma", 1.0);
+    c.env_sigma = config.get("environment.sigma", 0.0);
+    c.effect_quantum_log2 = (int32_t)config.get("traits.quantum_log2", -24.0);
+    c.max_alleles = (uint32_t)config.get("alleles.capacity", 1048576.0);
+    auto gen = fitness_toolkit::getInstance()->get_tool(config);
+    c.fitness_kind = gen ? gen->device_kind() : CLOTHO_FIT_CONSTANT;
+    c.fitness_scale = gen ? gen->device_scale() : 1.0;
+    return c;
+}
+
+class device_population {
+public:
+    explicit device_population(const clotho_config& cfg, int device = 0) : m_cfg(cfg) {
+        check(clotho_create(&cfg, device, &m_ctx), nullptr, "clotho_create");
+    }
+    ~device_population() { clotho_destroy(m_ctx); }
+    device_population(const device_population&) = delete;
+    device_population& operator=(const device_population&) = delete;
+
+    clotho_ctx* ctx() const { return m_ctx; }
+    const clotho_config& config() const { return m_cfg; }
+
+    clotho_shape shape() const {
+        clotho_shape s{};
+        check(clotho_get_shape(m_ctx, &s), m_ctx, "clotho_get_shape");
+        return s;
+    }
+    // population_space accessors
+    unsigned int getIndividualCount() const { return shape().n_individuals; }
+    unsigned int haploid_genome_count() const { return shape().haplotypes; }
+    unsigned int getMaxBlocks() const { return shape().words; }
+    unsigned int getMaxAlleles() const { return shape().alleles; }
+    std::vector<uint64_t> rows() const {
+        const clotho_shape s = shape();
+        std::vector<uint64_t> r((size_t)s.haplotypes * s.words);
+        check(clotho_download_population(m_ctx, r.data(), r.size()), m_ctx, "download");
+        return r;
+    }
+    std::vector<double> genome_traits() const {
+        std::vector<double> t((size_t)shape().haplotypes * m_cfg.trait_count);
+        check(clotho_download_traits(m_ctx, t.data(), t.size()), m_ctx, "download_traits");
+        return t;
+    }
+
+private:
+    clotho_config m_cfg;
+    clotho_ctx* m_ctx = nullptr;
+};
+
+// AlleleSpace<double, unsigned>-compatible view (positions, ages) plus effects.
+struct allele_space_view {
+    std::vector<double> positions, effects;
+    std::vector<uint32_t> ages;
+    size_t size() const { return positions.size(); }
+    double getAllelePosition(size_t i) const { return positions[i]; }
+    uint32_t getAlleleAge(size_t i) const { return ages[i]; }
+};
+
+// ---------------------------------------------------------------------------
+// operator families
+// ---------------------------------------------------------------------------
+
+// mutation_allocator<RNG, IntType>: the reference draws M ~ Poisson(mu * N) on
+// the host; here the per-gamete Poisson(mu) counts are drawn on the device in
+// the draw phase.  allocate(N) returns the expected count mu * N; the drawn M
+// of the last generation is new_mutations() (shape().new_mutations).
+template <class RNG, class IntType = size_t>
+class mutation_allocator {
+public:
+    mutation_allocator(RNG* rng, property_tree& config)
+        : m_rng(rng), m_mu(config.get("mutation.mu", 1e-4)) {}
+    IntType allocate(size_t N) const { return (IntType)std::llround(m_mu * (double)N); }
+
+private:
+    RNG* m_rng;
+    double m_mu;
+};
+
+// common_crossover<RNG, population, alleles>: recombination + mutation scatter
+// of every gamete of the generation (the draw phase supplies parents, events
+// and strand swaps).
+template <class RNG, class Population, class Alleles>
+class common_crossover {
+public:
+    common_crossover(RNG* rng, Alleles* alleles, double recomb_rate, double bias_rate)
+        : m_rng(rng), m_alleles(alleles), m_rho(recomb_rate), m_bias(bias_rate) {}
+    void operator()(Population& pop) {
+        check(clotho_phase_recombine_mutate(pop.ctx()), pop.ctx(), "phase_recombine_mutate");
+    }
+
+private:
+    RNG* m_rng;
+    Alleles* m_alleles;
+    double m_rho, m_bias;
+};
+
+// free_space_accumulator_mt<Block, Size> with the free_space_details accessors.
+template <class BlockType = uint64_t, class SizeType = unsigned int>
+class free_space_accumulator_mt {
+public:
+    typedef SizeType size_type;
+    typedef const size_type* iterator;
+    explicit free_space_accumulator_mt(unsigned int /*thread count*/) {}
+    void resetBuffers(unsigned int /*block_count*/) {}
+    // device column OR/AND partials were accumulated by the recombination
+    // phase; finalize reduces them and compacts the fixed and lost lists.
+    void finalize(device_population& pop) {
+        check(clotho_phase_free_space(pop.ctx()), pop.ctx(), "phase_free_space");
+        refresh(pop);
+    }
+    void refresh(device_population& pop) {
+        const clotho_shape s = pop.shape();
+        m_indices.assign(s.free_count, 0);
+        std::vector<uint32_t> fx(std::max<uint32_t>(s.fixed_count, 1)),
+            ls(std::max<uint32_t>(s.lost_count, 1));
+        check(clotho_download_free_space(pop.ctx(), fx.data(), ls.data(),
+                                         std::max(fx.size(), ls.size())),
+              pop.ctx(), "download_free_space");
+        for (uint32_t i = 0; i < s.fixed_count; ++i) m_indices[i] = fx[i];
+        for (uint32_t i = 0; i < s.lost_count; ++i) m_indices[s.fixed_count + i] = ls[i];
+        m_fixed = s.fixed_count;
+        m_lost = s.lost_count;
+        m_size = s.alleles;
+    }
+    size_type fixed_size() const { return m_fixed; }
+    size_type lost_size() const { return m_lost; }
+    size_type free_size() const { return m_fixed + m_lost; }
+    size_type variable_count() const { return m_size - free_size(); }
+    iterator fixed_begin() const { return m_indices.data(); }
+    iterator fixed_end() const { return m_indices.data() + m_fixed; }
+    iterator lost_begin() const { return m_indices.data() + m_fixed; }
+    iterator lost_end() const { return m_indices.data() + m_fixed + m_lost; }
+    iterator free_begin() const { return m_indices.data(); }
+    iterator free_end() const { return m_indices.data() + m_fixed + m_lost; }
+
+private:
+    std::vector<size_type> m_indices;
+    size_type m_fixed = 0, m_lost = 0, m_size = 0;
+};
+
+// GeneralFitness: phenotype (traits) + fitness of the child population.
+class GeneralFitness {
+public:
+    typedef double fitness_type;
+    typedef std::vector<double>::const_iterator const_iterator;
+    explicit GeneralFitness(property_tree& config)
+        : m_gen(fitness_toolkit::getInstance()->get_tool(config)) {}
+    void operator()(device_population& pop) {
+        check(clotho_phase_traits(pop.ctx()), pop.ctx(), "phase_traits");
+        check(clotho_phase_fitness(pop.ctx()), pop.ctx(), "phase_fitness");
+        m_cached = false;
+    }
+    const std::vector<double>& values(device_population& pop) {
+        if (!m_cached) {
+            m_fit.assign(pop.getIndividualCount(), 1.0);
+            check(clotho_download_fitness(pop.ctx(), m_fit.data(), m_fit.size()), pop.ctx(),
+                  "download_fitness");
+            m_cached = true;
+        }
+        return m_fit;
+    }
+    std::shared_ptr<ifitness_generator> generator() const { return m_gen; }
+
+private:
+    std::shared_ptr<ifitness_generator> m_gen;
+    std::vector<double> m_fit;
+    bool m_cached = false;
+};
+
+// ---------------------------------------------------------------------------
+// Engine<RNG, Real, Block, Size, mi355x_pipeline>
+// ---------------------------------------------------------------------------
+struct mi355x_pipeline {};
+
+template <class RNG, class RealType, class BlockType, class SizeType, class Mode>
+class Engine;
+
+template <class RNG, class RealType, class BlockType, class SizeType>
+class Engine<RNG, RealType, BlockType, SizeType, mi355x_pipeline> {
+public:
+    typedef device_population sequence_space_type;
+    typedef allele_space_view allele_type;
+    typedef free_space_accumulator_mt<BlockType, SizeType> free_space_type;
+    typedef common_crossover<RNG, device_population, allele_type> crossover_type;
+
+    Engine(RNG* rng, property_tree& config)
+        : m_rng(rng),
+          m_pop(make_config(config), device_of(config)),
+          m_fit(config),
+          m_free(1),
+          m_xover(rng, &m_alleles, config.get("recombination.rho", 1e-4),
+                  config.get("recombination.sequence_bias.p", 0.5)),
+          m_generation(1) {}
+
+    // one generation: draw, recombine+mutate, traits+fitness, free space, commit
+    void simulate() {
+        clotho_ctx* c = m_pop.ctx();
+        check(clotho_phase_draw(c), c, "phase_draw");
+        m_xover(m_pop);
+        m_fit(m_pop);
+        check(clotho_phase_free_space(c), c, "phase_free_space");
+        check(clotho_phase_commit(c), c, "phase_commit");
+        ++m_generation;
+    }
+    // many generations without host round trips (hipGraph replay when asked)
+    void simulate(unsigned int generations, bool graph) {
+        check(clotho_run(m_pop.ctx(), generations, graph ? 1 : 0), m_pop.ctx(), "clotho_run");
+        m_generation += generations;
+    }
+    size_t getGeneration() const { return m_generation; }
+    sequence_space_type* getChildPopulation() { return &m_pop; }
+    allele_type* getAlleleSpace() {
+        const clotho_shape s = m_pop.shape();
+        m_alleles.positions.assign(s.alleles, 0.0);
+        m_alleles.ages.assign(s.alleles, 0);
+        m_alleles.effects.assign((size_t)s.alleles * m_pop.config().trait_count, 0.0);
+        check(clotho_download_alleles(m_pop.ctx(), m_alleles.positions.data(),
+                                      m_alleles.effects.data(), m_alleles.ages.data(), s.alleles),
+              m_pop.ctx(), "download_alleles");
+        return &m_alleles;
+    }
+    free_space_type* getFreeSpace() {
+        m_free.refresh(m_pop);
+        return &m_free;
+    }
+    const std::vector<double>& getFitness() { return m_fit.values(m_pop); }
+    // performance.* of the reference, as per-phase milliseconds of the last
+    // simulate(n, false) run with timing enabled
+    std::map<std::string, double> getPerformanceResults() {
+        double ms[6] = {0, 0, 0, 0, 0, 0};
+        const int n = clotho_timings(m_pop.ctx(), ms, 6);
+        std::map<std::string, double> out;
+        const char* names[6] = {"generation", "draw", "crossover", "phenotype", "fixed",
+                                "fitness"};
+        for (int i = 0; i < 6; ++i) out[std::string("performance.") + names[i]] = n ? ms[i] / n : 0;
+        return out;
+    }
+
+private:
+    RNG* m_rng;
+    device_population m_pop;
+    allele_type m_alleles;
+    GeneralFitness m_fit;
+    free_space_type m_free;
+    crossover_type m_xover;
+    size_t m_generation;
+};
+
+}  // namespace clotho_amd
+
+#endif  // CLOTHO_AMD_ENGINE_HPP_

@@ -31,6 +31,8 @@ def test_contraction_modes(h, sites, t):
     assert np.max(normwise(z64, ref)) < 1e-13
     z32, _ = api.trait_contract(rows, eff, api.TRAITS_PLAIN_FP32)
     assert np.max(normwise(z32, ref)) < 1e-5
+    zw, _ = api.trait_contract(rows, eff, api.TRAITS_WALK_FP32)
+    assert np.max(normwise(zw, ref)) < 1e-5
     tol = {1: 1e-2, 2: 3e-5, 3: 1e-6}
     for terms in (1, 2, 3):
         zm, _ = api.trait_contract(rows, eff, api.TRAITS_MFMA_BF16, split_terms=terms)

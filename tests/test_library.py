@@ -1,0 +1,70 @@
+"""CPU checks of the C ABI library and the C++ adapters: the library loads
+without a GPU, exports every entry point include/clotho_amd.h declares,
+rejects invalid configurations before touching a device, and the adapter
+driver parses the reference's configuration keys."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from clotho_amd import api, build
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "clotho_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(clotho_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25
+    L = api.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(api.EXPORTS) == set(names)
+
+
+def test_invalid_configs_rejected_before_device():
+    L = api.lib()
+    ctx = C.c_void_p()
+    bad = [
+        api.make_config(0, 1.0, 1.0),                       # N = 0
+        api.make_config(10, 1.0, 1.0, bias=1.5),            # bias outside [0, 1]
+        api.make_config(10, 1.0, 1.0, neutral_p=-0.1),
+        api.make_config(10, 100.0, 1.0),                     # mu beyond the Poisson table
+        api.make_config(10, 1.0, 1.0, traits=33),            # trait capacity
+        api.make_config(10, 0.0, 1.0, fitness=api.FIT_GAUSSIAN),  # sigma = 0
+        api.make_config(10, 1.0, 1.0, quantum_log2=3),
+        api.make_config(10, 1.0, 1.0, max_alleles=0),
+    ]
+    for cfg in bad:
+        assert L.clotho_create(C.addressof(cfg), 0, C.byref(ctx)) == -1
+
+
+def test_poisson_tables_agree_with_oracle():
+    for lam in (0.0, 0.01, 1.0, 5.0, 10.0, 20.0, 40.0):
+        assert api.poisson_table(lam).tobytes() == oracle.poisson_table(lam).tobytes()
+
+
+def test_engine_adapter_registry_and_config():
+    demo = build.DEMO
+    if not os.path.exists(demo):
+        build.build_host_demo()
+    out = subprocess.run([demo, "--registry", "fitness.name=quadratic",
+                          "configuration.population_growth.params.B=250", "mutation.mu=0.5"],
+                         check=True, capture_output=True, text=True).stdout
+    d = json.loads(out)
+    assert set(d["fitness_tools"]) == {"constant", "gaussian", "quadratic", "gaussian_fixed",
+                                       "mv_stabilizing"}
+    assert d["n"] == 250 and d["mu"] == 0.5 and d["fitness_kind"] == api.FIT_QUADRATIC
+    r = subprocess.run([demo, "--registry", "population_growth.params.A=1.1"],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "constant N" in r.stderr
