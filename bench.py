@@ -141,6 +141,9 @@ def main():
 
     graph = not args.eager
     sim.run(args.warmup, graph=graph)
+    chunks = sim.autotune()
+    log(f"[bench] autotuned recombination row chunks: {chunks}")
+    sim.run(1, graph=graph)
     sim.sync()
     t0 = time.perf_counter()
     sim.run(args.steps, graph=graph)
@@ -184,7 +187,7 @@ def main():
                                "neutral 1/f standing variation, rho=5, mu=0.01/gamete, "
                                "1 trait N(0,0.1^2), env N(0,1), fitness exp(-z^2/2)",
                    "haplotypes": h, "words": words, "rho": RHO, "mu": MU,
-                   "graph": graph},
+                   "graph": graph, "row_chunks": chunks},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
                      "kernel": "k_recombine", "kernel_ms": rec_ms,

@@ -111,6 +111,7 @@ EXPORTS = [
     "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit", "clotho_sync",
     "clotho_timings", "clotho_set_timing", "clotho_trait_contract",
     "clotho_set_synthetic_population", "clotho_stream_copy", "clotho_trait_contract_synthetic",
+    "clotho_autotune",
 ]
 
 
@@ -148,6 +149,7 @@ def lib():
               "clotho_sync"):
         getattr(L, n).argtypes = [vp]
     L.clotho_timings.argtypes = [vp, f64p, C.c_int]
+    L.clotho_autotune.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.clotho_set_timing.argtypes = [vp, C.c_int]
     L.clotho_set_timing.restype = None
     L.clotho_trait_contract.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, C.c_uint32,
@@ -278,6 +280,12 @@ class Simulation:
                                                   ev.ctypes.data, mu.ctypes.data, h),
                     "download_gametes")
         return top, bot, ev, mu
+
+    def autotune(self):
+        """Pick the recombination row-chunk count by timing (state unchanged)."""
+        ch = C.c_uint32(0)
+        self._check(lib().clotho_autotune(self.ctx, C.byref(ch)), "autotune")
+        return ch.value
 
     def set_timing(self, on=True):
         lib().clotho_set_timing(self.ctx, 1 if on else 0)

@@ -63,7 +63,7 @@ void free_buffers(clotho_ctx* c) {
     void* ptrs[] = {B.rows[0],    B.rows[1],   B.traits[0], B.traits[1],  B.fitness[0],
                     B.fitness[1], B.keys,      B.effects,   B.ages,       B.free_list,
                     B.fixed_bits, B.mut_count, B.mut_local, B.mut_bsum,   B.top,
-                    B.sorted,     B.pm,        B.digits,
+                    B.sorted,     B.pm,        B.digits,    B.bucket,
                     B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
                     B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
                     B.cdf_rho,    B.cdf_mu,    B.state};
@@ -231,13 +231,16 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     }
     P.tiles = (P.w_cap + 63) / 64;
     {
-        const uint32_t target_blocks = 2048;
-        uint32_t ch = (target_blocks + P.tiles - 1) / P.tiles;
+        // default launch shape: ~4096 recombination blocks; clotho_autotune
+        // may pick another row-chunk count up to max_chunks
+        P.max_chunks = std::max<uint32_t>(1, std::min<uint32_t>(
+                                                 (8192 + P.tiles - 1) / P.tiles, P.h / 8));
+        uint32_t ch = (4096 + P.tiles - 1) / P.tiles;
         ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, P.h / 16));
-        P.chunks = std::max<uint32_t>(1, ch);
+        P.chunks = std::min<uint32_t>(P.max_chunks, std::max<uint32_t>(1, ch));
     }
     P.mut_blocks = (P.h + kScanBlock - 1) / kScanBlock;
-    P.word_blocks = (P.w_cap + kScanBlock - 1) / kScanBlock;
+    P.word_blocks = (P.w_cap + 63) / 64;  // 64-word free-space blocks
     const uint32_t sel_blocks = (P.n + kScanBlock - 1) / kScanBlock;
     const uint32_t T1 = std::max<uint32_t>(1, P.traits);
 
@@ -256,6 +259,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.fixed_bits, P.w_cap);
     rc |= dalloc(c, &B.sorted, (size_t)P.w_cap * 64);
     rc |= dalloc(c, &B.pm, (size_t)P.w_cap * 65);
+    rc |= dalloc(c, &B.bucket, (size_t)P.w_cap * 64);
     rc |= dalloc(c, &B.digits, i8_packed_elems(std::max<uint32_t>(1, P.traits), P.w_cap));
     rc |= dalloc(c, &B.mut_count, P.h);
     rc |= dalloc(c, &B.mut_local, P.h);
@@ -264,8 +268,8 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.bottom, P.h);
     rc |= dalloc(c, &B.n_events, P.h);
     rc |= dalloc(c, &B.events, (size_t)P.h * P.ev_cap);
-    rc |= dalloc(c, &B.var_part, (size_t)(P.chunks + 1) * P.w_cap);
-    rc |= dalloc(c, &B.fix_part, (size_t)P.chunks * P.w_cap);
+    rc |= dalloc(c, &B.var_part, (size_t)(P.max_chunks + 1) * P.w_cap);
+    rc |= dalloc(c, &B.fix_part, (size_t)P.max_chunks * P.w_cap);
     rc |= dalloc(c, &B.word_cnt, P.w_cap);
     rc |= dalloc(c, &B.word_bsum, 2 * P.word_blocks);
     rc |= dalloc(c, &B.cum_local, P.n);
@@ -286,7 +290,13 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     ok = ok && hipMemset(B.keys, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
          hipMemset(B.effects, 0, sizeof(int32_t) * P.s_cap * T1) == hipSuccess &&
          hipMemset(B.ages, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
-         hipMemset(B.fixed_bits, 0, sizeof(uint64_t) * P.w_cap) == hipSuccess;
+         hipMemset(B.fixed_bits, 0, sizeof(uint64_t) * P.w_cap) == hipSuccess &&
+         hipMemset(B.top, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
+         hipMemset(B.bottom, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
+         hipMemset(B.n_events, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
+         hipMemset(B.mut_count, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
+         hipMemset(B.mut_local, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
+         hipMemset(B.mut_bsum, 0, sizeof(uint32_t) * P.mut_blocks) == hipSuccess;
     if (ok) {
         std::vector<double> ones(P.n, 1.0);
         ok = hipMemcpy(B.fitness[0], ones.data(), 8 * P.n, hipMemcpyHostToDevice) == hipSuccess;
@@ -679,5 +689,56 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
 }
 
 int clotho_generation(clotho_ctx* c) { return clotho_run(c, 1, 0); }
+
+// Launch-shape autotuner: times the recombination kernel for candidate
+// row-chunk counts and keeps the fastest.  It replays the last generation's
+// gametes (parents, events, mutation counts; zeros before the first
+// generation) against the resident population and writes only buffers the
+// next generation rewrites: the spare row buffer, the column partials and the
+// per-word order statistics.  Visible state is unchanged.
+int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    if (st.wp == 0 || st.wc == 0) {  // empty parents: nothing to time
+        if (chosen_chunks) *chosen_chunks = c->p.chunks;
+        return CLOTHO_OK;
+    }
+    const uint32_t base = c->p.chunks;
+    std::vector<uint32_t> cand;
+    for (uint32_t f : {1u, 2u, 4u, 8u}) {
+        cand.push_back(std::max<uint32_t>(1, base / f));
+        cand.push_back(std::min<uint32_t>(c->p.max_chunks, base * f));
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t best_chunks = base;
+    for (uint32_t ch : cand) {
+        c->p.chunks = ch;
+        HIP_TRY(c, launch_recombine(c));  // warm-up
+        HIP_TRY(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 3; ++r) HIP_TRY(c, launch_recombine(c));
+        HIP_TRY(c, hipEventRecord(e1, c->stream));
+        HIP_TRY(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_chunks = ch;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->p.chunks = best_chunks;
+    destroy_graphs(c);  // captured launches bake the shape in
+    if (chosen_chunks) *chosen_chunks = best_chunks;
+    return CLOTHO_OK;
+}
 
 }  // extern "C"

@@ -332,14 +332,16 @@ __global__ void __launch_bounds__(kBlock) k_gametes(Params P, const GenState* st
 // Recombination + mutation (+ free-space partials)
 // ---------------------------------------------------------------------------
 // Per-word order statistics, once per generation: the 64 position keys of
-// word w sorted ascending (sorted[w][k]) and prefix masks pm[w][c] = bits of
-// the c smallest keys.  For a gamete, {j in w : key_j >= e} = ~pm[w][c(e)]
-// with c(e) = #{k : sorted[w][k] < e}, so the crossover mask of the word,
+// word w sorted ascending (sorted[w][k]), prefix masks pm[w][c] = bits of the
+// c smallest keys, and a 256-bucket table bucket[w][b] = #{k : key_k < b<<24}
+// (four buckets per dword).  For a gamete event e, c(e) = #{k : key_k < e}
+// is bucket[e>>24] plus the few keys of that bucket below e, and
+// {j in w : key_j >= e} = ~pm[w][c(e)]; the crossover mask of the word,
 // {j : odd #{events e <= key_j}} (block_crossover + region_classifier's
 // upper_bound), is the XOR over events of ~pm[w][c(e)].
 __global__ void __launch_bounds__(kBlock) k_word_sort(Params P, const GenState* st,
                                                       const uint32_t* keys, uint32_t* sorted,
-                                                      uint64_t* pm) {
+                                                      uint64_t* pm, uint32_t* bucket) {
     __shared__ uint32_t s_key[kWaves][64];
     __shared__ uint32_t s_perm[kWaves][64];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -368,30 +370,40 @@ __global__ void __launch_bounds__(kBlock) k_word_sort(Params P, const GenState* 
     }
     pm[(size_t)w * 65 + lane + 1] = x;
     if (lane == 0) pm[(size_t)w * 65] = 0;
+    // buckets 4 lane .. 4 lane + 3: count of keys below b << 24
+    uint32_t packed = 0;
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t bound = (4 * lane + q) << 24;
+        uint32_t cnt = 0;
+        for (uint32_t k = 0; k < 64; ++k) cnt += s_key[wave][k] < bound ? 1u : 0u;
+        packed |= cnt << (8 * q);  // cnt <= 64 fits a byte
+    }
+    bucket[(size_t)w * 64 + lane] = packed;
 }
 
 // Grid: x = 64-word column tiles, y = row chunks; kRecWaves waves per block,
-// one 64-bit word per lane.  The tile's sorted keys ([k][lane]) and prefix
-// masks ([c][lane]) sit in LDS, so the per-event lower_bound is a 7-step
-// binary search on the lane's own bank column.  Parent words of the next
-// row are loaded while the current row is classified.  For each child row:
+// one 64-bit word per lane.  The tile's bucket table ([b/4][lane]), sorted
+// keys ([k][lane], 4 sentinel rows) and prefix masks ([c][lane]) sit in LDS,
+// each lane reading its own bank column.  Parent words of the next row are
+// loaded while the current row is classified.  For each child row:
 //   t, b  = top/bottom parent words with fixed columns cleared
 //   mask  = XOR_e ~pm[c(e)]
 //   child = (t & ~mask) | (b & mask), then XOR of this row's new mutations
 // OR/AND of the child words accumulate in registers (free_space_accumulator).
-constexpr int kRecWaves = 8;
+constexpr int kRecWaves = 16;
 constexpr int kRecBlock = 64 * kRecWaves;
 
 __global__ void __launch_bounds__(kRecBlock) k_recombine(
     Params P, const GenState* st, const uint64_t* __restrict__ parent,
     uint64_t* __restrict__ child, const uint32_t* __restrict__ sorted,
-    const uint64_t* __restrict__ pm, const uint32_t* __restrict__ free_list,
-    const uint64_t* __restrict__ fixed_bits, const uint32_t* __restrict__ top_in,
-    const uint32_t* __restrict__ bot_in, const uint32_t* __restrict__ nev_in,
-    const uint32_t* __restrict__ events, const uint32_t* __restrict__ count,
-    const uint32_t* __restrict__ local, const uint32_t* __restrict__ boff,
-    uint64_t* var_part, uint64_t* fix_part) {
-    __shared__ uint32_t s_sorted[64 * 64];
+    const uint64_t* __restrict__ pm, const uint32_t* __restrict__ bucket,
+    const uint32_t* __restrict__ free_list, const uint64_t* __restrict__ fixed_bits,
+    const uint32_t* __restrict__ top_in, const uint32_t* __restrict__ bot_in,
+    const uint32_t* __restrict__ nev_in, const uint32_t* __restrict__ events,
+    const uint32_t* __restrict__ count, const uint32_t* __restrict__ local,
+    const uint32_t* __restrict__ boff, uint64_t* var_part, uint64_t* fix_part) {
+    __shared__ uint32_t s_bucket[64 * 64];
+    __shared__ uint32_t s_sorted[68 * 64];
     __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
 
     const uint32_t wc = st->wc;
@@ -405,8 +417,10 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
     const bool inpar = valid && w < wp;
     const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
 
-    for (uint32_t k = wave; k < 64; k += kRecWaves)
-        s_sorted[k * 64 + lane] = valid ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
+    for (uint32_t k = wave; k < 68; k += kRecWaves)
+        s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
+    for (uint32_t q = wave; q < 64; q += kRecWaves)
+        s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
     for (uint32_t c = wave; c < 65; c += kRecWaves)
         s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
     const uint64_t keep = inpar ? ~fixed_bits[w] : ~(uint64_t)0;
@@ -438,12 +452,15 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
         const uint32_t* ev = events + (size_t)o * P.ev_cap;
         for (uint32_t e = 0; e < ne; ++e) {
             const uint32_t key = ev[e];
-            // lower_bound over the 63 smallest keys, then the 64th
-            uint32_t pos = 0;
-#pragma unroll
-            for (uint32_t sstep = 32; sstep >= 1; sstep >>= 1)
-                pos += (s_sorted[(pos + sstep - 1) * 64 + lane] < key) ? sstep : 0u;
-            pos += (s_sorted[pos * 64 + lane] < key) ? 1u : 0u;
+            const uint32_t tb = s_bucket[(key >> 26) * 64 + lane];
+            uint32_t pos = (tb >> ((key >> 21) & 24u)) & 0xFFu;
+            const uint32_t* sk = s_sorted + pos * 64 + lane;
+            const uint32_t c0 = sk[0] < key, c1 = sk[64] < key, c2 = sk[128] < key;
+            const bool more = sk[192] < key;
+            pos += c0 + c1 + c2 + (more ? 1u : 0u);
+            if (more) {  // four or more keys of this bucket below the event: rare
+                while (pos < 64 && s_sorted[pos * 64 + lane] < key) ++pos;
+            }
             m ^= ~s_pm[pos * 64 + lane];
         }
         uint64_t c = (t & ~m) | (b & m);
@@ -482,6 +499,10 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
 // ---------------------------------------------------------------------------
 // Free space: reduce chunk partials per word, classify, ordered compaction
 // ---------------------------------------------------------------------------
+// 64 words per block; the four waves fold interleaved chunk partials, then
+// wave 0 classifies its word and scans the per-word counts.
+constexpr uint32_t kFreeWords = 64;
+
 __global__ void __launch_bounds__(kBlock) k_free_reduce(Params P, const GenState* st,
                                                         const uint64_t* var_part,
                                                         const uint64_t* fix_part,
@@ -489,34 +510,50 @@ __global__ void __launch_bounds__(kBlock) k_free_reduce(Params P, const GenState
                                                         uint64_t* lost_bits, uint32_t* word_cnt,
                                                         uint32_t* word_bsum,
                                                         uint32_t word_blocks) {
-    __shared__ uint32_t lds[kWaves];
-    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint64_t s_or[kWaves][64];
+    __shared__ uint64_t s_and[kWaves][64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t w = blockIdx.x * kFreeWords + lane;
     const uint32_t wc = st->wc, S = st->all_size;
-    uint32_t cf = 0, cl = 0;
-    uint64_t fx = 0, ls = 0;
+    uint64_t vo = 0, va = ~(uint64_t)0;
     if (w < wc) {
-        uint64_t vo = 0, va = ~(uint64_t)0;
-        for (uint32_t c = 0; c < chunks; ++c) {
+        for (uint32_t c = wave; c < chunks; c += kWaves) {
             vo |= var_part[(size_t)c * P.w_cap + w];
             va &= fix_part[(size_t)c * P.w_cap + w];
         }
+    }
+    s_or[wave][lane] = vo;
+    s_and[wave][lane] = va;
+    __syncthreads();
+    if (wave) return;  // no barrier below
+    for (int q = 1; q < kWaves; ++q) {
+        vo |= s_or[q][lane];
+        va &= s_and[q][lane];
+    }
+    uint32_t cf = 0, cl = 0;
+    if (w < wc) {
         const uint32_t lo = w * 64;
         uint64_t validm = 0;
         if (S > lo) validm = (S - lo >= 64) ? ~(uint64_t)0 : (((uint64_t)1 << (S - lo)) - 1);
-        fx = va & validm;
-        ls = ~vo & validm;
+        const uint64_t fx = va & validm, ls = ~vo & validm;
         cf = (uint32_t)__popcll(fx);
         cl = (uint32_t)__popcll(ls);
         fixed_bits[w] = fx;
         lost_bits[w] = ls;
     }
-    uint32_t tf, tl;
-    const uint32_t incf = block_inclusive_scan<uint32_t>(cf, lds, tf);
-    const uint32_t incl = block_inclusive_scan<uint32_t>(cl, lds, tl);
-    if (w < wc) word_cnt[w] = ((incf - cf) & 0xFFFFu) | ((incl - cl) << 16);
-    if (threadIdx.x == 0) {
-        word_bsum[blockIdx.x] = tf;
-        word_bsum[word_blocks + blockIdx.x] = tl;
+    // wave-level exclusive scans of the fixed and lost counts
+    uint32_t xf = cf, xl = cl;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t yf = __shfl_up(xf, d, 64), yl = __shfl_up(xl, d, 64);
+        if ((int)lane >= d) {
+            xf += yf;
+            xl += yl;
+        }
+    }
+    if (w < wc) word_cnt[w] = ((xf - cf) & 0xFFFFu) | ((xl - cl) << 16);
+    if (lane == 63) {
+        word_bsum[blockIdx.x] = xf;
+        word_bsum[word_blocks + blockIdx.x] = xl;
     }
 }
 
@@ -529,7 +566,7 @@ __global__ void __launch_bounds__(kBlock) k_free_compact(GenState* st, const uin
     __shared__ uint32_t s_off[2];
     __shared__ uint32_t s_red[2][kBlock];
     const uint32_t wc = st->wc;
-    const uint32_t nb = (wc + kBlock - 1) / kBlock;
+    const uint32_t nb = (wc + kFreeWords - 1) / kFreeWords;
     if (blockIdx.x >= nb) return;
     // offsets of this block and the fixed total, from the block totals
     uint32_t pf = 0, tfix = 0, pl = 0;
@@ -563,8 +600,8 @@ __global__ void __launch_bounds__(kBlock) k_free_compact(GenState* st, const uin
         __syncthreads();
     }
     const uint32_t nfixed = s_red[0][0];
-    const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
-    if (w < wc) {
+    const uint32_t w = blockIdx.x * kFreeWords + threadIdx.x;
+    if (threadIdx.x < kFreeWords && w < wc) {
         const uint32_t wcnt = word_cnt[w];
         uint32_t pos = s_off[0] + (wcnt & 0xFFFFu);
         uint64_t fx = fixed_bits[w];
@@ -753,10 +790,10 @@ hipError_t launch_recombine(clotho_ctx* c) {
     const int par = c->cur, chi = 1 - c->cur;
     (void)chi;
     hipLaunchKernelGGL(k_word_sort, dim3((P.w_cap + kWaves - 1) / kWaves), dim3(kBlock), 0,
-                       c->stream, P, B.state, B.keys, B.sorted, B.pm);
+                       c->stream, P, B.state, B.keys, B.sorted, B.pm, B.bucket);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_recombine, dim3(P.tiles, P.chunks), dim3(kRecBlock), 0, c->stream, P,
-                       B.state, B.rows[par], B.rows[chi], B.sorted, B.pm, B.free_list,
+                       B.state, B.rows[par], B.rows[chi], B.sorted, B.pm, B.bucket, B.free_list,
                        B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
                        B.mut_local, B.mut_bsum, B.var_part, B.fix_part);
     LAUNCH_CHECK();
@@ -768,7 +805,7 @@ hipError_t launch_traits(clotho_ctx* c) { return launch_traits_of(c, 1 - c->cur)
 hipError_t launch_free_space(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
-    uint64_t* lost_bits = B.var_part + (size_t)P.chunks * P.w_cap;  // spare row of partials
+    uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;  // spare row of partials
     hipLaunchKernelGGL(k_free_reduce, dim3(P.word_blocks), dim3(kBlock), 0, c->stream, P,
                        B.state, B.var_part, B.fix_part, P.chunks, B.fixed_bits, lost_bits,
                        B.word_cnt, B.word_bsum, P.word_blocks);
@@ -806,7 +843,7 @@ hipError_t launch_population_analysis(clotho_ctx* c) {
         const hipError_t e = launch_traits_of(c, cur);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_column_reduce, dim3(P.word_blocks, P.chunks), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_column_reduce, dim3((P.w_cap + kBlock - 1) / kBlock, P.chunks), dim3(kBlock), 0,
                        c->stream, P, B.state, B.rows[cur], B.var_part, B.fix_part);
     LAUNCH_CHECK();
     return launch_free_space(c);

@@ -56,7 +56,8 @@ struct Params {
     uint32_t chunks;       // row chunks of the recombination grid
     uint32_t tiles;        // 64-word column tiles at capacity
     uint32_t mut_blocks;   // scan blocks over gametes
-    uint32_t word_blocks;  // scan blocks over words at capacity
+    uint32_t word_blocks;  // 64-word free-space blocks at capacity
+    uint32_t max_chunks;   // row-chunk capacity of the partial buffers
 };
 
 // Device buffers.  Index [c] is the population that is current after the
@@ -72,6 +73,7 @@ struct Buffers {
     uint64_t* fixed_bits;     // [w_cap] AND of all rows, idx < all_size
     uint32_t* sorted;         // [w_cap][64] position keys of each word, ascending
     uint64_t* pm;             // [w_cap][65] bits of the c smallest keys of each word
+    uint32_t* bucket;         // [w_cap][64] per word: #{keys < b<<24}, four b per dword
     uint4* digits;            // [ceil(T/4)][w_cap][64] int8 digit planes of the effects
     uint32_t* mut_count;      // [h]
     uint32_t* mut_local;      // [h] exclusive scan inside a block
@@ -84,6 +86,7 @@ struct Buffers {
     uint64_t* fix_part;       // [chunks][w_cap] AND partials
     uint32_t* word_cnt;       // [w_cap] (fixed count << 16 | lost count) per word
     uint32_t* word_bsum;      // [2*word_blocks] block totals of fixed / lost counts
+                              // (partials are sized for max_chunks rows)
     double* cum_local;        // [n] quantised fitness inclusive scan inside a block
     double* cum_block;        // [sel blocks] inclusive scan of block totals
     double* fit_stats;        // [2*sel blocks] per-block max, all-equal flag

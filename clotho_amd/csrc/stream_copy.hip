@@ -7,10 +7,21 @@
 
 namespace {
 
+// one pass: each thread issues four 16-byte loads before its four stores
 __global__ void __launch_bounds__(256) k_stream_copy(const uint4* __restrict__ src,
                                                      uint4* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-        dst[i] = src[i];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + 256 * (size_t)k;
+        v[k] = i < n ? src[i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + 256 * (size_t)k;
+        if (i < n) dst[i] = v[k];
+    }
 }
 
 }  // namespace
@@ -28,7 +39,7 @@ extern "C" int clotho_stream_copy(size_t bytes, int repeats, double* gbps, int d
         hipMemset(a, 1, n * sizeof(uint4)) == hipSuccess &&
         hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
         hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-        const unsigned blocks = 256 * 32;
+        const unsigned blocks = (unsigned)((n + 1023) / 1024);
         hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, a, b, n);
         bool ok = hipGetLastError() == hipSuccess && hipEventRecord(e0, s) == hipSuccess;
         for (int r = 0; ok && r < repeats; ++r) {

@@ -88,16 +88,27 @@ __global__ void __launch_bounds__(256) k_pack_i8(const int32_t* __restrict__ eff
 }
 
 // Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
+// With one N tile (T <= 4) the block stages each group's B fragments (16 KB)
+// in LDS, double buffered, so the four waves share one copy; the next
+// group's rows are loaded before the current group is multiplied.  Bits
+// expand to int8 0/1 through a 256-entry LDS table (byte -> 8 bytes).
 template <int RT, int NT>
 __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ rows, uint32_t h,
                                                    const uint32_t* p_words, uint32_t T,
                                                    const uint4* __restrict__ packed,
                                                    int64_t* __restrict__ traits) {
+    constexpr bool kLdsB = (NT == 1);
+    __shared__ uint4 s_b[kLdsB ? 2 * 16 * 64 : 1];
+    __shared__ uint2 s_lut[256];
     const uint32_t W = *p_words;
     const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t m = lane & 15, g = lane >> 4;
     const uint32_t base = (blockIdx.x * 4 + wave) * (16 * RT);
+    {
+        const uint32_t v = threadIdx.x;
+        s_lut[v] = make_uint2(spread4(v & 15), spread4(v >> 4));
+    }
     v4i acc[RT][NT];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
@@ -111,9 +122,8 @@ __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ 
         rv[r] = row < h;
         rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
     }
-    for (uint32_t G = 0; G < NG; ++G) {
+    auto load_rows = [&](uint32_t G, uint4 (&x)[RT][2]) {
         const uint32_t w = kGroupWords * G + 4 * g;
-        uint4 x[RT][2];
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             if (rv[r] && w < W) {
@@ -125,7 +135,24 @@ __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ 
                 x[r][1] = make_uint4(0, 0, 0, 0);
             }
         }
-        const uint4* bp = packed + (size_t)G * 16 * 64 + lane;
+    };
+    auto stage_b = [&](uint32_t G) {
+        if (kLdsB && G < NG) {
+            uint4* dst = s_b + (G & 1) * 16 * 64;
+            const uint4* src = packed + (size_t)G * 16 * 64;
+            for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) dst[i] = src[i];
+        }
+    };
+    uint4 x[RT][2];
+    if (NG) load_rows(0, x);
+    stage_b(0);
+    __syncthreads();
+    for (uint32_t G = 0; G < NG; ++G) {
+        uint4 xn[RT][2];
+        if (G + 1 < NG) load_rows(G + 1, xn);
+        stage_b(G + 1);
+        const uint4* bg = kLdsB ? s_b + (G & 1) * 16 * 64 + lane
+                                : packed + (size_t)G * 16 * 64 + lane;
 #pragma unroll
         for (uint32_t q = 0; q < 16; ++q) {
             v4i a[RT];
@@ -136,18 +163,24 @@ __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ 
                                   : ((q >> 1) & 3) == 1 ? v.y
                                   : ((q >> 1) & 3) == 2 ? v.z : v.w;
                 const uint32_t b16 = (q & 1) ? (dw >> 16) : (dw & 0xFFFFu);
-                a[r] = v4i{(int)spread4(b16 & 15), (int)spread4((b16 >> 4) & 15),
-                           (int)spread4((b16 >> 8) & 15), (int)spread4(b16 >> 12)};
+                const uint2 lo = s_lut[b16 & 0xFFu], hi = s_lut[b16 >> 8];
+                a[r] = v4i{(int)lo.x, (int)lo.y, (int)hi.x, (int)hi.y};
             }
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const uint4 bq = bp[((size_t)nt * NG * 16 + q) * 64];
+                const uint4 bq = kLdsB ? bg[q * 64] : bg[((size_t)nt * NG * 16 + q) * 64];
                 const v4i bv = v4i{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
 #pragma unroll
                 for (int r = 0; r < RT; ++r)
                     acc[r][nt] =
                         __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
             }
+        }
+        __syncthreads();  // staged B of group G + 1 complete; buffer G & 1 free
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            x[r][0] = xn[r][0];
+            x[r][1] = xn[r][1];
         }
     }
     // recombine the digit planes: column n = 4 t_local + p held by lane m = n
@@ -322,8 +355,8 @@ hipError_t launch_traits_i8(const uint64_t* rows, uint32_t h, const uint32_t* p_
                             uint32_t traits, const uint4* packed, int64_t* out, hipStream_t s) {
     const uint32_t nt = i8_ntiles(traits);
     if (nt <= 1) {
-        const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
-        hipLaunchKernelGGL((k_traits_i8<2, 1>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+        const uint32_t blocks = (h + 4 * 64 - 1) / (4 * 64);
+        hipLaunchKernelGGL((k_traits_i8<4, 1>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
                            traits, packed, out);
     } else if (nt <= 2) {
         const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);

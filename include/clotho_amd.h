@@ -180,6 +180,11 @@ int clotho_sync(clotho_ctx* ctx);
  * generations timed (eager runs; a graph run fills [0] only). */
 int clotho_timings(clotho_ctx* ctx, double* ms, int cap);
 void clotho_set_timing(clotho_ctx* ctx, int enabled);
+/* Launch-shape autotuner (no reference counterpart; the reference's batch
+ * shape is ceil(N/(T+1)) rows per host thread, QS/engine_parallel_pipeline.hpp:205-216):
+ * times the coming generation's recombination for candidate row-chunk counts
+ * on the resident population and keeps the fastest.  State is unchanged. */
+int clotho_autotune(clotho_ctx* ctx, uint32_t* chosen_chunks);
 
 /* ---- standalone trait contraction -------------------------------------- */
 /* z[h][t] = sum_j bit(h,j) * effects[j][t] over a device-resident genotype

@@ -465,6 +465,12 @@ public:
         check(clotho_run(m_pop.ctx(), generations, graph ? 1 : 0), m_pop.ctx(), "clotho_run");
         m_generation += generations;
     }
+    // pick the recombination launch shape for the resident population
+    unsigned int autotune() {
+        uint32_t ch = 0;
+        check(clotho_autotune(m_pop.ctx(), &ch), m_pop.ctx(), "clotho_autotune");
+        return ch;
+    }
     size_t getGeneration() const { return m_generation; }
     sequence_space_type* getChildPopulation() { return &m_pop; }
     allele_type* getAlleleSpace() {

@@ -110,3 +110,20 @@ def test_synthetic_population_bits():
             d = oracle.draw(77, 0, 7, 0, r, j // 4)
             assert bits[j] == (d[j % 4] < thr[j])
         assert not bits[700:].any()
+
+
+def test_autotune_keeps_parity():
+    rows, pos, eff, fit = _uploaded_population(300, 2000, 1, 5)
+    cfg = api.make_config(300, 2.0, 5.0, seed=37, traits=1, neutral_p=0.2, sigma=0.2,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=0.5, max_alleles=1 << 13)
+    sim = api.Simulation(cfg)
+    orc = __import__("oracle").OracleSim(cfg)
+    sim.set_population(rows, pos, eff, fit)
+    orc.set_population(rows, pos, eff, fit)
+    assert sim.autotune() >= 1  # before any generation: zero gametes
+    compare_state(sim, orc, "after autotune")
+    for g in range(4):
+        sim.run(1, graph=(g % 2 == 1))
+        orc.step()
+        assert sim.autotune() >= 1
+        compare_state(sim, orc, f"autotuned generation {g + 1}")
