@@ -66,7 +66,7 @@ def standing_variation(seed=42):
 def traffic_from_profiles():
     """HBM bytes per recombination launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated by a copy kernel), if any."""
-    p = os.path.join(ROOT, "profiles", "recombine_c1_traffic.json")
+    p = os.path.join(ROOT, "profiles", "k_recombine_N1e5_W784_rho5_traffic.json")
     if not os.path.exists(p):
         return None
     try:

@@ -712,6 +712,12 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
         cand.push_back(std::max<uint32_t>(1, base / f));
         cand.push_back(std::min<uint32_t>(c->p.max_chunks, base * f));
     }
+    // grids that fill whole waves of resident blocks (two per CU)
+    const uint32_t resident = 2u * (uint32_t)std::max(1, c->prop.multiProcessorCount);
+    for (uint32_t k = 1; k <= 16; ++k) {
+        const uint32_t ch = (k * resident) / std::max<uint32_t>(1, c->p.tiles);
+        if (ch >= 1 && ch <= c->p.max_chunks) cand.push_back(ch);
+    }
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     hipEvent_t e0 = nullptr, e1 = nullptr;

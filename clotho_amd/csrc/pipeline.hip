@@ -384,15 +384,39 @@ __global__ void __launch_bounds__(kBlock) k_word_sort(Params P, const GenState* 
 // Grid: x = 64-word column tiles, y = row chunks; kRecWaves waves per block,
 // one 64-bit word per lane.  The tile's bucket table ([b/4][lane]), sorted
 // keys ([k][lane], 4 sentinel rows) and prefix masks ([c][lane]) sit in LDS,
-// each lane reading its own bank column.  Parent words of the next row are
-// loaded while the current row is classified.  For each child row:
+// each lane reading its own bank column.  For each child row:
 //   t, b  = top/bottom parent words with fixed columns cleared
 //   mask  = XOR_e ~pm[c(e)]
 //   child = (t & ~mask) | (b & mask), then XOR of this row's new mutations
 // OR/AND of the child words accumulate in registers (free_space_accumulator).
+//
+// Latency structure: a wave owns rows r0 + wave + kRecWaves j.  The gamete
+// records of 64 such rows (parents, event and mutation counts, mutation
+// offset) are loaded with one vector load each, lane q holding row j0 + q, and
+// read back with v_readlane, so the row loop issues no scalar loads.  Even and
+// odd rows use separate register sets: the loads of row q + 2 are issued as
+// soon as row q is done and stay in flight while row q + 1 is classified (no
+// register rotation, hence no wait on loads still in flight).  Lane e holds
+// event key e.  Events are classified four at a time so that their LDS
+// lookups overlap; a group is padded with rank-64 entries (pm[64] is all
+// ones), and since #real + #padding = 4, pm[c0] ^ pm[c1] ^ pm[c2] ^ pm[c3]
+// equals the XOR of the complements of the real events.
 constexpr int kRecWaves = 16;
 constexpr int kRecBlock = 64 * kRecWaves;
+// dword 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
+// data format, offsets range-checked against num_records in bytes
+constexpr int kBufferWord3 = 0x00020000;
 
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+// Memory operations of the row loop are issued unconditionally (indices
+// clamped in bounds, the child store through a range-checked buffer resource
+// that drops the lanes past the row), so the number of vector memory
+// operations in flight is the same on every path and the waits before a row's
+// compute cover only that row's loads.
+template <bool kWideEvents>
 __global__ void __launch_bounds__(kRecBlock) k_recombine(
     Params P, const GenState* st, const uint64_t* __restrict__ parent,
     uint64_t* __restrict__ child, const uint32_t* __restrict__ sorted,
@@ -415,7 +439,9 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
     const uint32_t w = w0 + lane;
     const bool valid = w < wc;
     const bool inpar = valid && w < wp;
+    const uint32_t col = wp ? min(w, wp - 1) : 0;  // in-bounds parent column
     const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
+    const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
 
     for (uint32_t k = wave; k < 68; k += kRecWaves)
         s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
@@ -423,61 +449,111 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
         s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
     for (uint32_t c = wave; c < 65; c += kRecWaves)
         s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
-    const uint64_t keep = inpar ? ~fixed_bits[w] : ~(uint64_t)0;
+    const uint64_t keep = inpar ? ~fixed_bits[w] : 0;  // also clears columns >= wp
     __syncthreads();
 
     const uint32_t rows_per_chunk = (P.h + gridDim.y - 1) / gridDim.y;
-    const uint32_t r0 = blockIdx.y * rows_per_chunk;
-    const uint32_t r1 = min(P.h, r0 + rows_per_chunk);
+    const uint32_t r0 = blockIdx.y * rows_per_chunk + wave;
+    const uint32_t r1 = min(P.h, blockIdx.y * rows_per_chunk + rows_per_chunk);
+    const uint32_t nrows = r1 > r0 ? (r1 - r0 + kRecWaves - 1) / kRecWaves : 0;
     uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-    uint32_t o = r0 + wave;
-    uint64_t t = 0, b = 0;
-    if (o < r1 && inpar) {
-        t = parent[(size_t)top_in[o] * wp + w];
-        b = parent[(size_t)bot_in[o] * wp + w];
-    }
-    for (; o < r1; o += kRecWaves) {
-        // prefetch the next row's parent words
-        const uint32_t on = o + kRecWaves;
-        uint64_t tn = 0, bn = 0;
-        if (on < r1 && inpar) {
-            tn = parent[(size_t)top_in[on] * wp + w];
-            bn = parent[(size_t)bot_in[on] * wp + w];
-        }
-        t &= keep;
-        b &= keep;
-        const uint32_t ne = nev_in[o];
-        uint64_t m = 0;
-        const uint32_t* ev = events + (size_t)o * P.ev_cap;
-        for (uint32_t e = 0; e < ne; ++e) {
-            const uint32_t key = ev[e];
-            const uint32_t tb = s_bucket[(key >> 26) * 64 + lane];
-            uint32_t pos = (tb >> ((key >> 21) & 24u)) & 0xFFu;
-            const uint32_t* sk = s_sorted + pos * 64 + lane;
-            const uint32_t c0 = sk[0] < key, c1 = sk[64] < key, c2 = sk[128] < key;
-            const bool more = sk[192] < key;
-            pos += c0 + c1 + c2 + (more ? 1u : 0u);
-            if (more) {  // four or more keys of this bucket below the event: rare
-                while (pos < 64 && s_sorted[pos * 64 + lane] < key) ++pos;
+    for (uint32_t j0 = 0; j0 < nrows; j0 += 64) {
+        const uint32_t nb = min(64u, nrows - j0);
+        // gamete records of rows j0 .. j0 + nb - 1, lane q = row j0 + q
+        auto row_of = [&](uint32_t q) { return r0 + kRecWaves * (j0 + q); };
+        const uint32_t ol = row_of(min(lane, nb - 1));
+        const uint32_t g_top = top_in[ol];
+        const uint32_t g_bot = bot_in[ol];
+        const uint32_t g_ne = nev_in[ol];
+        const uint32_t g_nm = count[ol];
+        const uint32_t g_k0 = local[ol] + boff[ol / kBlock];
+
+        // rows past the batch reload its last row (in bounds, never processed)
+        auto load_row = [&](uint32_t q, uint64_t& t, uint64_t& b, uint32_t& lo, uint32_t& hi) {
+            const uint32_t qq = min(q, nb - 1);
+            const uint32_t* ev = events + (size_t)row_of(qq) * ev_cap;
+            lo = ev[min(lane, ev_cap - 1)];
+            if (kWideEvents) hi = ev[min(lane + 64, ev_cap - 1)];
+            t = parent[(size_t)lane_read(g_top, qq) * wp + col];
+            b = parent[(size_t)lane_read(g_bot, qq) * wp + col];
+        };
+
+        auto process = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
+            const uint32_t ne = lane_read(g_ne, q);
+            uint64_t m = 0;
+            for (uint32_t e = 0; e < ne; e += 4) {
+                uint32_t key[4], p[4];
+                bool more[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t x = e + i;
+                    const uint32_t a = lane_read(lo, x & 63);
+                    key[i] = (kWideEvents && x >= 64) ? lane_read(hi, x & 63) : a;
+                }
+                uint32_t tb[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                uint32_t sk[4][4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const bool live = e + i < ne;
+                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) +
+                                       (sk[i][1] < key[i] ? 1u : 0u) +
+                                       (sk[i][2] < key[i] ? 1u : 0u);
+                    more[i] = live && sk[i][3] < key[i];
+                    p[i] = live ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
+                }
+                if (more[0] | more[1] | more[2] | more[3]) {
+                    // four or more keys of a bucket below an event: rare
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (more[i])
+                            while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
+                }
+                m ^= (s_pm[p[0] * 64 + lane] ^ s_pm[p[1] * 64 + lane]) ^
+                     (s_pm[p[2] * 64 + lane] ^ s_pm[p[3] * 64 + lane]);
             }
-            m ^= ~s_pm[pos * 64 + lane];
-        }
-        uint64_t c = (t & ~m) | (b & m);
-        // new mutations of this row (free columns: zero in every parent row)
-        const uint32_t nm = count[o];
-        if (nm) {
-            const uint32_t k0 = local[o] + boff[o / kBlock];
-            for (uint32_t k = k0; k < k0 + nm; ++k) {
-                const uint32_t slot = k < F ? free_list[k] : A + (k - F);
-                if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+            t &= keep;
+            b &= keep;
+            uint64_t c = (t & ~m) | (b & m);
+            // new mutations of this row (free columns: zero in every parent row)
+            const uint32_t nm = lane_read(g_nm, q);
+            if (nm) {
+                const uint32_t k0 = lane_read(g_k0, q);
+                for (uint32_t k = k0; k < k0 + nm; ++k) {
+                    const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+                    if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+                }
             }
+            {
+                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                    child + (size_t)row_of(q) * wc, 0, (int)(wc * 8u), kBufferWord3);
+                typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+                const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
+                __builtin_amdgcn_raw_buffer_store_b64(v, row, valid ? w * 8u : 0x80000000u, 0,
+                                                      0);
+            }
+            acc_or |= c;
+            acc_and &= c;
+        };
+
+        uint64_t tA = 0, bA = 0, tB = 0, bB = 0;
+        uint32_t loA = 0, hiA = 0, loB = 0, hiB = 0;
+        load_row(0, tA, bA, loA, hiA);
+        load_row(1, tB, bB, loB, hiB);
+        for (uint32_t q = 0; q < nb; q += 2) {
+            process(q, tA, bA, loA, hiA);
+            load_row(q + 2, tA, bA, loA, hiA);
+            if (q + 1 < nb) process(q + 1, tB, bB, loB, hiB);
+            load_row(q + 3, tB, bB, loB, hiB);
         }
-        if (valid) child[(size_t)o * wc + w] = c;
-        acc_or |= c;
-        acc_and &= c;
-        t = tn;
-        b = bn;
     }
     __syncthreads();  // every wave is past its last prefix-mask read
     uint64_t* s_or = s_pm;
@@ -792,7 +868,8 @@ hipError_t launch_recombine(clotho_ctx* c) {
     hipLaunchKernelGGL(k_word_sort, dim3((P.w_cap + kWaves - 1) / kWaves), dim3(kBlock), 0,
                        c->stream, P, B.state, B.keys, B.sorted, B.pm, B.bucket);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_recombine, dim3(P.tiles, P.chunks), dim3(kRecBlock), 0, c->stream, P,
+    auto kernel = P.ev_cap > 64 ? k_recombine<true> : k_recombine<false>;
+    hipLaunchKernelGGL(kernel, dim3(P.tiles, P.chunks), dim3(kRecBlock), 0, c->stream, P,
                        B.state, B.rows[par], B.rows[chi], B.sorted, B.pm, B.bucket, B.free_list,
                        B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
                        B.mut_local, B.mut_bsum, B.var_part, B.fix_part);
