@@ -112,7 +112,12 @@ EXPORTS = [
     "clotho_timings", "clotho_set_timing", "clotho_trait_contract",
     "clotho_set_synthetic_population", "clotho_stream_copy", "clotho_trait_contract_synthetic",
     "clotho_autotune",
+    "clotho_batch_create", "clotho_batch_destroy", "clotho_batch_last_error",
+    "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
+    "clotho_batch_set_timing",
 ]
+
+BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
 
 
 def lib():
@@ -158,6 +163,19 @@ def lib():
     L.clotho_trait_contract_synthetic.argtypes = [C.c_uint32, C.c_uint32, vp, C.c_uint64, vp,
                                                   C.c_uint32, C.c_int, C.c_int, C.c_int, vp,
                                                   C.POINTER(C.c_double), C.c_int]
+    L.clotho_batch_create.argtypes = [vp, C.c_uint32, C.c_int, C.POINTER(vp)]
+    L.clotho_batch_destroy.argtypes = [vp]
+    L.clotho_batch_destroy.restype = None
+    L.clotho_batch_last_error.argtypes = [vp]
+    L.clotho_batch_last_error.restype = C.c_char_p
+    L.clotho_batch_size.argtypes = [vp]
+    L.clotho_batch_size.restype = C.c_uint32
+    L.clotho_batch_replicate.argtypes = [vp, C.c_uint32]
+    L.clotho_batch_replicate.restype = vp
+    L.clotho_batch_run.argtypes = [vp, C.c_uint32, C.c_int, C.POINTER(C.c_double)]
+    L.clotho_batch_timings.argtypes = [vp, f64p, C.c_int]
+    L.clotho_batch_set_timing.argtypes = [vp, C.c_int]
+    L.clotho_batch_set_timing.restype = None
     _lib = L
     return L
 
@@ -344,3 +362,72 @@ def trait_contract_synthetic(h, freqs, effects, mode, seed=42, split_terms=3, re
     if rc != 0:
         raise ClothoError(f"trait_contract_synthetic failed ({ERRORS.get(rc, rc)})")
     return z, ms.value
+
+
+class _Replicate(Simulation):
+    """A replicate context borrowed from a Batch (the batch owns and frees it)."""
+
+    def __init__(self, cfg, ctx, owner):
+        self.cfg = cfg
+        self.ctx = C.c_void_p(ctx)
+        self._owner = owner
+
+    def close(self):
+        self.ctx = C.c_void_p()
+
+
+class Batch:
+    """Replicate populations advanced together by batched persistent kernels."""
+
+    def __init__(self, cfgs, device: int = 0):
+        cfgs = list(cfgs)
+        arr = (Config * len(cfgs))()
+        for i, c in enumerate(cfgs):
+            C.memmove(C.byref(arr[i]), C.byref(c), C.sizeof(Config))
+        self.cfgs = cfgs
+        self.h = C.c_void_p()
+        rc = lib().clotho_batch_create(arr, len(cfgs), device, C.byref(self.h))
+        if rc != 0:
+            raise ClothoError(f"clotho_batch_create failed ({ERRORS.get(rc, rc)})")
+        self._reps = [_Replicate(c, lib().clotho_batch_replicate(self.h, i), self)
+                      for i, c in enumerate(cfgs)]
+
+    def __len__(self):
+        return len(self._reps)
+
+    def replicate(self, r) -> Simulation:
+        return self._reps[r]
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().clotho_batch_last_error(self.h)
+            raise ClothoError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def run(self, n=1, mode="graph"):
+        """Advance every replicate n generations; returns device milliseconds."""
+        ms = C.c_double(0.0)
+        self._check(lib().clotho_batch_run(self.h, int(n), BATCH_MODES[mode], C.byref(ms)),
+                    f"batch_run({mode})")
+        return ms.value
+
+    def set_timing(self, on=True):
+        lib().clotho_batch_set_timing(self.h, 1 if on else 0)
+
+    def timings(self):
+        out = np.zeros(6, np.float64)
+        n = lib().clotho_batch_timings(self.h, out, 6)
+        return n, out
+
+    def close(self):
+        if self.h:
+            for r in self._reps:
+                r.close()
+            lib().clotho_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -72,6 +72,10 @@ void free_buffers(clotho_ctx* c) {
     std::memset(&B, 0, sizeof(B));
 }
 
+}  // namespace
+
+namespace clotho {
+
 void destroy_graphs(clotho_ctx* c) {
     for (int i = 0; i < 2; ++i)
         if (c->graphs.exec[i]) {
@@ -142,7 +146,7 @@ int capture_graphs(clotho_ctx* c) {
     return CLOTHO_OK;
 }
 
-}  // namespace
+}  // namespace clotho
 
 extern "C" {
 

@@ -17,6 +17,7 @@
 #include "device_common.hpp"
 #include "pipeline.hpp"
 #include "traits.hpp"
+#include "traits_i8.hpp"
 
 namespace clotho {
 namespace {
@@ -60,215 +61,266 @@ __device__ inline T block_inclusive_scan(T v, T* lds, T& total) {
 // Per-gamete mutation counts, Poisson(mu) by inversion (the reference's
 // Poisson(mu * 2N) total with uniform rows has the same law), and their
 // block-local exclusive scan.
-__global__ void __launch_bounds__(kBlock) k_mut_counts(Params P, const GenState* st,
-                                                       const double* cdf_mu, uint32_t* count,
-                                                       uint32_t* local, uint32_t* bsum) {
-    __shared__ uint32_t lds[kWaves];
-    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t gen = st->gen;
-    uint32_t c = 0;
-    if (o < P.h) {
-        const Draw4 d = philox_draw(P.seed, gen, S_MUTCOUNT, P.replicate, o, 0);
-        c = poisson_draw(cdf_mu, P.n_cdf_mu, u53(d.x[0], d.x[1]));
-        count[o] = c;
+struct MutCounts {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const double* cdf_mu = B.cdf_mu;
+        uint32_t* count = B.mut_count;
+        uint32_t* local = B.mut_local;
+        uint32_t* bsum = B.mut_bsum;
+        __shared__ uint32_t lds[kWaves];
+        const uint32_t o = blk * kBlock + threadIdx.x;
+        const uint32_t gen = st->gen;
+        uint32_t c = 0;
+        if (o < P.h) {
+            const Draw4 d = philox_draw(P.seed, gen, S_MUTCOUNT, P.replicate, o, 0);
+            c = poisson_draw(cdf_mu, P.n_cdf_mu, u53(d.x[0], d.x[1]));
+            count[o] = c;
+        }
+        uint32_t total;
+        const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
+        if (o < P.h) local[o] = inc - c;
+        if (threadIdx.x == 0) bsum[blk] = total;
     }
-    uint32_t total;
-    const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
-    if (o < P.h) local[o] = inc - c;
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
+};
 
 // Per-block max and all-equal flag of the parent fitness (selection set-up).
-__global__ void __launch_bounds__(kBlock) k_fitness_stats(Params P, const double* fit,
-                                                          double* stats) {
-    __shared__ double smax[kBlock];
-    __shared__ int seq[kBlock];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const double f0 = fit[0];
-    double m = 0.0;
-    int eq = 1;
-    if (i < P.n) {
-        const double f = fit[i];
-        m = f;
-        eq = (f == f0) ? 1 : 0;
-    }
-    smax[threadIdx.x] = m;
-    seq[threadIdx.x] = eq;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
-            seq[threadIdx.x] = seq[threadIdx.x] & seq[threadIdx.x + s];
+struct FitnessStats {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const double* fit = B.fitness[sel];
+        double* stats = B.fit_stats;
+        __shared__ double smax[kBlock];
+        __shared__ int seq[kBlock];
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const double f0 = fit[0];
+        double m = 0.0;
+        int eq = 1;
+        if (i < P.n) {
+            const double f = fit[i];
+            m = f;
+            eq = (f == f0) ? 1 : 0;
         }
+        smax[threadIdx.x] = m;
+        seq[threadIdx.x] = eq;
         __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+                seq[threadIdx.x] = seq[threadIdx.x] & seq[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            stats[2 * blk] = smax[0];
+            stats[2 * blk + 1] = seq[0] ? 1.0 : 0.0;
+        }
     }
-    if (threadIdx.x == 0) {
-        stats[2 * blockIdx.x] = smax[0];
-        stats[2 * blockIdx.x + 1] = seq[0] ? 1.0 : 0.0;
-    }
-}
+};
 
 // Single block: mutation offsets, child allele count / width (child_max_alleles),
 // capacity check, selection quantisation exponent.
-__global__ void __launch_bounds__(1024) k_prepare(Params P, GenState* st, uint32_t* mut_bsum,
-                                                  const double* stats) {
-    __shared__ uint32_t s_carry;
-    __shared__ uint32_t s_wave[16];
-    __shared__ double s_max[1024];
-    __shared__ int s_eq[1024];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    // exclusive scan of the per-block mutation totals, in place
-    for (uint32_t base = 0; base < P.mut_blocks; base += 1024) {
-        const uint32_t idx = base + tid;
-        const uint32_t v = idx < P.mut_blocks ? mut_bsum[idx] : 0;
-        uint32_t x = v;
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) s_wave[wave] = x;
+struct Prepare {
+    static constexpr int kThreads = 1024;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return 1u; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        GenState* st = B.state;
+        uint32_t* mut_bsum = B.mut_bsum;
+        const double* stats = B.fit_stats;
+        (void)blk;
+        __shared__ uint32_t s_carry;
+        __shared__ uint32_t s_wave[16];
+        __shared__ double s_max[1024];
+        __shared__ int s_eq[1024];
+        const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        if (tid == 0) s_carry = 0;
         __syncthreads();
-        uint32_t off = s_carry, tot = 0;
-        for (int q = 0; q < 16; ++q) {
-            if (q < wave) off += s_wave[q];
-            tot += s_wave[q];
-        }
-        if (idx < P.mut_blocks) mut_bsum[idx] = off + x - v;
-        __syncthreads();
-        if (tid == 0) s_carry += tot;
-        __syncthreads();
-    }
-    // selection: constant fitness -> uniform; else quantisation exponent
-    double m = 0.0;
-    int eq = 1;
-    for (uint32_t b = tid; b < st->sel_blocks; b += 1024) {
-        m = fmax(m, stats[2 * b]);
-        eq &= stats[2 * b + 1] != 0.0 ? 1 : 0;
-    }
-    s_max[tid] = m;
-    s_eq[tid] = eq;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (tid < s) {
-            s_max[tid] = fmax(s_max[tid], s_max[tid + s]);
-            s_eq[tid] &= s_eq[tid + s];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        const uint32_t M = s_carry;
-        const uint32_t F = st->free_count;
-        const uint32_t all_size = st->alleles + (M > F ? M - F : 0);
-        st->m = M;
-        uint32_t as = all_size;
-        if (all_size > P.s_cap) {
-            if (!st->error) {
-                st->error = 1;
-                st->error_gen = st->gen;
+        // exclusive scan of the per-block mutation totals, in place
+        for (uint32_t base = 0; base < P.mut_blocks; base += 1024) {
+            const uint32_t idx = base + tid;
+            const uint32_t v = idx < P.mut_blocks ? mut_bsum[idx] : 0;
+            uint32_t x = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
             }
-            as = P.s_cap;
+            if (lane == 63) s_wave[wave] = x;
+            __syncthreads();
+            uint32_t off = s_carry, tot = 0;
+            for (int q = 0; q < 16; ++q) {
+                if (q < wave) off += s_wave[q];
+                tot += s_wave[q];
+            }
+            if (idx < P.mut_blocks) mut_bsum[idx] = off + x - v;
+            __syncthreads();
+            if (tid == 0) s_carry += tot;
+            __syncthreads();
         }
-        st->all_size = as;
-        st->wc = ((as + 511) / 512) * 8;
-        const double mx = s_max[0];
-        int constant = s_eq[0] || !(mx > 0.0);
-        st->fit_constant = constant ? 1u : 0u;
-        if (!constant) {
-            int L = 0;
-            while (((uint64_t)1 << L) < P.n) ++L;
-            st->sel_shift = 52 - L - (det_ilogb(mx) + 1);
+        // selection: constant fitness -> uniform; else quantisation exponent
+        double m = 0.0;
+        int eq = 1;
+        for (uint32_t b = tid; b < st->sel_blocks; b += 1024) {
+            m = fmax(m, stats[2 * b]);
+            eq &= stats[2 * b + 1] != 0.0 ? 1 : 0;
+        }
+        s_max[tid] = m;
+        s_eq[tid] = eq;
+        __syncthreads();
+        for (int s = 512; s > 0; s >>= 1) {
+            if (tid < s) {
+                s_max[tid] = fmax(s_max[tid], s_max[tid + s]);
+                s_eq[tid] &= s_eq[tid + s];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const uint32_t M = s_carry;
+            const uint32_t F = st->free_count;
+            const uint32_t all_size = st->alleles + (M > F ? M - F : 0);
+            st->m = M;
+            uint32_t as = all_size;
+            if (all_size > P.s_cap) {
+                if (!st->error) {
+                    st->error = 1;
+                    st->error_gen = st->gen;
+                }
+                as = P.s_cap;
+            }
+            st->all_size = as;
+            st->wc = ((as + 511) / 512) * 8;
+            const double mx = s_max[0];
+            int constant = s_eq[0] || !(mx > 0.0);
+            st->fit_constant = constant ? 1u : 0u;
+            if (!constant) {
+                int L = 0;
+                while (((uint64_t)1 << L) < P.n) ++L;
+                st->sel_shift = 52 - L - (det_ilogb(mx) + 1);
+            }
         }
     }
-}
+};
 
 // New-allele attributes for every mutation of gamete o: slot = free[k] (fixed
 // ascending, then lost ascending) else appended (generate_child_mutations);
 // position uniform (AlleleGenerator), age = generation, per-trait effect
 // neutral ? 0 : quantised Normal(mean, sigma) (TraitSpaceGenerator2).
-__global__ void __launch_bounds__(kBlock) k_mut_attrs(Params P, const GenState* st,
-                                                      const uint32_t* count,
-                                                      const uint32_t* local,
-                                                      const uint32_t* boff,
-                                                      const uint32_t* free_list, uint32_t* keys,
-                                                      int32_t* effects, uint32_t* ages) {
-    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
-    if (o >= P.h) return;
-    const uint32_t c = count[o];
-    if (!c) return;
-    const uint32_t gen = st->gen, F = st->free_count, A = st->alleles;
-    const uint32_t k0 = local[o] + boff[o / kBlock];
-    for (uint32_t k = k0; k < k0 + c; ++k) {
-        const uint32_t slot = k < F ? free_list[k] : A + (k - F);
-        if (slot >= P.s_cap) continue;
-        const Draw4 d = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 0);
-        keys[slot] = d.x[0];
-        ages[slot] = gen;
-        for (uint32_t t = 0; t < P.traits; ++t) {
-            const Draw4 e = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 1 + t);
-            int32_t q = 0;
-            if (!(u53(e.x[0], e.x[1]) < P.neutral_p)) {
-                const double w = P.trait_mean + P.trait_sigma * normal_ppf(uopen(e.x[2], e.x[3]));
-                double n = rint(w * P.eff_scale);
-                if (n > kEffectLimit) n = kEffectLimit;
-                if (n < -kEffectLimit) n = -kEffectLimit;
-                q = (int32_t)n;
+struct MutAttrs {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const uint32_t* count = B.mut_count;
+        const uint32_t* local = B.mut_local;
+        const uint32_t* boff = B.mut_bsum;
+        const uint32_t* free_list = B.free_list;
+        uint32_t* keys = B.keys;
+        int32_t* effects = B.effects;
+        uint32_t* ages = B.ages;
+        const uint32_t o = blk * kBlock + threadIdx.x;
+        if (o >= P.h) return;
+        const uint32_t c = count[o];
+        if (!c) return;
+        const uint32_t gen = st->gen, F = st->free_count, A = st->alleles;
+        const uint32_t k0 = local[o] + boff[o / kBlock];
+        for (uint32_t k = k0; k < k0 + c; ++k) {
+            const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+            if (slot >= P.s_cap) continue;
+            const Draw4 d = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 0);
+            keys[slot] = d.x[0];
+            ages[slot] = gen;
+            for (uint32_t t = 0; t < P.traits; ++t) {
+                const Draw4 e = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 1 + t);
+                int32_t q = 0;
+                if (!(u53(e.x[0], e.x[1]) < P.neutral_p)) {
+                    const double w = P.trait_mean + P.trait_sigma * normal_ppf(uopen(e.x[2], e.x[3]));
+                    double n = rint(w * P.eff_scale);
+                    if (n > kEffectLimit) n = kEffectLimit;
+                    if (n < -kEffectLimit) n = -kEffectLimit;
+                    q = (int32_t)n;
+                }
+                effects[(size_t)slot * P.traits + t] = q;
             }
-            effects[(size_t)slot * P.traits + t] = q;
         }
     }
-}
+};
 
 // Quantised fitness q_i = floor(f_i 2^shift) and its block-local inclusive
 // scan; every partial sum is an integer below 2^52, exact in FP64.
-__global__ void __launch_bounds__(kBlock) k_sel_scan(Params P, const GenState* st,
-                                                     const double* fit, double* cum_local,
-                                                     double* cum_block) {
-    __shared__ double lds[kWaves];
-    if (st->fit_constant) return;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const int sh = st->sel_shift;
-    double q = 0.0;
-    if (i < P.n) {
-        const double f = fit[i];
-        q = floor(det_ldexp(f > 0.0 ? f : 0.0, sh));
+struct SelScan {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const double* fit = B.fitness[sel];
+        double* cum_local = B.cum_local;
+        double* cum_block = B.cum_block;
+        __shared__ double lds[kWaves];
+        if (st->fit_constant) return;
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const int sh = st->sel_shift;
+        double q = 0.0;
+        if (i < P.n) {
+            const double f = fit[i];
+            q = floor(det_ldexp(f > 0.0 ? f : 0.0, sh));
+        }
+        double total;
+        const double inc = block_inclusive_scan<double>(q, lds, total);
+        if (i < P.n) cum_local[i] = inc;
+        if (threadIdx.x == 0) cum_block[blk] = total;
     }
-    double total;
-    const double inc = block_inclusive_scan<double>(q, lds, total);
-    if (i < P.n) cum_local[i] = inc;
-    if (threadIdx.x == 0) cum_block[blockIdx.x] = total;
-}
+};
 
-__global__ void __launch_bounds__(1024) k_sel_blockscan(GenState* st, double* cum_block) {
-    __shared__ double s_wave[16];
-    __shared__ double s_carry;
-    if (st->fit_constant) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t nb = st->sel_blocks;
-    if (tid == 0) s_carry = 0.0;
-    __syncthreads();
-    for (uint32_t base = 0; base < nb; base += 1024) {
-        const uint32_t idx = base + tid;
-        double x = idx < nb ? cum_block[idx] : 0.0;
-        for (int d = 1; d < 64; d <<= 1) {
-            const double y = __shfl_up(x, d, 64);
-            if (lane >= d) x = x + y;
+struct SelBlockscan {
+    static constexpr int kThreads = 1024;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return 1u; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        GenState* st = B.state;
+        double* cum_block = B.cum_block;
+        (void)blk;
+        __shared__ double s_wave[16];
+        __shared__ double s_carry;
+        if (st->fit_constant) return;
+        const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        const uint32_t nb = st->sel_blocks;
+        if (tid == 0) s_carry = 0.0;
+        __syncthreads();
+        for (uint32_t base = 0; base < nb; base += 1024) {
+            const uint32_t idx = base + tid;
+            double x = idx < nb ? cum_block[idx] : 0.0;
+            for (int d = 1; d < 64; d <<= 1) {
+                const double y = __shfl_up(x, d, 64);
+                if (lane >= d) x = x + y;
+            }
+            if (lane == 63) s_wave[wave] = x;
+            __syncthreads();
+            double off = s_carry, tot = 0.0;
+            for (int q = 0; q < 16; ++q) {
+                if (q < wave) off = off + s_wave[q];
+                tot = tot + s_wave[q];
+            }
+            if (idx < nb) cum_block[idx] = off + x;
+            __syncthreads();
+            if (tid == 0) s_carry = s_carry + tot;
+            __syncthreads();
         }
-        if (lane == 63) s_wave[wave] = x;
-        __syncthreads();
-        double off = s_carry, tot = 0.0;
-        for (int q = 0; q < 16; ++q) {
-            if (q < wave) off = off + s_wave[q];
-            tot = tot + s_wave[q];
-        }
-        if (idx < nb) cum_block[idx] = off + x;
-        __syncthreads();
-        if (tid == 0) s_carry = s_carry + tot;
-        __syncthreads();
+        if (tid == 0) st->sel_total = (uint64_t)s_carry;
     }
-    if (tid == 0) st->sel_total = (uint64_t)s_carry;
-}
+};
 
 // Parent pick for one uniform: uniform over [0, N) when fitness is constant,
 // else min{i : target < C[i]} with target = floor(k * C[N-1] / 2^53).
@@ -302,31 +354,40 @@ __device__ inline uint32_t pick_parent(const Params& P, const GenState* st, uint
 // Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
 // count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
 // (common_crossover::operator(), fill_events), and the top/bottom parent rows.
-__global__ void __launch_bounds__(kBlock) k_gametes(Params P, const GenState* st,
-                                                    const double* cdf_rho,
-                                                    const double* cum_local,
-                                                    const double* cum_block, uint32_t* top_out,
-                                                    uint32_t* bot_out, uint32_t* nev_out,
-                                                    uint32_t* events) {
-    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
-    if (o >= P.h) return;
-    const uint32_t gen = st->gen;
-    const uint32_t i = o >> 1;
-    const Draw4 pd = philox_draw(P.seed, gen, S_PARENTS, P.replicate, i, 0);
-    const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
-    const uint32_t p = pick_parent(P, st, k, cum_local, cum_block);
-    const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
-    const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
-    const bool swap = u53(g.x[2], g.x[3]) < P.bias;
-    top_out[o] = swap ? 2 * p + 1 : 2 * p;
-    bot_out[o] = swap ? 2 * p : 2 * p + 1;
-    nev_out[o] = ne;
-    uint32_t* ev = events + (size_t)o * P.ev_cap;
-    for (uint32_t e = 0; e < ne; e += 4) {
-        const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
-        for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
+struct Gametes {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const double* cdf_rho = B.cdf_rho;
+        const double* cum_local = B.cum_local;
+        const double* cum_block = B.cum_block;
+        uint32_t* top_out = B.top;
+        uint32_t* bot_out = B.bottom;
+        uint32_t* nev_out = B.n_events;
+        uint32_t* events = B.events;
+        const uint32_t o = blk * kBlock + threadIdx.x;
+        if (o >= P.h) return;
+        const uint32_t gen = st->gen;
+        const uint32_t i = o >> 1;
+        const Draw4 pd = philox_draw(P.seed, gen, S_PARENTS, P.replicate, i, 0);
+        const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
+        const uint32_t p = pick_parent(P, st, k, cum_local, cum_block);
+        const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
+        const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
+        const bool swap = u53(g.x[2], g.x[3]) < P.bias;
+        top_out[o] = swap ? 2 * p + 1 : 2 * p;
+        bot_out[o] = swap ? 2 * p : 2 * p + 1;
+        nev_out[o] = ne;
+        uint32_t* ev = events + (size_t)o * P.ev_cap;
+        for (uint32_t e = 0; e < ne; e += 4) {
+            const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
+            for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
+        }
     }
-}
+};
 
 // ---------------------------------------------------------------------------
 // Recombination + mutation (+ free-space partials)
@@ -339,47 +400,56 @@ __global__ void __launch_bounds__(kBlock) k_gametes(Params P, const GenState* st
 // {j in w : key_j >= e} = ~pm[w][c(e)]; the crossover mask of the word,
 // {j : odd #{events e <= key_j}} (block_crossover + region_classifier's
 // upper_bound), is the XOR over events of ~pm[w][c(e)].
-__global__ void __launch_bounds__(kBlock) k_word_sort(Params P, const GenState* st,
-                                                      const uint32_t* keys, uint32_t* sorted,
-                                                      uint64_t* pm, uint32_t* bucket) {
-    __shared__ uint32_t s_key[kWaves][64];
-    __shared__ uint32_t s_perm[kWaves][64];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * kWaves + wave;
-    const uint32_t wc = st->wc, A = st->alleles;
-    if (w >= wc) return;  // uniform per wave; no block barrier follows
-    const uint32_t slot = w * 64 + lane;
-    const uint32_t key = slot < A ? keys[slot] : 0u;
-    s_key[wave][lane] = key;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    uint32_t rank = 0;
-    for (uint32_t k = 0; k < 64; ++k) {
-        const uint32_t v = s_key[wave][k];
-        rank += (v < key || (v == key && k < lane)) ? 1u : 0u;
+struct WordSort {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.w_cap + kWaves - 1) / kWaves; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const uint32_t* keys = B.keys;
+        uint32_t* sorted = B.sorted;
+        uint64_t* pm = B.pm;
+        uint32_t* bucket = B.bucket;
+        __shared__ uint32_t s_key[kWaves][64];
+        __shared__ uint32_t s_perm[kWaves][64];
+        const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const uint32_t w = blk * kWaves + wave;
+        const uint32_t wc = st->wc, A = st->alleles;
+        if (w >= wc) return;  // uniform per wave; no block barrier follows
+        const uint32_t slot = w * 64 + lane;
+        const uint32_t key = slot < A ? keys[slot] : 0u;
+        s_key[wave][lane] = key;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < 64; ++k) {
+            const uint32_t v = s_key[wave][k];
+            rank += (v < key || (v == key && k < lane)) ? 1u : 0u;
+        }
+        s_perm[wave][rank] = lane;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const uint32_t j = s_perm[wave][lane];  // slot bit of the lane-th smallest key
+        sorted[(size_t)w * 64 + lane] = s_key[wave][j];
+        uint64_t x = (uint64_t)1 << j;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t y = __shfl_up(x, d, 64);
+            if ((int)lane >= d) x |= y;
+        }
+        pm[(size_t)w * 65 + lane + 1] = x;
+        if (lane == 0) pm[(size_t)w * 65] = 0;
+        // buckets 4 lane .. 4 lane + 3: count of keys below b << 24
+        uint32_t packed = 0;
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t bound = (4 * lane + q) << 24;
+            uint32_t cnt = 0;
+            for (uint32_t k = 0; k < 64; ++k) cnt += s_key[wave][k] < bound ? 1u : 0u;
+            packed |= cnt << (8 * q);  // cnt <= 64 fits a byte
+        }
+        bucket[(size_t)w * 64 + lane] = packed;
     }
-    s_perm[wave][rank] = lane;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint32_t j = s_perm[wave][lane];  // slot bit of the lane-th smallest key
-    sorted[(size_t)w * 64 + lane] = s_key[wave][j];
-    uint64_t x = (uint64_t)1 << j;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t y = __shfl_up(x, d, 64);
-        if ((int)lane >= d) x |= y;
-    }
-    pm[(size_t)w * 65 + lane + 1] = x;
-    if (lane == 0) pm[(size_t)w * 65] = 0;
-    // buckets 4 lane .. 4 lane + 3: count of keys below b << 24
-    uint32_t packed = 0;
-    for (uint32_t q = 0; q < 4; ++q) {
-        const uint32_t bound = (4 * lane + q) << 24;
-        uint32_t cnt = 0;
-        for (uint32_t k = 0; k < 64; ++k) cnt += s_key[wave][k] < bound ? 1u : 0u;
-        packed |= cnt << (8 * q);  // cnt <= 64 fits a byte
-    }
-    bucket[(size_t)w * 64 + lane] = packed;
-}
+};
 
 // Grid: x = 64-word column tiles, y = row chunks; kRecWaves waves per block,
 // one 64-bit word per lane.  The tile's bucket table ([b/4][lane]), sorted
@@ -417,160 +487,176 @@ __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
 // operations in flight is the same on every path and the waits before a row's
 // compute cover only that row's loads.
 template <bool kWideEvents>
-__global__ void __launch_bounds__(kRecBlock) k_recombine(
-    Params P, const GenState* st, const uint64_t* __restrict__ parent,
-    uint64_t* __restrict__ child, const uint32_t* __restrict__ sorted,
-    const uint64_t* __restrict__ pm, const uint32_t* __restrict__ bucket,
-    const uint32_t* __restrict__ free_list, const uint64_t* __restrict__ fixed_bits,
-    const uint32_t* __restrict__ top_in, const uint32_t* __restrict__ bot_in,
-    const uint32_t* __restrict__ nev_in, const uint32_t* __restrict__ events,
-    const uint32_t* __restrict__ count, const uint32_t* __restrict__ local,
-    const uint32_t* __restrict__ boff, uint64_t* var_part, uint64_t* fix_part) {
-    __shared__ uint32_t s_bucket[64 * 64];
-    __shared__ uint32_t s_sorted[68 * 64];
-    __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
+struct Recombine {
+    static constexpr int kThreads = kRecBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.tiles * P.chunks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const uint64_t* __restrict__ parent = B.rows[sel];
+        uint64_t* __restrict__ child = B.rows[1 - sel];
+        const uint32_t* __restrict__ sorted = B.sorted;
+        const uint64_t* __restrict__ pm = B.pm;
+        const uint32_t* __restrict__ bucket = B.bucket;
+        const uint32_t* __restrict__ free_list = B.free_list;
+        const uint64_t* __restrict__ fixed_bits = B.fixed_bits;
+        const uint32_t* __restrict__ top_in = B.top;
+        const uint32_t* __restrict__ bot_in = B.bottom;
+        const uint32_t* __restrict__ nev_in = B.n_events;
+        const uint32_t* __restrict__ events = B.events;
+        const uint32_t* __restrict__ count = B.mut_count;
+        const uint32_t* __restrict__ local = B.mut_local;
+        const uint32_t* __restrict__ boff = B.mut_bsum;
+        uint64_t* var_part = B.var_part;
+        uint64_t* fix_part = B.fix_part;
+        const uint32_t bx = blk % P.tiles, by = blk / P.tiles;
+        __shared__ uint32_t s_bucket[64 * 64];
+        __shared__ uint32_t s_sorted[68 * 64];
+        __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
 
-    const uint32_t wc = st->wc;
-    const uint32_t w0 = blockIdx.x * 64;
-    if (w0 >= wc) return;  // uniform for the block
-    const uint32_t wp = st->wp;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t w = w0 + lane;
-    const bool valid = w < wc;
-    const bool inpar = valid && w < wp;
-    const uint32_t col = wp ? min(w, wp - 1) : 0;  // in-bounds parent column
-    const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
-    const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
+        const uint32_t wc = st->wc;
+        const uint32_t w0 = bx * 64;
+        if (w0 >= wc) return;  // uniform for the block
+        const uint32_t wp = st->wp;
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const uint32_t w = w0 + lane;
+        const bool valid = w < wc;
+        const bool inpar = valid && w < wp;
+        const uint32_t col = wp ? min(w, wp - 1) : 0;  // in-bounds parent column
+        const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
+        const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
 
-    for (uint32_t k = wave; k < 68; k += kRecWaves)
-        s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
-    for (uint32_t q = wave; q < 64; q += kRecWaves)
-        s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
-    for (uint32_t c = wave; c < 65; c += kRecWaves)
-        s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
-    const uint64_t keep = inpar ? ~fixed_bits[w] : 0;  // also clears columns >= wp
-    __syncthreads();
+        for (uint32_t k = wave; k < 68; k += kRecWaves)
+            s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
+        for (uint32_t q = wave; q < 64; q += kRecWaves)
+            s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
+        for (uint32_t c = wave; c < 65; c += kRecWaves)
+            s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
+        const uint64_t keep = inpar ? ~fixed_bits[w] : 0;  // also clears columns >= wp
+        __syncthreads();
 
-    const uint32_t rows_per_chunk = (P.h + gridDim.y - 1) / gridDim.y;
-    const uint32_t r0 = blockIdx.y * rows_per_chunk + wave;
-    const uint32_t r1 = min(P.h, blockIdx.y * rows_per_chunk + rows_per_chunk);
-    const uint32_t nrows = r1 > r0 ? (r1 - r0 + kRecWaves - 1) / kRecWaves : 0;
-    uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
+        const uint32_t rows_per_chunk = (P.h + P.chunks - 1) / P.chunks;
+        const uint32_t r0 = by * rows_per_chunk + wave;
+        const uint32_t r1 = min(P.h, by * rows_per_chunk + rows_per_chunk);
+        const uint32_t nrows = r1 > r0 ? (r1 - r0 + kRecWaves - 1) / kRecWaves : 0;
+        uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-    for (uint32_t j0 = 0; j0 < nrows; j0 += 64) {
-        const uint32_t nb = min(64u, nrows - j0);
-        // gamete records of rows j0 .. j0 + nb - 1, lane q = row j0 + q
-        auto row_of = [&](uint32_t q) { return r0 + kRecWaves * (j0 + q); };
-        const uint32_t ol = row_of(min(lane, nb - 1));
-        const uint32_t g_top = top_in[ol];
-        const uint32_t g_bot = bot_in[ol];
-        const uint32_t g_ne = nev_in[ol];
-        const uint32_t g_nm = count[ol];
-        const uint32_t g_k0 = local[ol] + boff[ol / kBlock];
+        for (uint32_t j0 = 0; j0 < nrows; j0 += 64) {
+            const uint32_t nb = min(64u, nrows - j0);
+            // gamete records of rows j0 .. j0 + nb - 1, lane q = row j0 + q
+            auto row_of = [&](uint32_t q) { return r0 + kRecWaves * (j0 + q); };
+            const uint32_t ol = row_of(min(lane, nb - 1));
+            const uint32_t g_top = top_in[ol];
+            const uint32_t g_bot = bot_in[ol];
+            const uint32_t g_ne = nev_in[ol];
+            const uint32_t g_nm = count[ol];
+            const uint32_t g_k0 = local[ol] + boff[ol / kBlock];
 
-        // rows past the batch reload its last row (in bounds, never processed)
-        auto load_row = [&](uint32_t q, uint64_t& t, uint64_t& b, uint32_t& lo, uint32_t& hi) {
-            const uint32_t qq = min(q, nb - 1);
-            const uint32_t* ev = events + (size_t)row_of(qq) * ev_cap;
-            lo = ev[min(lane, ev_cap - 1)];
-            if (kWideEvents) hi = ev[min(lane + 64, ev_cap - 1)];
-            t = parent[(size_t)lane_read(g_top, qq) * wp + col];
-            b = parent[(size_t)lane_read(g_bot, qq) * wp + col];
-        };
+            // rows past the batch reload its last row (in bounds, never processed)
+            auto load_row = [&](uint32_t q, uint64_t& t, uint64_t& b, uint32_t& lo, uint32_t& hi) {
+                const uint32_t qq = min(q, nb - 1);
+                const uint32_t* ev = events + (size_t)row_of(qq) * ev_cap;
+                lo = ev[min(lane, ev_cap - 1)];
+                if (kWideEvents) hi = ev[min(lane + 64, ev_cap - 1)];
+                t = parent[(size_t)lane_read(g_top, qq) * wp + col];
+                b = parent[(size_t)lane_read(g_bot, qq) * wp + col];
+            };
 
-        auto process = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
-            const uint32_t ne = lane_read(g_ne, q);
-            uint64_t m = 0;
-            for (uint32_t e = 0; e < ne; e += 4) {
-                uint32_t key[4], p[4];
-                bool more[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint32_t x = e + i;
-                    const uint32_t a = lane_read(lo, x & 63);
-                    key[i] = (kWideEvents && x >= 64) ? lane_read(hi, x & 63) : a;
-                }
-                uint32_t tb[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
-                uint32_t sk[4][4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const bool live = e + i < ne;
-                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) +
-                                       (sk[i][1] < key[i] ? 1u : 0u) +
-                                       (sk[i][2] < key[i] ? 1u : 0u);
-                    more[i] = live && sk[i][3] < key[i];
-                    p[i] = live ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
-                }
-                if (more[0] | more[1] | more[2] | more[3]) {
-                    // four or more keys of a bucket below an event: rare
-#pragma unroll
+            auto process = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
+                const uint32_t ne = lane_read(g_ne, q);
+                uint64_t m = 0;
+                for (uint32_t e = 0; e < ne; e += 4) {
+                    uint32_t key[4], p[4];
+                    bool more[4];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t x = e + i;
+                        const uint32_t a = lane_read(lo, x & 63);
+                        key[i] = (kWideEvents && x >= 64) ? lane_read(hi, x & 63) : a;
+                    }
+                    uint32_t tb[4];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                    uint32_t sk[4][4];
+    #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (more[i])
-                            while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
+    #pragma unroll
+                        for (int j = 0; j < 4; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
+    #pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const bool live = e + i < ne;
+                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) +
+                                           (sk[i][1] < key[i] ? 1u : 0u) +
+                                           (sk[i][2] < key[i] ? 1u : 0u);
+                        more[i] = live && sk[i][3] < key[i];
+                        p[i] = live ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
+                    }
+                    if (more[0] | more[1] | more[2] | more[3]) {
+                        // four or more keys of a bucket below an event: rare
+    #pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (more[i])
+                                while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
+                    }
+                    m ^= (s_pm[p[0] * 64 + lane] ^ s_pm[p[1] * 64 + lane]) ^
+                         (s_pm[p[2] * 64 + lane] ^ s_pm[p[3] * 64 + lane]);
                 }
-                m ^= (s_pm[p[0] * 64 + lane] ^ s_pm[p[1] * 64 + lane]) ^
-                     (s_pm[p[2] * 64 + lane] ^ s_pm[p[3] * 64 + lane]);
-            }
-            t &= keep;
-            b &= keep;
-            uint64_t c = (t & ~m) | (b & m);
-            // new mutations of this row (free columns: zero in every parent row)
-            const uint32_t nm = lane_read(g_nm, q);
-            if (nm) {
-                const uint32_t k0 = lane_read(g_k0, q);
-                for (uint32_t k = k0; k < k0 + nm; ++k) {
-                    const uint32_t slot = k < F ? free_list[k] : A + (k - F);
-                    if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+                t &= keep;
+                b &= keep;
+                uint64_t c = (t & ~m) | (b & m);
+                // new mutations of this row (free columns: zero in every parent row)
+                const uint32_t nm = lane_read(g_nm, q);
+                if (nm) {
+                    const uint32_t k0 = lane_read(g_k0, q);
+                    for (uint32_t k = k0; k < k0 + nm; ++k) {
+                        const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+                        if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+                    }
                 }
-            }
-            {
-                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
-                    child + (size_t)row_of(q) * wc, 0, (int)(wc * 8u), kBufferWord3);
-                typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-                const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
-                __builtin_amdgcn_raw_buffer_store_b64(v, row, valid ? w * 8u : 0x80000000u, 0,
-                                                      0);
-            }
-            acc_or |= c;
-            acc_and &= c;
-        };
+                {
+                    const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                        child + (size_t)row_of(q) * wc, 0, (int)(wc * 8u), kBufferWord3);
+                    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+                    const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
+                    __builtin_amdgcn_raw_buffer_store_b64(v, row, valid ? w * 8u : 0x80000000u, 0,
+                                                          0);
+                }
+                acc_or |= c;
+                acc_and &= c;
+            };
 
-        uint64_t tA = 0, bA = 0, tB = 0, bB = 0;
-        uint32_t loA = 0, hiA = 0, loB = 0, hiB = 0;
-        load_row(0, tA, bA, loA, hiA);
-        load_row(1, tB, bB, loB, hiB);
-        for (uint32_t q = 0; q < nb; q += 2) {
-            process(q, tA, bA, loA, hiA);
-            load_row(q + 2, tA, bA, loA, hiA);
-            if (q + 1 < nb) process(q + 1, tB, bB, loB, hiB);
-            load_row(q + 3, tB, bB, loB, hiB);
+            uint64_t tA = 0, bA = 0, tB = 0, bB = 0;
+            uint32_t loA = 0, hiA = 0, loB = 0, hiB = 0;
+            load_row(0, tA, bA, loA, hiA);
+            load_row(1, tB, bB, loB, hiB);
+            for (uint32_t q = 0; q < nb; q += 2) {
+                process(q, tA, bA, loA, hiA);
+                load_row(q + 2, tA, bA, loA, hiA);
+                if (q + 1 < nb) process(q + 1, tB, bB, loB, hiB);
+                load_row(q + 3, tB, bB, loB, hiB);
+            }
+        }
+        __syncthreads();  // every wave is past its last prefix-mask read
+        uint64_t* s_or = s_pm;
+        uint64_t* s_and = s_pm + kRecWaves * 64;
+        s_or[wave * 64 + lane] = acc_or;
+        s_and[wave * 64 + lane] = acc_and;
+        __syncthreads();
+        if (wave == 0 && valid) {
+            uint64_t vo = s_or[lane], va = s_and[lane];
+            for (int q = 1; q < kRecWaves; ++q) {
+                vo |= s_or[q * 64 + lane];
+                va &= s_and[q * 64 + lane];
+            }
+            var_part[(size_t)by * P.w_cap + w] = vo;
+            fix_part[(size_t)by * P.w_cap + w] = va;
         }
     }
-    __syncthreads();  // every wave is past its last prefix-mask read
-    uint64_t* s_or = s_pm;
-    uint64_t* s_and = s_pm + kRecWaves * 64;
-    s_or[wave * 64 + lane] = acc_or;
-    s_and[wave * 64 + lane] = acc_and;
-    __syncthreads();
-    if (wave == 0 && valid) {
-        uint64_t vo = s_or[lane], va = s_and[lane];
-        for (int q = 1; q < kRecWaves; ++q) {
-            vo |= s_or[q * 64 + lane];
-            va &= s_and[q * 64 + lane];
-        }
-        var_part[(size_t)blockIdx.y * P.w_cap + w] = vo;
-        fix_part[(size_t)blockIdx.y * P.w_cap + w] = va;
-    }
-}
+};
 
 // ---------------------------------------------------------------------------
 // Free space: reduce chunk partials per word, classify, ordered compaction
@@ -579,126 +665,143 @@ __global__ void __launch_bounds__(kRecBlock) k_recombine(
 // wave 0 classifies its word and scans the per-word counts.
 constexpr uint32_t kFreeWords = 64;
 
-__global__ void __launch_bounds__(kBlock) k_free_reduce(Params P, const GenState* st,
-                                                        const uint64_t* var_part,
-                                                        const uint64_t* fix_part,
-                                                        uint32_t chunks, uint64_t* fixed_bits,
-                                                        uint64_t* lost_bits, uint32_t* word_cnt,
-                                                        uint32_t* word_bsum,
-                                                        uint32_t word_blocks) {
-    __shared__ uint64_t s_or[kWaves][64];
-    __shared__ uint64_t s_and[kWaves][64];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t w = blockIdx.x * kFreeWords + lane;
-    const uint32_t wc = st->wc, S = st->all_size;
-    uint64_t vo = 0, va = ~(uint64_t)0;
-    if (w < wc) {
-        for (uint32_t c = wave; c < chunks; c += kWaves) {
-            vo |= var_part[(size_t)c * P.w_cap + w];
-            va &= fix_part[(size_t)c * P.w_cap + w];
+struct FreeReduce {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.word_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const uint64_t* var_part = B.var_part;
+        const uint64_t* fix_part = B.fix_part;
+        const uint32_t chunks = P.chunks;
+        uint64_t* fixed_bits = B.fixed_bits;
+        uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;
+        uint32_t* word_cnt = B.word_cnt;
+        uint32_t* word_bsum = B.word_bsum;
+        const uint32_t word_blocks = P.word_blocks;
+        __shared__ uint64_t s_or[kWaves][64];
+        __shared__ uint64_t s_and[kWaves][64];
+        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const uint32_t w = blk * kFreeWords + lane;
+        const uint32_t wc = st->wc, S = st->all_size;
+        uint64_t vo = 0, va = ~(uint64_t)0;
+        if (w < wc) {
+            for (uint32_t c = wave; c < chunks; c += kWaves) {
+                vo |= var_part[(size_t)c * P.w_cap + w];
+                va &= fix_part[(size_t)c * P.w_cap + w];
+            }
+        }
+        s_or[wave][lane] = vo;
+        s_and[wave][lane] = va;
+        __syncthreads();
+        if (wave) return;  // no barrier below
+        for (int q = 1; q < kWaves; ++q) {
+            vo |= s_or[q][lane];
+            va &= s_and[q][lane];
+        }
+        uint32_t cf = 0, cl = 0;
+        if (w < wc) {
+            const uint32_t lo = w * 64;
+            uint64_t validm = 0;
+            if (S > lo) validm = (S - lo >= 64) ? ~(uint64_t)0 : (((uint64_t)1 << (S - lo)) - 1);
+            const uint64_t fx = va & validm, ls = ~vo & validm;
+            cf = (uint32_t)__popcll(fx);
+            cl = (uint32_t)__popcll(ls);
+            fixed_bits[w] = fx;
+            lost_bits[w] = ls;
+        }
+        // wave-level exclusive scans of the fixed and lost counts
+        uint32_t xf = cf, xl = cl;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t yf = __shfl_up(xf, d, 64), yl = __shfl_up(xl, d, 64);
+            if ((int)lane >= d) {
+                xf += yf;
+                xl += yl;
+            }
+        }
+        if (w < wc) word_cnt[w] = ((xf - cf) & 0xFFFFu) | ((xl - cl) << 16);
+        if (lane == 63) {
+            word_bsum[blk] = xf;
+            word_bsum[word_blocks + blk] = xl;
         }
     }
-    s_or[wave][lane] = vo;
-    s_and[wave][lane] = va;
-    __syncthreads();
-    if (wave) return;  // no barrier below
-    for (int q = 1; q < kWaves; ++q) {
-        vo |= s_or[q][lane];
-        va &= s_and[q][lane];
-    }
-    uint32_t cf = 0, cl = 0;
-    if (w < wc) {
-        const uint32_t lo = w * 64;
-        uint64_t validm = 0;
-        if (S > lo) validm = (S - lo >= 64) ? ~(uint64_t)0 : (((uint64_t)1 << (S - lo)) - 1);
-        const uint64_t fx = va & validm, ls = ~vo & validm;
-        cf = (uint32_t)__popcll(fx);
-        cl = (uint32_t)__popcll(ls);
-        fixed_bits[w] = fx;
-        lost_bits[w] = ls;
-    }
-    // wave-level exclusive scans of the fixed and lost counts
-    uint32_t xf = cf, xl = cl;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t yf = __shfl_up(xf, d, 64), yl = __shfl_up(xl, d, 64);
-        if ((int)lane >= d) {
-            xf += yf;
-            xl += yl;
-        }
-    }
-    if (w < wc) word_cnt[w] = ((xf - cf) & 0xFFFFu) | ((xl - cl) << 16);
-    if (lane == 63) {
-        word_bsum[blockIdx.x] = xf;
-        word_bsum[word_blocks + blockIdx.x] = xl;
-    }
-}
+};
 
-__global__ void __launch_bounds__(kBlock) k_free_compact(GenState* st, const uint64_t* fixed_bits,
-                                                         const uint64_t* lost_bits,
-                                                         const uint32_t* word_cnt,
-                                                         const uint32_t* word_bsum,
-                                                         uint32_t word_blocks,
-                                                         uint32_t* free_list) {
-    __shared__ uint32_t s_off[2];
-    __shared__ uint32_t s_red[2][kBlock];
-    const uint32_t wc = st->wc;
-    const uint32_t nb = (wc + kFreeWords - 1) / kFreeWords;
-    if (blockIdx.x >= nb) return;
-    // offsets of this block and the fixed total, from the block totals
-    uint32_t pf = 0, tfix = 0, pl = 0;
-    for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
-        const uint32_t a = word_bsum[q], l = word_bsum[word_blocks + q];
-        tfix += a;
-        if (q < blockIdx.x) {
-            pf += a;
-            pl += l;
+struct FreeCompact {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.word_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        GenState* st = B.state;
+        const uint64_t* fixed_bits = B.fixed_bits;
+        const uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;
+        const uint32_t* word_cnt = B.word_cnt;
+        const uint32_t* word_bsum = B.word_bsum;
+        const uint32_t word_blocks = P.word_blocks;
+        uint32_t* free_list = B.free_list;
+        __shared__ uint32_t s_off[2];
+        __shared__ uint32_t s_red[2][kBlock];
+        const uint32_t wc = st->wc;
+        const uint32_t nb = (wc + kFreeWords - 1) / kFreeWords;
+        if (blk >= nb) return;
+        // offsets of this block and the fixed total, from the block totals
+        uint32_t pf = 0, tfix = 0, pl = 0;
+        for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
+            const uint32_t a = word_bsum[q], l = word_bsum[word_blocks + q];
+            tfix += a;
+            if (q < blk) {
+                pf += a;
+                pl += l;
+            }
         }
-    }
-    s_red[0][threadIdx.x] = pf;
-    s_red[1][threadIdx.x] = pl;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
-            s_red[1][threadIdx.x] += s_red[1][threadIdx.x + s];
+        s_red[0][threadIdx.x] = pf;
+        s_red[1][threadIdx.x] = pl;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+                s_red[1][threadIdx.x] += s_red[1][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            s_off[0] = s_red[0][0];
+            s_off[1] = s_red[1][0];
         }
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        s_off[0] = s_red[0][0];
-        s_off[1] = s_red[1][0];
-    }
-    __syncthreads();
-    s_red[0][threadIdx.x] = tfix;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+        s_red[0][threadIdx.x] = tfix;
         __syncthreads();
-    }
-    const uint32_t nfixed = s_red[0][0];
-    const uint32_t w = blockIdx.x * kFreeWords + threadIdx.x;
-    if (threadIdx.x < kFreeWords && w < wc) {
-        const uint32_t wcnt = word_cnt[w];
-        uint32_t pos = s_off[0] + (wcnt & 0xFFFFu);
-        uint64_t fx = fixed_bits[w];
-        while (fx) {
-            free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(fx);
-            fx &= fx - 1;
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+            __syncthreads();
         }
-        pos = nfixed + s_off[1] + (wcnt >> 16);
-        uint64_t ls = lost_bits[w];
-        while (ls) {
-            free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(ls);
-            ls &= ls - 1;
+        const uint32_t nfixed = s_red[0][0];
+        const uint32_t w = blk * kFreeWords + threadIdx.x;
+        if (threadIdx.x < kFreeWords && w < wc) {
+            const uint32_t wcnt = word_cnt[w];
+            uint32_t pos = s_off[0] + (wcnt & 0xFFFFu);
+            uint64_t fx = fixed_bits[w];
+            while (fx) {
+                free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(fx);
+                fx &= fx - 1;
+            }
+            pos = nfixed + s_off[1] + (wcnt >> 16);
+            uint64_t ls = lost_bits[w];
+            while (ls) {
+                free_list[pos++] = w * 64 + (uint32_t)__builtin_ctzll(ls);
+                ls &= ls - 1;
+            }
+        }
+        if (blk == nb - 1 && threadIdx.x == 0) {
+            uint32_t tl = 0;
+            for (uint32_t q = 0; q < nb; ++q) tl += word_bsum[word_blocks + q];
+            st->n_fixed = nfixed;
+            st->free_count = nfixed + tl;
         }
     }
-    if (blockIdx.x == nb - 1 && threadIdx.x == 0) {
-        uint32_t tl = 0;
-        for (uint32_t q = 0; q < nb; ++q) tl += word_bsum[word_blocks + q];
-        st->n_fixed = nfixed;
-        st->free_count = nfixed + tl;
-    }
-}
+};
 
 // Column OR/AND straight from a resident population (after an upload).
 __global__ void __launch_bounds__(kBlock) k_column_reduce(Params P, const GenState* st,
@@ -747,23 +850,31 @@ __device__ inline double fitness_of(const Params& P, const double* z) {
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_fitness(Params P, const GenState* st,
-                                                    const int64_t* traits, double* fit) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    const uint32_t gen = st->gen;
-    double z[kMaxTraits];
-    for (uint32_t t = 0; t < P.traits; ++t) {
-        const int64_t g = traits[(size_t)(2 * i) * P.traits + t] +
-                          traits[(size_t)(2 * i + 1) * P.traits + t];
-        z[t] = (double)g * P.eff_quantum;
-        if (P.env_sigma > 0.0) {
-            const Draw4 d = philox_draw(P.seed, gen, S_ENV, P.replicate, i, t);
-            z[t] = z[t] + P.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
+struct Fitness {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const GenState* st = B.state;
+        const int64_t* traits = B.traits[1 - sel];
+        double* fit = B.fitness[1 - sel];
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        if (i >= P.n) return;
+        const uint32_t gen = st->gen;
+        double z[kMaxTraits];
+        for (uint32_t t = 0; t < P.traits; ++t) {
+            const int64_t g = traits[(size_t)(2 * i) * P.traits + t] +
+                              traits[(size_t)(2 * i + 1) * P.traits + t];
+            z[t] = (double)g * P.eff_quantum;
+            if (P.env_sigma > 0.0) {
+                const Draw4 d = philox_draw(P.seed, gen, S_ENV, P.replicate, i, t);
+                z[t] = z[t] + P.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
+            }
         }
+        fit[i] = fitness_of(P, z);
     }
-    fit[i] = fitness_of(P, z);
-}
+};
 
 
 // Synthetic standing variation: bit j of row r set when Philox word
@@ -785,11 +896,17 @@ __global__ void __launch_bounds__(kBlock) k_synth_rows(Params P, uint64_t seed,
     rows[id] = x;
 }
 
-__global__ void k_commit(GenState* st) {
-    st->alleles = st->all_size;
-    st->wp = st->wc;
-    st->gen = st->gen + 1;
-}
+struct Commit {
+    static constexpr int kThreads = 64;
+    __host__ __device__ static uint32_t blocks(const Params&) { return 1u; }
+    __device__ static __forceinline__ void run(const Params&, const Buffers& B, uint32_t, uint32_t) {
+        if (threadIdx.x) return;
+        GenState* st = B.state;
+        st->alleles = st->all_size;
+        st->wp = st->wc;
+        st->gen = st->gen + 1;
+    }
+};
 
 __global__ void __launch_bounds__(kBlock) k_any_nonzero(const int32_t* effects, size_t n,
                                                         uint32_t* flag_out) {
@@ -807,6 +924,109 @@ __global__ void __launch_bounds__(kBlock) k_any_nonzero(const int32_t* effects, 
     if (threadIdx.x == 0) flag_out[blockIdx.x] = s[0];
 }
 
+// Exact genome traits of population `sel` (i8::traits_i8_block); the effect
+// digit planes are packed first (i8::pack_i8_block).
+struct PackI8 {
+    static constexpr int kThreads = 256;
+    __host__ __device__ static uint32_t ng_cap(const Params& P) {
+        return (P.w_cap + i8::kGroupWords - 1) / i8::kGroupWords;
+    }
+    __host__ __device__ static uint32_t blocks(const Params& P) {
+        return P.traits ? ((P.traits + 3) / 4) * ng_cap(P) * 4 : 0u;  // 1024 items per (nt, G)
+    }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t,
+                                               uint32_t blk) {
+        i8::pack_i8_block(B.effects, P.traits, (P.traits + 3) / 4, &B.state->wc,
+                          &B.state->all_size, ng_cap(P), B.digits, blk);
+    }
+};
+
+template <int RT, int NT>
+struct TraitsI8 {
+    static constexpr int kThreads = 256;
+    __host__ __device__ static uint32_t blocks(const Params& P) {
+        return P.traits ? (P.h + 64 * RT - 1) / (64 * RT) : 0u;
+    }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        i8::traits_i8_block<RT, NT>(B.rows[sel], P.h, &B.state->wc, P.traits, B.digits,
+                                    B.traits[sel], blk);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Kernels.  Single population: one block per unit of Op::blocks, arguments by
+// value.  Batched: a persistent grid walks the concatenated block ranges of
+// all replicates (offs[r] = first virtual block of replicate r), with the
+// replicate's Params and Buffers read from device memory; a barrier separates
+// consecutive virtual blocks so that shared memory is reused safely.
+// ---------------------------------------------------------------------------
+#define CLOTHO_SINGLE(name, Op)                                                                 \
+    __global__ void __launch_bounds__(Op::kThreads) name(Params P, Buffers B, uint32_t sel) {  \
+        Op::run(P, B, sel, blockIdx.x);                                                         \
+    }
+
+CLOTHO_SINGLE(k_mut_counts, MutCounts)
+CLOTHO_SINGLE(k_fitness_stats, FitnessStats)
+CLOTHO_SINGLE(k_prepare, Prepare)
+CLOTHO_SINGLE(k_mut_attrs, MutAttrs)
+CLOTHO_SINGLE(k_sel_scan, SelScan)
+CLOTHO_SINGLE(k_sel_blockscan, SelBlockscan)
+CLOTHO_SINGLE(k_gametes, Gametes)
+CLOTHO_SINGLE(k_word_sort, WordSort)
+using RecombineNarrow = Recombine<false>;  // at most 64 events per gamete
+using RecombineWide = Recombine<true>;
+CLOTHO_SINGLE(k_recombine, RecombineNarrow)
+CLOTHO_SINGLE(k_recombine_wide, RecombineWide)
+CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
+using Traits4 = TraitsI8<4, 1>;   // T <= 4
+using Traits8 = TraitsI8<2, 2>;   // T <= 8
+using Traits16 = TraitsI8<2, 4>;  // T <= 16
+using Traits32 = TraitsI8<1, 8>;  // T <= 32
+CLOTHO_SINGLE(k_traits_i8_pop, Traits4)
+CLOTHO_SINGLE(k_traits_i8_pop_t8, Traits8)
+CLOTHO_SINGLE(k_traits_i8_pop_t16, Traits16)
+CLOTHO_SINGLE(k_traits_i8_pop_t32, Traits32)
+CLOTHO_SINGLE(k_free_reduce, FreeReduce)
+CLOTHO_SINGLE(k_free_compact, FreeCompact)
+CLOTHO_SINGLE(k_fitness, Fitness)
+CLOTHO_SINGLE(k_commit, Commit)
+#undef CLOTHO_SINGLE
+
+// Occupancy floor of a batched kernel: the recombination kernel needs two
+// resident 1024-thread blocks per CU (8 waves per SIMD, <= 64 VGPRs) to keep
+// enough parent loads in flight.
+template <class Op>
+struct MinWaves {
+    static constexpr int value = 1;
+};
+template <bool W>
+struct MinWaves<Recombine<W>> {
+    static constexpr int value = 8;
+};
+
+template <class Op>
+__global__ void __launch_bounds__(Op::kThreads)
+    __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value))) k_batched(const RepView* __restrict__ views,
+                                                          const uint32_t* __restrict__ offs,
+                                                          uint32_t R, uint32_t sel) {
+    const uint32_t total = offs[R];
+    for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
+        // replicate: the largest r < R with offs[r] <= v (empty ranges are skipped)
+        uint32_t lo = 0, hi = R;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offs[mid] <= v)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const RepView& rv = views[lo];
+        Op::run(rv.P, rv.B, sel, v - offs[lo]);
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -818,112 +1038,159 @@ __global__ void __launch_bounds__(kBlock) k_any_nonzero(const int32_t* effects, 
         if (e_ != hipSuccess) return e_;            \
     } while (0)
 
+namespace {
+
+template <class Op>
+hipError_t launch_single(void (*kernel)(Params, Buffers, uint32_t), clotho_ctx* c, uint32_t sel) {
+    const uint32_t nb = Op::blocks(c->p);
+    if (!nb) return hipSuccess;
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(Op::kThreads), 0, c->stream, c->p, c->b, sel);
+    return hipGetLastError();
+}
+
+#define TRY_LAUNCH(expr)                     \
+    do {                                     \
+        const hipError_t e_ = (expr);        \
+        if (e_ != hipSuccess) return e_;     \
+    } while (0)
+
+}  // namespace
+
 hipError_t launch_draw(clotho_ctx* c) {
-    const Params& P = c->p;
-    Buffers& B = c->b;
-    const int par = c->cur;  // parent = current population
-    const uint32_t sel_blocks = (P.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_mut_counts, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.cdf_mu, B.mut_count, B.mut_local, B.mut_bsum);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fitness_stats, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P,
-                       B.fitness[par], B.fit_stats);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, c->stream, P, B.state, B.mut_bsum,
-                       B.fit_stats);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mut_attrs, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.mut_count, B.mut_local, B.mut_bsum, B.free_list, B.keys, B.effects,
-                       B.ages);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sel_scan, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.fitness[par], B.cum_local, B.cum_block);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sel_blockscan, dim3(1), dim3(1024), 0, c->stream, B.state, B.cum_block);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gametes, dim3(P.mut_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.cdf_rho, B.cum_local, B.cum_block, B.top, B.bottom, B.n_events,
-                       B.events);
-    LAUNCH_CHECK();
+    const uint32_t par = (uint32_t)c->cur;  // parent = current population
+    TRY_LAUNCH(launch_single<MutCounts>(k_mut_counts, c, par));
+    TRY_LAUNCH(launch_single<FitnessStats>(k_fitness_stats, c, par));
+    TRY_LAUNCH(launch_single<Prepare>(k_prepare, c, par));
+    TRY_LAUNCH(launch_single<MutAttrs>(k_mut_attrs, c, par));
+    TRY_LAUNCH(launch_single<SelScan>(k_sel_scan, c, par));
+    TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
+    TRY_LAUNCH(launch_single<Gametes>(k_gametes, c, par));
     return hipSuccess;
 }
 
 // Exact genome traits of population [which] (width st->wc, alleles st->all_size).
 static hipError_t launch_traits_of(clotho_ctx* c, int which) {
-    const Params& P = c->p;
-    Buffers& B = c->b;
-    if (!P.traits) return hipSuccess;
-    hipError_t e = launch_pack_i8(B.effects, P.traits, &B.state->wc, &B.state->all_size, P.w_cap,
-                                  B.digits, c->stream);
-    if (e != hipSuccess) return e;
-    return launch_traits_i8(B.rows[which], P.h, &B.state->wc, P.traits, B.digits,
-                            B.traits[which], c->stream);
+    if (!c->p.traits) return hipSuccess;
+    const uint32_t sel = (uint32_t)which;
+    TRY_LAUNCH(launch_single<PackI8>(k_pack_i8_pop, c, sel));
+    const uint32_t nt = (c->p.traits + 3) / 4;
+    if (nt <= 1) return launch_single<Traits4>(k_traits_i8_pop, c, sel);
+    if (nt <= 2) return launch_single<Traits8>(k_traits_i8_pop_t8, c, sel);
+    if (nt <= 4) return launch_single<Traits16>(k_traits_i8_pop_t16, c, sel);
+    return launch_single<Traits32>(k_traits_i8_pop_t32, c, sel);
 }
 
 hipError_t launch_recombine(clotho_ctx* c) {
-    const Params& P = c->p;
-    Buffers& B = c->b;
-    const int par = c->cur, chi = 1 - c->cur;
-    (void)chi;
-    hipLaunchKernelGGL(k_word_sort, dim3((P.w_cap + kWaves - 1) / kWaves), dim3(kBlock), 0,
-                       c->stream, P, B.state, B.keys, B.sorted, B.pm, B.bucket);
-    LAUNCH_CHECK();
-    auto kernel = P.ev_cap > 64 ? k_recombine<true> : k_recombine<false>;
-    hipLaunchKernelGGL(kernel, dim3(P.tiles, P.chunks), dim3(kRecBlock), 0, c->stream, P,
-                       B.state, B.rows[par], B.rows[chi], B.sorted, B.pm, B.bucket, B.free_list,
-                       B.fixed_bits, B.top, B.bottom, B.n_events, B.events, B.mut_count,
-                       B.mut_local, B.mut_bsum, B.var_part, B.fix_part);
-    LAUNCH_CHECK();
-    return hipSuccess;
+    const uint32_t par = (uint32_t)c->cur;
+    TRY_LAUNCH(launch_single<WordSort>(k_word_sort, c, par));
+    if (c->p.ev_cap > 64) return launch_single<RecombineWide>(k_recombine_wide, c, par);
+    return launch_single<RecombineNarrow>(k_recombine, c, par);
 }
 
 hipError_t launch_traits(clotho_ctx* c) { return launch_traits_of(c, 1 - c->cur); }
 
 hipError_t launch_free_space(clotho_ctx* c) {
-    const Params& P = c->p;
-    Buffers& B = c->b;
-    uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;  // spare row of partials
-    hipLaunchKernelGGL(k_free_reduce, dim3(P.word_blocks), dim3(kBlock), 0, c->stream, P,
-                       B.state, B.var_part, B.fix_part, P.chunks, B.fixed_bits, lost_bits,
-                       B.word_cnt, B.word_bsum, P.word_blocks);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_free_compact, dim3(P.word_blocks), dim3(kBlock), 0, c->stream, B.state,
-                       B.fixed_bits, lost_bits, B.word_cnt, B.word_bsum, P.word_blocks,
-                       B.free_list);
-    LAUNCH_CHECK();
-    return hipSuccess;
+    TRY_LAUNCH(launch_single<FreeReduce>(k_free_reduce, c, 0));
+    return launch_single<FreeCompact>(k_free_compact, c, 0);
 }
 
 hipError_t launch_fitness(clotho_ctx* c) {
-    const Params& P = c->p;
-    Buffers& B = c->b;
-    const int chi = 1 - c->cur;
-    const uint32_t sel_blocks = (P.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_fitness, dim3(sel_blocks), dim3(kBlock), 0, c->stream, P, B.state,
-                       B.traits[chi], B.fitness[chi]);
-    LAUNCH_CHECK();
-    return hipSuccess;
+    return launch_single<Fitness>(k_fitness, c, (uint32_t)c->cur);
 }
 
-hipError_t launch_commit(clotho_ctx* c) {
-    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, c->stream, c->b.state);
-    LAUNCH_CHECK();
-    return hipSuccess;
-}
+hipError_t launch_commit(clotho_ctx* c) { return launch_single<Commit>(k_commit, c, 0); }
 
 // After an upload into population [cur]: traits, free-space classification.
 hipError_t launch_population_analysis(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
     const int cur = c->cur;
-    {
-        const hipError_t e = launch_traits_of(c, cur);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k_column_reduce, dim3((P.w_cap + kBlock - 1) / kBlock, P.chunks), dim3(kBlock), 0,
-                       c->stream, P, B.state, B.rows[cur], B.var_part, B.fix_part);
-    LAUNCH_CHECK();
+    TRY_LAUNCH(launch_traits_of(c, cur));
+    hipLaunchKernelGGL(k_column_reduce, dim3((P.w_cap + kBlock - 1) / kBlock, P.chunks),
+                       dim3(kBlock), 0, c->stream, P, B.state, B.rows[cur], B.var_part,
+                       B.fix_part);
+    TRY_LAUNCH(hipGetLastError());
     return launch_free_space(c);
+}
+
+// ---------------------------------------------------------------------------
+// Batched replicates: one persistent launch per kernel of the generation.
+// ---------------------------------------------------------------------------
+void batch_op_blocks(const Params& P, uint32_t* out) {
+    out[kOpMutCounts] = MutCounts::blocks(P);
+    out[kOpFitnessStats] = FitnessStats::blocks(P);
+    out[kOpPrepare] = Prepare::blocks(P);
+    out[kOpMutAttrs] = MutAttrs::blocks(P);
+    out[kOpSelScan] = SelScan::blocks(P);
+    out[kOpSelBlockscan] = SelBlockscan::blocks(P);
+    out[kOpGametes] = Gametes::blocks(P);
+    out[kOpWordSort] = WordSort::blocks(P);
+    out[kOpRecombine] = RecombineNarrow::blocks(P);
+    out[kOpPackI8] = PackI8::blocks(P);
+    const uint32_t nt = (P.traits + 3) / 4;
+    out[kOpTraits] = nt <= 1 ? Traits4::blocks(P)
+                   : nt <= 4 ? Traits8::blocks(P)  // same rows per block as Traits16
+                             : Traits32::blocks(P);
+    out[kOpFreeReduce] = FreeReduce::blocks(P);
+    out[kOpFreeCompact] = FreeCompact::blocks(P);
+    out[kOpFitness] = Fitness::blocks(P);
+    out[kOpCommit] = Commit::blocks(P);
+}
+
+namespace {
+
+template <class Op>
+hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_t s) {
+    const uint32_t total = L.total[op];
+    if (!total) return hipSuccess;
+    const uint32_t per_cu = Op::kThreads >= 1024 ? 2u : 8u;
+    const uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
+    hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::kThreads), 0, s, L.views,
+                       L.offs + (size_t)op * (L.R + 1), L.R, sel);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream_t s,
+                                   hipEvent_t* marks) {
+    auto mark = [&](int i) { return marks ? hipEventRecord(marks[i], s) : hipSuccess; };
+    TRY_LAUNCH(mark(0));
+    TRY_LAUNCH(launch_batched<MutCounts>(L, kOpMutCounts, par, s));
+    TRY_LAUNCH(launch_batched<FitnessStats>(L, kOpFitnessStats, par, s));
+    TRY_LAUNCH(launch_batched<Prepare>(L, kOpPrepare, par, s));
+    TRY_LAUNCH(launch_batched<MutAttrs>(L, kOpMutAttrs, par, s));
+    TRY_LAUNCH(launch_batched<SelScan>(L, kOpSelScan, par, s));
+    TRY_LAUNCH(launch_batched<SelBlockscan>(L, kOpSelBlockscan, par, s));
+    TRY_LAUNCH(launch_batched<Gametes>(L, kOpGametes, par, s));
+    TRY_LAUNCH(mark(1));
+    TRY_LAUNCH(launch_batched<WordSort>(L, kOpWordSort, par, s));
+    if (L.wide_events)
+        TRY_LAUNCH(launch_batched<RecombineWide>(L, kOpRecombine, par, s));
+    else
+        TRY_LAUNCH(launch_batched<RecombineNarrow>(L, kOpRecombine, par, s));
+    TRY_LAUNCH(mark(2));
+    if (L.traits) {
+        const uint32_t chi = 1 - par;
+        TRY_LAUNCH(launch_batched<PackI8>(L, kOpPackI8, chi, s));
+        const uint32_t nt = (L.traits + 3) / 4;
+        if (nt <= 1)
+            TRY_LAUNCH(launch_batched<Traits4>(L, kOpTraits, chi, s));
+        else if (nt <= 2)
+            TRY_LAUNCH(launch_batched<Traits8>(L, kOpTraits, chi, s));
+        else if (nt <= 4)
+            TRY_LAUNCH(launch_batched<Traits16>(L, kOpTraits, chi, s));
+        else
+            TRY_LAUNCH(launch_batched<Traits32>(L, kOpTraits, chi, s));
+    }
+    TRY_LAUNCH(mark(3));
+    TRY_LAUNCH(launch_batched<FreeReduce>(L, kOpFreeReduce, par, s));
+    TRY_LAUNCH(launch_batched<FreeCompact>(L, kOpFreeCompact, par, s));
+    TRY_LAUNCH(mark(4));
+    TRY_LAUNCH(launch_batched<Fitness>(L, kOpFitness, par, s));
+    TRY_LAUNCH(launch_batched<Commit>(L, kOpCommit, par, s));
+    TRY_LAUNCH(mark(5));
+    return hipSuccess;
 }
 
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {

@@ -95,6 +95,43 @@ struct Buffers {
     GenState* state;
 };
 
+// One replicate as the batched kernels see it (device memory, read-only).
+struct RepView {
+    Params P;
+    Buffers B;
+};
+
+// Kernels of one generation, in launch order; the batched launch of kernel k
+// walks offs[k][0 .. R] (first virtual block of each replicate).
+enum BatchOp : int {
+    kOpMutCounts,
+    kOpFitnessStats,
+    kOpPrepare,
+    kOpMutAttrs,
+    kOpSelScan,
+    kOpSelBlockscan,
+    kOpGametes,
+    kOpWordSort,
+    kOpRecombine,
+    kOpPackI8,
+    kOpTraits,
+    kOpFreeReduce,
+    kOpFreeCompact,
+    kOpFitness,
+    kOpCommit,
+    kOpCount
+};
+
+struct BatchLaunch {
+    const RepView* views = nullptr;  // device [R]
+    const uint32_t* offs = nullptr;  // device [kOpCount][R + 1]
+    uint32_t R = 0;
+    uint32_t total[kOpCount] = {};   // offs[k][R]
+    uint32_t traits = 0;             // trait count shared by every replicate
+    bool wide_events = false;        // some replicate keeps more than 64 events per gamete
+    uint32_t cus = 1;                // compute units of the device
+};
+
 struct Timing {
     bool enabled = false;
     double ms[6] = {0, 0, 0, 0, 0, 0};
@@ -134,4 +171,13 @@ hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
 hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
                              uint32_t words);
+// Context helpers (capi.hip).
+void destroy_graphs(clotho_ctx* c);
+int read_state(clotho_ctx* c, GenState* st);
+int write_state(clotho_ctx* c, const GenState& st);
+int capture_graphs(clotho_ctx* c);  // one generation per parity, on the context stream
+// Batched replicates (all kernels of one generation for every replicate).
+void batch_op_blocks(const Params& P, uint32_t* out);  // kOpCount block counts
+hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream_t s,
+                                   hipEvent_t* marks);  // marks: 6 events or null
 }  // namespace clotho

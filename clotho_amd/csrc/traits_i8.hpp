@@ -1,0 +1,193 @@
+// traits_i8.hpp — exact genome traits on v_mfma_i32_16x16x64_i8 (device
+// code shared by the pipeline's single-population and batched launches and
+// the standalone contraction).
+//
+// Quantised int32 effects are written as four balanced base-256 digits in
+// [-128, 127]; genotype bits expand to int8 0/1; each 16x16x64 product
+// accumulates exactly in int32 and the digit planes recombine in int64, so
+// the sums equal the sequential FP64 sums of the reference bit for bit
+// (common_phenotype_accumulator,
+// include/clotho/data_spaces/phenotype_evaluator/common_phenotype_accumulator_row_block_alignment.hpp:47-66).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clotho {
+namespace i8 {
+
+typedef int i8x4 __attribute__((ext_vector_type(4)));
+
+// four genotype bits -> four int8 0/1 bytes
+__device__ inline uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
+
+__device__ inline int32_t balanced_digit(int32_t q, int p) {
+    int32_t d = 0;
+    for (int i = 0; i <= p; ++i) {
+        d = ((q + 128) & 255) - 128;
+        q = (q - d) >> 8;
+    }
+    return d;
+}
+
+// K-block order.  Rows are read 16 words (one 128-byte line) at a time per
+// row: lane group g = lane / 16 loads words 4g..4g+3 of the group with two
+// 16-byte loads, and K block q (of 16 per group) takes 16-bit slice q % 4 of
+// the lane's word q / 4.  The site order inside a K block is free as long as
+// the packed effects follow it, so no lane ever needs another lane's bits.
+constexpr uint32_t kGroupWords = 16;
+
+__device__ inline uint32_t i8_site(uint32_t G, uint32_t q, uint32_t g, uint32_t i) {
+    return 64 * (kGroupWords * G + 4 * g + q / 4) + 16 * (q & 3) + i;
+}
+
+// B fragments of the exact path: packed[((nt * NG + G) * 16 + q) * 64 + lane]
+// holds the 16 int8 digits for column n = lane % 16 (trait 4 nt + n / 4,
+// digit plane n % 4) and sites i8_site(G, q, lane / 16, i).
+__device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff, uint32_t T,
+                                              uint32_t ntiles, const uint32_t* p_words,
+                                              const uint32_t* p_alleles, uint32_t ng_cap,
+                                              uint4* __restrict__ packed, uint32_t block) {
+    const uint32_t W = *p_words, A = *p_alleles;
+    const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
+    const size_t id = (size_t)block * 256 + threadIdx.x;
+    const uint32_t lane = (uint32_t)(id & 63);
+    const uint32_t q = (uint32_t)((id >> 6) & 15);
+    const size_t gn = id >> 10;
+    const uint32_t G = (uint32_t)(gn % ng_cap), nt = (uint32_t)(gn / ng_cap);
+    if (nt >= ntiles || G >= NG) return;
+    const uint32_t n = lane & 15, g = lane >> 4;
+    const uint32_t t = 4 * nt + n / 4, p = n & 3;
+    uint32_t d[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t s = i8_site(G, q, g, i);
+        const int32_t v = (s < A && t < T) ? eff[(size_t)s * T + t] : 0;
+        d[i >> 2] |= ((uint32_t)balanced_digit(v, (int)p) & 0xFFu) << (8 * (i & 3));
+    }
+    packed[(((size_t)nt * NG + G) * 16 + q) * 64 + lane] = make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
+// With one N tile (T <= 4) the block stages each group's B fragments (16 KB)
+// in LDS, double buffered, so the four waves share one copy; the next
+// group's rows are loaded before the current group is multiplied.  Bits
+// expand to int8 0/1 through a 256-entry LDS table (byte -> 8 bytes).
+template <int RT, int NT>
+__device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows, uint32_t h,
+                                                const uint32_t* p_words, uint32_t T,
+                                                const uint4* __restrict__ packed,
+                                                int64_t* __restrict__ traits, uint32_t block) {
+    constexpr bool kLdsB = (NT == 1);
+    __shared__ uint4 s_b[kLdsB ? 2 * 16 * 64 : 1];
+    __shared__ uint2 s_lut[256];
+    const uint32_t W = *p_words;
+    const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t m = lane & 15, g = lane >> 4;
+    const uint32_t base = (block * 4 + wave) * (16 * RT);
+    {
+        const uint32_t v = threadIdx.x;
+        s_lut[v] = make_uint2(spread4(v & 15), spread4(v >> 4));
+    }
+    i8x4 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[r][nt] = i8x4{0, 0, 0, 0};
+    const uint64_t* rp[RT];
+    bool rv[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const uint32_t row = base + 16 * r + m;
+        rv[r] = row < h;
+        rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
+    }
+    auto load_rows = [&](uint32_t G, uint4 (&x)[RT][2]) {
+        const uint32_t w = kGroupWords * G + 4 * g;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            if (rv[r] && w < W) {
+                const uint4* src = reinterpret_cast<const uint4*>(rp[r] + w);
+                x[r][0] = src[0];
+                x[r][1] = src[1];
+            } else {
+                x[r][0] = make_uint4(0, 0, 0, 0);
+                x[r][1] = make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto stage_b = [&](uint32_t G) {
+        if (kLdsB && G < NG) {
+            uint4* dst = s_b + (G & 1) * 16 * 64;
+            const uint4* src = packed + (size_t)G * 16 * 64;
+            for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) dst[i] = src[i];
+        }
+    };
+    uint4 x[RT][2];
+    if (NG) load_rows(0, x);
+    stage_b(0);
+    __syncthreads();
+    for (uint32_t G = 0; G < NG; ++G) {
+        uint4 xn[RT][2];
+        if (G + 1 < NG) load_rows(G + 1, xn);
+        stage_b(G + 1);
+        const uint4* bg = kLdsB ? s_b + (G & 1) * 16 * 64 + lane
+                                : packed + (size_t)G * 16 * 64 + lane;
+#pragma unroll
+        for (uint32_t q = 0; q < 16; ++q) {
+            i8x4 a[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const uint4 v = x[r][q >> 3];
+                const uint32_t dw = ((q >> 1) & 3) == 0 ? v.x
+                                  : ((q >> 1) & 3) == 1 ? v.y
+                                  : ((q >> 1) & 3) == 2 ? v.z : v.w;
+                const uint32_t b16 = (q & 1) ? (dw >> 16) : (dw & 0xFFFFu);
+                const uint2 lo = s_lut[b16 & 0xFFu], hi = s_lut[b16 >> 8];
+                a[r] = i8x4{(int)lo.x, (int)lo.y, (int)hi.x, (int)hi.y};
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const uint4 bq = kLdsB ? bg[q * 64] : bg[((size_t)nt * NG * 16 + q) * 64];
+                const i8x4 bv = i8x4{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+                    acc[r][nt] =
+                        __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // staged B of group G + 1 complete; buffer G & 1 free
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            x[r][0] = xn[r][0];
+            x[r][1] = xn[r][1];
+        }
+    }
+    // recombine the digit planes: column n = 4 t_local + p held by lane m = n
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t c0 = acc[r][nt][i];
+                const int64_t c1 = __shfl_down((int)acc[r][nt][i], 1, 64);
+                const int64_t c2 = __shfl_down((int)acc[r][nt][i], 2, 64);
+                const int64_t c3 = __shfl_down((int)acc[r][nt][i], 3, 64);
+                const uint32_t row = base + 16 * r + 4 * g + i;
+                const uint32_t t = 4 * nt + m / 4;
+                if ((m & 3) == 0 && row < h && t < T)
+                    traits[(size_t)row * T + t] = c0 + (c1 << 8) + (c2 << 16) + (c3 << 24);
+            }
+        }
+    }
+}
+
+// Launch shape of traits_i8_block: rows per block and the (RT, NT) choice by
+// trait count.
+__host__ __device__ inline uint32_t traits_i8_rows_per_block(uint32_t ntiles) {
+    return ntiles <= 1 ? 256u : ntiles <= 4 ? 128u : 64u;
+}
+
+}  // namespace i8
+}  // namespace clotho

@@ -186,6 +186,38 @@ void clotho_set_timing(clotho_ctx* ctx, int enabled);
  * on the resident population and keeps the fastest.  State is unchanged. */
 int clotho_autotune(clotho_ctx* ctx, uint32_t* chosen_chunks);
 
+/* ---- replicate batches ------------------------------------------------- */
+/* Many independent populations of uneven size and parameters advanced
+ * together: each kernel of the generation runs once for the whole batch as a
+ * persistent launch over the concatenated blocks of every replicate, and a
+ * batched generation can be captured as one hipGraph.  No reference
+ * counterpart: the reference runs one Engine<...>::simulate per process
+ * (QS/engine_parallel_pipeline.hpp:172-239, driven by QS/qtlsim_main.cpp:117-171);
+ * a batch runs R of them, replicate r drawing the Philox streams of
+ * cfgs[r].replicate, so its results equal a lone context with that config.
+ * Every replicate must have the same trait count.  Replicate contexts are
+ * owned by the batch; borrow one with clotho_batch_replicate for uploads,
+ * downloads and shapes (not for clotho_run while it belongs to the batch). */
+typedef struct clotho_batch clotho_batch;
+#define CLOTHO_BATCH_GRAPH 0      /* batched kernels, one hipGraph per generation   */
+#define CLOTHO_BATCH_EAGER 1      /* batched kernels launched directly              */
+#define CLOTHO_BATCH_SEQUENTIAL 2 /* each replicate's own generation graph in turn  */
+int clotho_batch_create(const clotho_config* cfgs, uint32_t count, int device,
+                        clotho_batch** out);
+void clotho_batch_destroy(clotho_batch* batch);
+const char* clotho_batch_last_error(const clotho_batch* batch);
+uint32_t clotho_batch_size(const clotho_batch* batch);
+clotho_ctx* clotho_batch_replicate(clotho_batch* batch, uint32_t r);
+/* Advance every replicate by n generations and wait; elapsed_ms (may be NULL)
+ * receives the device time of the n generations (HIP events on the batch
+ * stream, graph capture excluded).  CLOTHO_ERR_CAPACITY names the first
+ * replicate whose allele table overflowed. */
+int clotho_batch_run(clotho_batch* batch, uint32_t n_generations, int mode, double* elapsed_ms);
+/* Per-phase kernel milliseconds of the last eager batched run, as
+ * clotho_timings; returns the number of generations timed. */
+int clotho_batch_timings(clotho_batch* batch, double* ms, int cap);
+void clotho_batch_set_timing(clotho_batch* batch, int enabled);
+
 /* ---- standalone trait contraction -------------------------------------- */
 /* z[h][t] = sum_j bit(h,j) * effects[j][t] over a device-resident genotype
  * matrix.  Host-buffer variant: uploads, runs `repeats` times, downloads z
