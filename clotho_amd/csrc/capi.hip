@@ -235,12 +235,13 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     }
     P.tiles = (P.w_cap + 63) / 64;
     {
-        // default launch shape: ~4096 recombination blocks; clotho_autotune
-        // may pick another row-chunk count up to max_chunks
+        // default launch shape: ~4096 recombination blocks, but at least ~32
+        // rows per wave (a block's order-statistic tables are loaded once per
+        // block); clotho_autotune may pick another row-chunk count up to max_chunks
         P.max_chunks = std::max<uint32_t>(1, std::min<uint32_t>(
                                                  (8192 + P.tiles - 1) / P.tiles, P.h / 8));
         uint32_t ch = (4096 + P.tiles - 1) / P.tiles;
-        ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, P.h / 16));
+        ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, (P.h + 511) / 512));
         P.chunks = std::min<uint32_t>(P.max_chunks, std::max<uint32_t>(1, ch));
     }
     P.mut_blocks = (P.h + kScanBlock - 1) / kScanBlock;

@@ -25,7 +25,28 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-__device__ inline uint32_t poisson_draw(const double* cdf, int n, double u) {
+// Buffers reach the batched kernels through device memory, so their pointers
+// carry no address space and the compiler emits FLAT accesses (which also
+// count against the LDS wait counter and stall every LDS wait behind the
+// parent-row prefetches).  Every kernel body therefore views its buffers
+// as global-address-space pointers.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// A block-uniform word of device memory, kept in a scalar register.  In the
+// persistent batched kernels the compiler cannot prove that GenState is not
+// clobbered by earlier stores of the same launch and reads it with vector
+// loads; making the value explicitly uniform keeps it out of VGPRs.
+template <class Ptr>
+__device__ __forceinline__ uint32_t uniform_read(Ptr p) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*p);
+}
+
+
+template <class Ptr>
+__device__ inline uint32_t poisson_draw(Ptr cdf, int n, double u) {
     uint32_t k = 0;
     while ((int)k + 1 < n && !(u < cdf[k])) ++k;
     return k;
@@ -67,11 +88,11 @@ struct MutCounts {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const double* cdf_mu = B.cdf_mu;
-        uint32_t* count = B.mut_count;
-        uint32_t* local = B.mut_local;
-        uint32_t* bsum = B.mut_bsum;
+        const auto st = gbl(B.state);
+        const auto cdf_mu = gbl(B.cdf_mu);
+        const auto count = gbl(B.mut_count);
+        const auto local = gbl(B.mut_local);
+        const auto bsum = gbl(B.mut_bsum);
         __shared__ uint32_t lds[kWaves];
         const uint32_t o = blk * kBlock + threadIdx.x;
         const uint32_t gen = st->gen;
@@ -95,8 +116,8 @@ struct FitnessStats {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const double* fit = B.fitness[sel];
-        double* stats = B.fit_stats;
+        const auto fit = gbl(B.fitness[sel]);
+        const auto stats = gbl(B.fit_stats);
         __shared__ double smax[kBlock];
         __shared__ int seq[kBlock];
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -133,9 +154,9 @@ struct Prepare {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        GenState* st = B.state;
-        uint32_t* mut_bsum = B.mut_bsum;
-        const double* stats = B.fit_stats;
+        const auto st = gbl(B.state);
+        const auto mut_bsum = gbl(B.mut_bsum);
+        const auto stats = gbl(B.fit_stats);
         (void)blk;
         __shared__ uint32_t s_carry;
         __shared__ uint32_t s_wave[16];
@@ -219,14 +240,14 @@ struct MutAttrs {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const uint32_t* count = B.mut_count;
-        const uint32_t* local = B.mut_local;
-        const uint32_t* boff = B.mut_bsum;
-        const uint32_t* free_list = B.free_list;
-        uint32_t* keys = B.keys;
-        int32_t* effects = B.effects;
-        uint32_t* ages = B.ages;
+        const auto st = gbl(B.state);
+        const auto count = gbl(B.mut_count);
+        const auto local = gbl(B.mut_local);
+        const auto boff = gbl(B.mut_bsum);
+        const auto free_list = gbl(B.free_list);
+        const auto keys = gbl(B.keys);
+        const auto effects = gbl(B.effects);
+        const auto ages = gbl(B.ages);
         const uint32_t o = blk * kBlock + threadIdx.x;
         if (o >= P.h) return;
         const uint32_t c = count[o];
@@ -263,10 +284,10 @@ struct SelScan {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const double* fit = B.fitness[sel];
-        double* cum_local = B.cum_local;
-        double* cum_block = B.cum_block;
+        const auto st = gbl(B.state);
+        const auto fit = gbl(B.fitness[sel]);
+        const auto cum_local = gbl(B.cum_local);
+        const auto cum_block = gbl(B.cum_block);
         __shared__ double lds[kWaves];
         if (st->fit_constant) return;
         const uint32_t i = blk * kBlock + threadIdx.x;
@@ -289,8 +310,8 @@ struct SelBlockscan {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        GenState* st = B.state;
-        double* cum_block = B.cum_block;
+        const auto st = gbl(B.state);
+        const auto cum_block = gbl(B.cum_block);
         (void)blk;
         __shared__ double s_wave[16];
         __shared__ double s_carry;
@@ -324,8 +345,9 @@ struct SelBlockscan {
 
 // Parent pick for one uniform: uniform over [0, N) when fitness is constant,
 // else min{i : target < C[i]} with target = floor(k * C[N-1] / 2^53).
-__device__ inline uint32_t pick_parent(const Params& P, const GenState* st, uint64_t k,
-                                       const double* cum_local, const double* cum_block) {
+template <class StPtr, class FPtr>
+__device__ inline uint32_t pick_parent(const Params& P, StPtr st, uint64_t k, FPtr cum_local,
+                                       FPtr cum_block) {
     if (st->fit_constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n);
     const double target = (double)__umul64hi(k << 11, st->sel_total);
     // first block whose inclusive total exceeds target
@@ -360,14 +382,14 @@ struct Gametes {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const double* cdf_rho = B.cdf_rho;
-        const double* cum_local = B.cum_local;
-        const double* cum_block = B.cum_block;
-        uint32_t* top_out = B.top;
-        uint32_t* bot_out = B.bottom;
-        uint32_t* nev_out = B.n_events;
-        uint32_t* events = B.events;
+        const auto st = gbl(B.state);
+        const auto cdf_rho = gbl(B.cdf_rho);
+        const auto cum_local = gbl(B.cum_local);
+        const auto cum_block = gbl(B.cum_block);
+        const auto top_out = gbl(B.top);
+        const auto bot_out = gbl(B.bottom);
+        const auto nev_out = gbl(B.n_events);
+        const auto events = gbl(B.events);
         const uint32_t o = blk * kBlock + threadIdx.x;
         if (o >= P.h) return;
         const uint32_t gen = st->gen;
@@ -381,7 +403,7 @@ struct Gametes {
         top_out[o] = swap ? 2 * p + 1 : 2 * p;
         bot_out[o] = swap ? 2 * p : 2 * p + 1;
         nev_out[o] = ne;
-        uint32_t* ev = events + (size_t)o * P.ev_cap;
+        const auto ev = events + (size_t)o * P.ev_cap;
         for (uint32_t e = 0; e < ne; e += 4) {
             const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
             for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
@@ -406,11 +428,11 @@ struct WordSort {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const uint32_t* keys = B.keys;
-        uint32_t* sorted = B.sorted;
-        uint64_t* pm = B.pm;
-        uint32_t* bucket = B.bucket;
+        const auto st = gbl(B.state);
+        const auto keys = gbl(B.keys);
+        const auto sorted = gbl(B.sorted);
+        const auto pm = gbl(B.pm);
+        const auto bucket = gbl(B.bucket);
         __shared__ uint32_t s_key[kWaves][64];
         __shared__ uint32_t s_perm[kWaves][64];
         const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -493,39 +515,40 @@ struct Recombine {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const uint64_t* __restrict__ parent = B.rows[sel];
-        uint64_t* __restrict__ child = B.rows[1 - sel];
-        const uint32_t* __restrict__ sorted = B.sorted;
-        const uint64_t* __restrict__ pm = B.pm;
-        const uint32_t* __restrict__ bucket = B.bucket;
-        const uint32_t* __restrict__ free_list = B.free_list;
-        const uint64_t* __restrict__ fixed_bits = B.fixed_bits;
-        const uint32_t* __restrict__ top_in = B.top;
-        const uint32_t* __restrict__ bot_in = B.bottom;
-        const uint32_t* __restrict__ nev_in = B.n_events;
-        const uint32_t* __restrict__ events = B.events;
-        const uint32_t* __restrict__ count = B.mut_count;
-        const uint32_t* __restrict__ local = B.mut_local;
-        const uint32_t* __restrict__ boff = B.mut_bsum;
-        uint64_t* var_part = B.var_part;
-        uint64_t* fix_part = B.fix_part;
+        const auto st = gbl(B.state);
+        const auto parent = gbl(B.rows[sel]);
+        const auto child = gbl(B.rows[1 - sel]);
+        const auto sorted = gbl(B.sorted);
+        const auto pm = gbl(B.pm);
+        const auto bucket = gbl(B.bucket);
+        const auto free_list = gbl(B.free_list);
+        const auto fixed_bits = gbl(B.fixed_bits);
+        const auto top_in = gbl(B.top);
+        const auto bot_in = gbl(B.bottom);
+        const auto nev_in = gbl(B.n_events);
+        const auto events = gbl(B.events);
+        const auto count = gbl(B.mut_count);
+        const auto local = gbl(B.mut_local);
+        const auto boff = gbl(B.mut_bsum);
+        const auto var_part = gbl(B.var_part);
+        const auto fix_part = gbl(B.fix_part);
         const uint32_t bx = blk % P.tiles, by = blk / P.tiles;
         __shared__ uint32_t s_bucket[64 * 64];
         __shared__ uint32_t s_sorted[68 * 64];
         __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
 
-        const uint32_t wc = st->wc;
+        const uint32_t wc = uniform_read(&st->wc);
         const uint32_t w0 = bx * 64;
         if (w0 >= wc) return;  // uniform for the block
-        const uint32_t wp = st->wp;
+        const uint32_t wp = uniform_read(&st->wp);
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const uint32_t w = w0 + lane;
         const bool valid = w < wc;
         const bool inpar = valid && w < wp;
         const uint32_t col = wp ? min(w, wp - 1) : 0;  // in-bounds parent column
-        const uint32_t A = st->alleles, F = st->free_count, S = st->all_size;
+        const uint32_t A = uniform_read(&st->alleles), F = uniform_read(&st->free_count),
+                       S = uniform_read(&st->all_size);
         const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
 
         for (uint32_t k = wave; k < 68; k += kRecWaves)
@@ -557,7 +580,7 @@ struct Recombine {
             // rows past the batch reload its last row (in bounds, never processed)
             auto load_row = [&](uint32_t q, uint64_t& t, uint64_t& b, uint32_t& lo, uint32_t& hi) {
                 const uint32_t qq = min(q, nb - 1);
-                const uint32_t* ev = events + (size_t)row_of(qq) * ev_cap;
+                const auto ev = events + (size_t)row_of(qq) * ev_cap;
                 lo = ev[min(lane, ev_cap - 1)];
                 if (kWideEvents) hi = ev[min(lane + 64, ev_cap - 1)];
                 t = parent[(size_t)lane_read(g_top, qq) * wp + col];
@@ -671,14 +694,14 @@ struct FreeReduce {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const uint64_t* var_part = B.var_part;
-        const uint64_t* fix_part = B.fix_part;
+        const auto st = gbl(B.state);
+        const auto var_part = gbl(B.var_part);
+        const auto fix_part = gbl(B.fix_part);
         const uint32_t chunks = P.chunks;
-        uint64_t* fixed_bits = B.fixed_bits;
-        uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;
-        uint32_t* word_cnt = B.word_cnt;
-        uint32_t* word_bsum = B.word_bsum;
+        const auto fixed_bits = gbl(B.fixed_bits);
+        const auto lost_bits = gbl(B.var_part) + (size_t)P.max_chunks * P.w_cap;
+        const auto word_cnt = gbl(B.word_cnt);
+        const auto word_bsum = gbl(B.word_bsum);
         const uint32_t word_blocks = P.word_blocks;
         __shared__ uint64_t s_or[kWaves][64];
         __shared__ uint64_t s_and[kWaves][64];
@@ -734,13 +757,13 @@ struct FreeCompact {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        GenState* st = B.state;
-        const uint64_t* fixed_bits = B.fixed_bits;
-        const uint64_t* lost_bits = B.var_part + (size_t)P.max_chunks * P.w_cap;
-        const uint32_t* word_cnt = B.word_cnt;
-        const uint32_t* word_bsum = B.word_bsum;
+        const auto st = gbl(B.state);
+        const auto fixed_bits = gbl(B.fixed_bits);
+        const auto lost_bits = gbl(B.var_part) + (size_t)P.max_chunks * P.w_cap;
+        const auto word_cnt = gbl(B.word_cnt);
+        const auto word_bsum = gbl(B.word_bsum);
         const uint32_t word_blocks = P.word_blocks;
-        uint32_t* free_list = B.free_list;
+        const auto free_list = gbl(B.free_list);
         __shared__ uint32_t s_off[2];
         __shared__ uint32_t s_red[2][kBlock];
         const uint32_t wc = st->wc;
@@ -856,9 +879,9 @@ struct Fitness {
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
-        const GenState* st = B.state;
-        const int64_t* traits = B.traits[1 - sel];
-        double* fit = B.fitness[1 - sel];
+        const auto st = gbl(B.state);
+        const auto traits = gbl(B.traits[1 - sel]);
+        const auto fit = gbl(B.fitness[1 - sel]);
         const uint32_t i = blk * kBlock + threadIdx.x;
         if (i >= P.n) return;
         const uint32_t gen = st->gen;
@@ -901,7 +924,7 @@ struct Commit {
     __host__ __device__ static uint32_t blocks(const Params&) { return 1u; }
     __device__ static __forceinline__ void run(const Params&, const Buffers& B, uint32_t, uint32_t) {
         if (threadIdx.x) return;
-        GenState* st = B.state;
+        const auto st = gbl(B.state);
         st->alleles = st->all_size;
         st->wp = st->wc;
         st->gen = st->gen + 1;
@@ -1021,6 +1044,7 @@ __global__ void __launch_bounds__(Op::kThreads)
             else
                 hi = mid;
         }
+        lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: descriptor reads stay scalar
         const RepView& rv = views[lo];
         Op::run(rv.P, rv.B, sel, v - offs[lo]);
         __syncthreads();

@@ -17,6 +17,16 @@ namespace clotho {
 namespace i8 {
 
 typedef int i8x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16-byte loads and stores
+typedef __attribute__((address_space(1))) u32x4 g_u32x4;
+
+// Callers may hold these buffers as generic pointers read from device memory;
+// viewing them in the global address space keeps every access a global (not
+// FLAT) instruction.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
 
 // four genotype bits -> four int8 0/1 bytes
 __device__ inline uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
@@ -44,11 +54,14 @@ __device__ inline uint32_t i8_site(uint32_t G, uint32_t q, uint32_t g, uint32_t 
 // B fragments of the exact path: packed[((nt * NG + G) * 16 + q) * 64 + lane]
 // holds the 16 int8 digits for column n = lane % 16 (trait 4 nt + n / 4,
 // digit plane n % 4) and sites i8_site(G, q, lane / 16, i).
-__device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff, uint32_t T,
+__device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff_in, uint32_t T,
                                               uint32_t ntiles, const uint32_t* p_words,
                                               const uint32_t* p_alleles, uint32_t ng_cap,
-                                              uint4* __restrict__ packed, uint32_t block) {
-    const uint32_t W = *p_words, A = *p_alleles;
+                                              uint4* __restrict__ packed_in, uint32_t block) {
+    const auto eff = as_global(eff_in);
+    const auto packed = as_global(packed_in);
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
+    const uint32_t A = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_alleles));
     const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
     const size_t id = (size_t)block * 256 + threadIdx.x;
     const uint32_t lane = (uint32_t)(id & 63);
@@ -64,7 +77,8 @@ __device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff, u
         const int32_t v = (s < A && t < T) ? eff[(size_t)s * T + t] : 0;
         d[i >> 2] |= ((uint32_t)balanced_digit(v, (int)p) & 0xFFu) << (8 * (i & 3));
     }
-    packed[(((size_t)nt * NG + G) * 16 + q) * 64 + lane] = make_uint4(d[0], d[1], d[2], d[3]);
+    *(g_u32x4*)(packed + (((size_t)nt * NG + G) * 16 + q) * 64 + lane) =
+        u32x4{d[0], d[1], d[2], d[3]};
 }
 
 // Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
@@ -73,14 +87,17 @@ __device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff, u
 // group's rows are loaded before the current group is multiplied.  Bits
 // expand to int8 0/1 through a 256-entry LDS table (byte -> 8 bytes).
 template <int RT, int NT>
-__device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows, uint32_t h,
+__device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows_in, uint32_t h,
                                                 const uint32_t* p_words, uint32_t T,
-                                                const uint4* __restrict__ packed,
-                                                int64_t* __restrict__ traits, uint32_t block) {
+                                                const uint4* __restrict__ packed_in,
+                                                int64_t* __restrict__ traits_in, uint32_t block) {
+    const auto rows = as_global(rows_in);
+    const auto packed = as_global(packed_in);
+    const auto traits = as_global(traits_in);
     constexpr bool kLdsB = (NT == 1);
-    __shared__ uint4 s_b[kLdsB ? 2 * 16 * 64 : 1];
+    __shared__ u32x4 s_b[kLdsB ? 2 * 16 * 64 : 1];
     __shared__ uint2 s_lut[256];
-    const uint32_t W = *p_words;
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
     const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t m = lane & 15, g = lane >> 4;
@@ -94,7 +111,7 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[r][nt] = i8x4{0, 0, 0, 0};
-    const uint64_t* rp[RT];
+    const __attribute__((address_space(1))) uint64_t* rp[RT];
     bool rv[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
@@ -102,43 +119,42 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
         rv[r] = row < h;
         rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
     }
-    auto load_rows = [&](uint32_t G, uint4 (&x)[RT][2]) {
+    auto load_rows = [&](uint32_t G, u32x4 (&x)[RT][2]) {
         const uint32_t w = kGroupWords * G + 4 * g;
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             if (rv[r] && w < W) {
-                const uint4* src = reinterpret_cast<const uint4*>(rp[r] + w);
+                const g_u32x4* src = (const g_u32x4*)(rp[r] + w);
                 x[r][0] = src[0];
                 x[r][1] = src[1];
             } else {
-                x[r][0] = make_uint4(0, 0, 0, 0);
-                x[r][1] = make_uint4(0, 0, 0, 0);
+                x[r][0] = u32x4{0, 0, 0, 0};
+                x[r][1] = u32x4{0, 0, 0, 0};
             }
         }
     };
     auto stage_b = [&](uint32_t G) {
         if (kLdsB && G < NG) {
-            uint4* dst = s_b + (G & 1) * 16 * 64;
-            const uint4* src = packed + (size_t)G * 16 * 64;
+            u32x4* dst = s_b + (G & 1) * 16 * 64;
+            const g_u32x4* src = (const g_u32x4*)(packed + (size_t)G * 16 * 64);
             for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) dst[i] = src[i];
         }
     };
-    uint4 x[RT][2];
+    const g_u32x4* packed4 = (const g_u32x4*)packed;
+    u32x4 x[RT][2];
     if (NG) load_rows(0, x);
     stage_b(0);
     __syncthreads();
     for (uint32_t G = 0; G < NG; ++G) {
-        uint4 xn[RT][2];
+        u32x4 xn[RT][2];
         if (G + 1 < NG) load_rows(G + 1, xn);
         stage_b(G + 1);
-        const uint4* bg = kLdsB ? s_b + (G & 1) * 16 * 64 + lane
-                                : packed + (size_t)G * 16 * 64 + lane;
 #pragma unroll
         for (uint32_t q = 0; q < 16; ++q) {
             i8x4 a[RT];
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
-                const uint4 v = x[r][q >> 3];
+                const u32x4 v = x[r][q >> 3];
                 const uint32_t dw = ((q >> 1) & 3) == 0 ? v.x
                                   : ((q >> 1) & 3) == 1 ? v.y
                                   : ((q >> 1) & 3) == 2 ? v.z : v.w;
@@ -148,7 +164,11 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
             }
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const uint4 bq = kLdsB ? bg[q * 64] : bg[((size_t)nt * NG * 16 + q) * 64];
+                u32x4 bq;
+                if constexpr (kLdsB)
+                    bq = s_b[(G & 1) * 16 * 64 + q * 64 + lane];
+                else
+                    bq = packed4[(((size_t)nt * NG + G) * 16 + q) * 64 + lane];
                 const i8x4 bv = i8x4{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
 #pragma unroll
                 for (int r = 0; r < RT; ++r)
