@@ -45,7 +45,8 @@ class Config(C.Structure):
         ("env_sigma", C.c_double),
         ("effect_quantum_log2", C.c_int32),
         ("max_alleles", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("compact_every", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -73,7 +74,7 @@ class Shape(C.Structure):
 
 def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, neutral_p=0.5,
                 mean=0.0, sigma=1.0, fitness_scale=1.0, env_sigma=0.0, quantum_log2=-24,
-                max_alleles=1 << 20, replicate=0):
+                max_alleles=1 << 20, replicate=0, compact=False):
     """Build a Config; defaults follow the reference's parameter defaults
     (sequence_bias.p 0.5, neutral.p 0.5, traits.mean 0, traits.sigma 1)."""
     c = Config()
@@ -92,6 +93,7 @@ def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, n
     c.env_sigma = env_sigma
     c.effect_quantum_log2 = quantum_log2
     c.max_alleles = max_alleles
+    c.compact_every = 1 if compact else 0
     return c
 
 
@@ -114,7 +116,7 @@ EXPORTS = [
     "clotho_autotune",
     "clotho_batch_create", "clotho_batch_destroy", "clotho_batch_last_error",
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
-    "clotho_batch_set_timing",
+    "clotho_batch_set_timing", "clotho_compact",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -151,7 +153,7 @@ def lib():
     for n in ("clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
               "clotho_phase_traits",
               "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit",
-              "clotho_sync"):
+              "clotho_sync", "clotho_compact"):
         getattr(L, n).argtypes = [vp]
     L.clotho_timings.argtypes = [vp, f64p, C.c_int]
     L.clotho_autotune.argtypes = [vp, C.POINTER(C.c_uint32)]
@@ -298,6 +300,9 @@ class Simulation:
                                                   ev.ctypes.data, mu.ctypes.data, h),
                     "download_gametes")
         return top, bot, ev, mu
+
+    def compact(self):
+        self._check(lib().clotho_compact(self.ctx), "compact")
 
     def autotune(self):
         """Pick the recombination row-chunk count by timing (state unchanged)."""

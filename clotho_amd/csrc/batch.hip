@@ -124,8 +124,9 @@ int clotho_batch_create(const clotho_config* cfgs, uint32_t count, int device,
                         clotho_batch** out) {
     if (!cfgs || !out || count == 0) return CLOTHO_ERR_ARG;
     *out = nullptr;
-    for (uint32_t r = 1; r < count; ++r)
-        if (cfgs[r].trait_count != cfgs[0].trait_count) return CLOTHO_ERR_ARG;
+    for (uint32_t r = 0; r < count; ++r)
+        if (cfgs[r].trait_count != cfgs[0].trait_count || cfgs[r].compact_every)
+            return CLOTHO_ERR_ARG;
     clotho_batch* b = new (std::nothrow) clotho_batch();
     if (!b) return CLOTHO_ERR_ARG;
     b->device = device;

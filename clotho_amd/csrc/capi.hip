@@ -113,9 +113,14 @@ int enqueue_generation(clotho_ctx* c, std::vector<hipEvent_t>* marks) {
     HIP_TRY(c, mark(4));
     HIP_TRY(c, launch_fitness(c));
     HIP_TRY(c, launch_commit(c));
+    // the child (population 1 - cur) is compacted back into population cur
+    if (c->cfg.compact_every) HIP_TRY(c, launch_compact(c, 1 - c->cur));
     HIP_TRY(c, mark(5));
     return CLOTHO_OK;
 }
+
+// Population index current after one generation from parent index cur.
+int after_generation(const clotho_ctx* c, int cur) { return c->cfg.compact_every ? cur : 1 - cur; }
 
 int capture_graphs(clotho_ctx* c) {
     if (c->graphs.valid) return CLOTHO_OK;
@@ -608,6 +613,14 @@ int clotho_phase_commit(clotho_ctx* c) {
     c->cur = 1 - c->cur;
     return CLOTHO_OK;
 }
+int clotho_compact(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, launch_compact(c, c->cur));
+    c->cur = 1 - c->cur;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return CLOTHO_OK;
+}
 int clotho_sync(clotho_ctx* c) {
     if (!c) return CLOTHO_ERR_ARG;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -642,7 +655,7 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
         }
         for (uint32_t g = 0; g < n; ++g) {
             HIP_TRY(c, hipGraphLaunch(c->graphs.exec[c->cur], c->stream));
-            c->cur = 1 - c->cur;
+            c->cur = after_generation(c, c->cur);
         }
         if (tm.enabled) {
             HIP_TRY(c, hipEventRecord(e1, c->stream));
@@ -665,7 +678,7 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
             }
             int rc = enqueue_generation(c, mk);
             if (rc) return rc;
-            c->cur = 1 - c->cur;
+            c->cur = after_generation(c, c->cur);
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         if (tm.enabled) {

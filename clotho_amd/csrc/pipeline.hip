@@ -847,6 +847,209 @@ __global__ void __launch_bounds__(kBlock) k_column_reduce(Params P, const GenSta
 }
 
 // ---------------------------------------------------------------------------
+// Column compaction: fixed and lost columns of the current population are
+// removed, the variable columns keep their order and are renumbered (no
+// reference counterpart; the reference reuses free slots instead,
+// QS/engine_parallel_pipeline.hpp:337-348, and its CUDA engine only clears
+// them, CU/data_spaces/population_space/population_space_helper_kernels.hpp:457-510).
+// Per word w the keep mask m_w = ~(fixed | lost) (valid slots only) is the
+// same for every row, so its compress plan is built once: dst[w] = kept bits
+// before word w, and the six move masks of the parallel bit compress of
+// Hacker's Delight (2nd ed., section 7-4, "compress"), after which packing a
+// row word is x &= m; x = (x ^ t) | (t >> 2^i) with t = x & mv_i, i = 0..5.
+// Scratch: keep masks in fix_part, dst in word_cnt, move masks in pm
+// ([6][w_cap]), compacted allele attributes in sorted / bucket / digits; all
+// of them are rebuilt by the next generation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t compress_bits(uint64_t x, const uint64_t (&mv)[6]) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const uint64_t t = x & mv[i];
+        x = (x ^ t) | (t >> (1 << i));
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(1024) k_compact_plan(Params P, Buffers B) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_carry;
+    const auto st = gbl(B.state);
+    const uint32_t W = uniform_read(&st->wp), A = uniform_read(&st->alleles);
+    const auto fixed_bits = gbl(B.fixed_bits);
+    const auto lost_bits = gbl(B.var_part) + (size_t)P.max_chunks * P.w_cap;
+    const auto keep_out = gbl(B.fix_part);
+    const auto dst = gbl(B.word_cnt);
+    const auto mv = gbl(B.pm);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < W; base += 1024) {
+        const uint32_t w = base + tid;
+        uint64_t keep = 0;
+        if (w < W) {
+            const uint32_t lo = w * 64;
+            const uint64_t valid =
+                A > lo ? (A - lo >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << (A - lo)) - 1)) : 0;
+            keep = ~(fixed_bits[w] | lost_bits[w]) & valid;
+            keep_out[w] = keep;
+            uint64_t m = keep, mk = ~m << 1;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                uint64_t mp = mk ^ (mk << 1);
+                mp ^= mp << 2;
+                mp ^= mp << 4;
+                mp ^= mp << 8;
+                mp ^= mp << 16;
+                mp ^= mp << 32;
+                const uint64_t v = mp & m;
+                mv[(size_t)i * P.w_cap + w] = v;
+                m = (m ^ v) | (v >> (1 << i));
+                mk &= ~mp;
+            }
+        }
+        const uint32_t cnt = (uint32_t)__popcll(keep);
+        uint32_t x = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        uint32_t off = s_carry, tot = 0;
+        for (int q = 0; q < 16; ++q) {
+            if (q < wave) off += s_wave[q];
+            tot += s_wave[q];
+        }
+        if (w < W) dst[w] = off + x - cnt;
+        __syncthreads();
+        if (tid == 0) s_carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) st->kept = s_carry;
+}
+
+// Kept slot j moves to dst[j/64] + popcount(keep below j).
+__global__ void __launch_bounds__(kBlock) k_compact_alleles(Params P, Buffers B) {
+    const auto st = gbl(B.state);
+    const uint32_t A = uniform_read(&st->alleles);
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= A) return;
+    const uint64_t keep = gbl(B.fix_part)[j / 64];
+    if (!((keep >> (j % 64)) & 1)) return;
+    const uint32_t n =
+        gbl(B.word_cnt)[j / 64] + (uint32_t)__popcll(keep & (((uint64_t)1 << (j % 64)) - 1));
+    gbl(B.sorted)[n] = gbl(B.keys)[j];
+    gbl(B.bucket)[n] = gbl(B.ages)[j];
+    const auto eff = gbl(B.effects);
+    const auto tmp = gbl((int32_t*)B.digits);
+    for (uint32_t t = 0; t < P.traits; ++t) tmp[(size_t)n * P.traits + t] = eff[(size_t)j * P.traits + t];
+}
+
+// Rows: block = one 64-word tile of the compacted row x a chunk of rows; each
+// wave packs one row at a time.  The input words whose kept bits fall in the
+// tile are [wa, wb) (dst is non-decreasing); every input word contributes its
+// packed bits to at most two output words of the tile, OR-ed into a per-wave
+// LDS row (no global atomics).
+constexpr int kCompactWaves = 4;
+
+__global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, Buffers B,
+                                                                     uint32_t src) {
+    __shared__ unsigned long long s_out[kCompactWaves][64];
+    const auto st = gbl(B.state);
+    const uint32_t V = uniform_read(&st->kept), W = uniform_read(&st->wp);
+    const uint32_t W2 = ((V + 511) / 512) * 8;
+    const uint32_t tile = blockIdx.x % P.tiles, chunk = blockIdx.x / P.tiles;
+    if (tile * 64 >= W2) return;  // uniform for the block
+    const auto in = gbl(B.rows[src]);
+    const auto out = gbl(B.rows[1 - src]);
+    const auto keepm = gbl(B.fix_part);
+    const auto dst = gbl(B.word_cnt);
+    const auto mvg = gbl(B.pm);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lo_bit = tile * 4096, hi_bit = lo_bit + 4096;
+    // wa = last word with dst <= lo_bit, wb = first word with dst >= hi_bit
+    uint32_t a = 0, b = W;
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (dst[mid] <= lo_bit) a = mid; else b = mid;
+    }
+    const uint32_t wa = a;
+    a = wa;
+    b = W;
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (dst[mid] >= hi_bit) b = mid; else a = mid + 1;
+    }
+    const uint32_t wb = a;
+    const uint32_t rows_per_chunk = (P.h + gridDim.x / P.tiles - 1) / (gridDim.x / P.tiles);
+    const uint32_t r0 = chunk * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
+    for (uint32_t o = r0 + wave; o < r1; o += kCompactWaves) {
+        s_out[wave][lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t base = wa; base < wb; base += 64) {
+            const uint32_t w = base + lane;
+            if (w >= wb) continue;
+            const uint64_t keep = keepm[w];
+            const uint32_t n = (uint32_t)__popcll(keep);
+            if (!n) continue;
+            uint64_t mv[6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) mv[i] = mvg[(size_t)i * P.w_cap + w];
+            const uint64_t x = compress_bits(in[(size_t)o * W + w] & keep, mv);
+            const int d = (int)dst[w] - (int)lo_bit;  // tile-relative bit of the packed word
+            if (d >= 0) {
+                const uint32_t u0 = (uint32_t)d >> 6, s = (uint32_t)d & 63;
+                if (u0 < 64) atomicOr(&s_out[wave][u0], (unsigned long long)(x << s));
+                if (s && s + n > 64 && u0 + 1 < 64)
+                    atomicOr(&s_out[wave][u0 + 1], (unsigned long long)(x >> (64 - s)));
+            } else if (-d < (int)n) {
+                atomicOr(&s_out[wave][0], (unsigned long long)(x >> (-d)));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const uint32_t u = tile * 64 + lane;
+        if (u < W2) out[(size_t)o * W2 + u] = s_out[wave][lane];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// Allele attributes back into place (tail zeroed), free-space bits cleared,
+// traits and fitness carried to the population's new buffer index.
+__global__ void __launch_bounds__(kBlock) k_compact_finish(Params P, Buffers B, uint32_t src) {
+    const auto st = gbl(B.state);
+    const uint32_t V = uniform_read(&st->kept), A = uniform_read(&st->alleles);
+    const uint32_t W = uniform_read(&st->wp);
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < A) {
+        gbl(B.keys)[i] = i < V ? gbl(B.sorted)[i] : 0u;
+        gbl(B.ages)[i] = i < V ? gbl(B.bucket)[i] : 0u;
+        const auto tmp = gbl((int32_t*)B.digits);
+        for (uint32_t t = 0; t < P.traits; ++t)
+            gbl(B.effects)[i * P.traits + t] = i < V ? tmp[i * P.traits + t] : 0;
+    }
+    if (i < W) {
+        gbl(B.fixed_bits)[i] = 0;
+        (gbl(B.var_part) + (size_t)P.max_chunks * P.w_cap)[i] = 0;
+    }
+    if (i < (size_t)P.h * P.traits) gbl(B.traits[1 - src])[i] = gbl(B.traits[src])[i];
+    if (i < P.n) gbl(B.fitness[1 - src])[i] = gbl(B.fitness[src])[i];
+}
+
+__global__ void k_compact_state(Buffers B) {
+    GenState* st = B.state;
+    const uint32_t V = st->kept;
+    st->alleles = V;
+    st->all_size = V;
+    st->wp = st->wc = ((V + 511) / 512) * 8;
+    st->free_count = 0;
+    st->n_fixed = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Fitness: GeneralFitness over individuals, z = genome(2i) + genome(2i+1)
 // ---------------------------------------------------------------------------
 __device__ inline double fitness_of(const Params& P, const double* z) {
@@ -1215,6 +1418,26 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
     TRY_LAUNCH(launch_batched<Commit>(L, kOpCommit, par, s));
     TRY_LAUNCH(mark(5));
     return hipSuccess;
+}
+
+hipError_t launch_compact(clotho_ctx* c, int src) {
+    const Params& P = c->p;
+    const uint32_t s = (uint32_t)src;
+    hipLaunchKernelGGL(k_compact_plan, dim3(1), dim3(1024), 0, c->stream, P, c->b);
+    TRY_LAUNCH(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_alleles, dim3((P.s_cap + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       c->stream, P, c->b);
+    TRY_LAUNCH(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_rows, dim3(P.tiles * P.chunks), dim3(64 * kCompactWaves), 0,
+                       c->stream, P, c->b, s);
+    TRY_LAUNCH(hipGetLastError());
+    const size_t n = std::max<size_t>(std::max<size_t>(P.s_cap, (size_t)P.h * P.traits),
+                                      std::max<size_t>(P.n, P.w_cap));
+    hipLaunchKernelGGL(k_compact_finish, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, c->stream, P, c->b, s);
+    TRY_LAUNCH(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_state, dim3(1), dim3(1), 0, c->stream, c->b);
+    return hipGetLastError();
 }
 
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {

@@ -35,6 +35,7 @@ struct GenState {
     uint32_t error_gen;    // generation of the first error
     uint32_t sel_blocks;   // number of fitness scan blocks
     uint64_t sel_total;    // sum of quantised fitness
+    uint32_t kept;         // variable columns kept by the last compaction plan
 };
 
 // Kernel-side constants that do not change between generations.
@@ -168,6 +169,9 @@ hipError_t launch_free_space(clotho_ctx* c);
 hipError_t launch_fitness(clotho_ctx* c);
 hipError_t launch_commit(clotho_ctx* c);
 hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
+// Column compaction of population [src] into [1 - src] (fixed and lost
+// columns removed, slots renumbered in order); the caller flips cur.
+hipError_t launch_compact(clotho_ctx* c, int src);
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
 hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
                              uint32_t words);
@@ -176,6 +180,7 @@ void destroy_graphs(clotho_ctx* c);
 int read_state(clotho_ctx* c, GenState* st);
 int write_state(clotho_ctx* c, const GenState& st);
 int capture_graphs(clotho_ctx* c);  // one generation per parity, on the context stream
+int after_generation(const clotho_ctx* c, int cur);  // population index after a generation
 // Batched replicates (all kernels of one generation for every replicate).
 void batch_op_blocks(const Params& P, uint32_t* out);  // kOpCount block counts
 hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream_t s,

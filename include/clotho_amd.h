@@ -97,7 +97,10 @@ typedef struct clotho_config {
     double env_sigma;         /* s.d. of environmental noise added per individual (0=off)*/
     int32_t effect_quantum_log2; /* effects are integer multiples of 2^q (e.g. -24)      */
     uint32_t max_alleles;     /* allele-slot capacity S_cap                              */
-    uint32_t reserved[4];
+    uint32_t compact_every;   /* 1: compact fixed and lost columns away after every
+                                 generation (clotho_compact); 0: keep the reference's
+                                 slot reuse                                              */
+    uint32_t reserved[3];
 } clotho_config;
 
 /* Shape and bookkeeping of the current (child) population after a generation. */
@@ -173,6 +176,15 @@ int clotho_phase_free_space(clotho_ctx* ctx);
 int clotho_phase_fitness(clotho_ctx* ctx);
 int clotho_phase_commit(clotho_ctx* ctx);
 int clotho_sync(clotho_ctx* ctx);
+/* Remove every fixed or lost column of the current population: variable
+ * slots keep their order and are renumbered 0..V-1, rows shrink to
+ * ceil(V/512)*8 words, positions/effects/ages follow their slots, the free
+ * lists become empty; traits and fitness are unchanged.  Synchronous.  No
+ * reference counterpart (the reference reuses free slots,
+ * QS/engine_parallel_pipeline.hpp:337-348); cfg.compact_every runs it after
+ * every generation of clotho_run / clotho_generation (phase-level callers
+ * call it themselves). */
+int clotho_compact(clotho_ctx* ctx);
 
 /* Kernel timing over the last clotho_run: milliseconds summed over the
  * generations for [0] whole generations, [1] draw, [2] recombine+mutate,
@@ -195,7 +207,7 @@ int clotho_autotune(clotho_ctx* ctx, uint32_t* chosen_chunks);
  * (QS/engine_parallel_pipeline.hpp:172-239, driven by QS/qtlsim_main.cpp:117-171);
  * a batch runs R of them, replicate r drawing the Philox streams of
  * cfgs[r].replicate, so its results equal a lone context with that config.
- * Every replicate must have the same trait count.  Replicate contexts are
+ * Every replicate must have the same trait count and compact_every = 0.  Replicate contexts are
  * owned by the batch; borrow one with clotho_batch_replicate for uploads,
  * downloads and shapes (not for clotho_run while it belongs to the batch). */
 typedef struct clotho_batch clotho_batch;

@@ -73,6 +73,8 @@ def lib():
     L.or_sim_set_threads.argtypes = [C.c_void_p, C.c_int]
     L.or_sim_step.argtypes = [C.c_void_p]
     L.or_sim_step.restype = C.c_int
+    L.or_sim_compact.argtypes = [C.c_void_p]
+    L.or_sim_compact.restype = C.c_int
     L.or_sim_shape.argtypes = [C.c_void_p, C.c_void_p]
     L.or_sim_rows.argtypes = [C.c_void_p, u64p]
     L.or_sim_alleles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -233,6 +235,9 @@ class OracleSim:
         rc = lib().or_sim_step(self.h)
         if rc != 0:
             raise RuntimeError(f"oracle step failed: {rc}")
+
+    def compact(self):
+        lib().or_sim_compact(self.h)
 
     def shape(self):
         s = self._Shape()

@@ -676,6 +676,8 @@ void offspring_batch(or_sim* s, const Selector& sel, const std::vector<uint32_t>
 
 }  // namespace
 
+extern "C" int or_sim_compact(or_sim* s);
+
 // Engine<parallel_pipeline>::simulate (examples/QTLSim/engine_parallel_pipeline.hpp:172-239).
 extern "C" int or_sim_step(or_sim* s) {
     const clotho_config& cfg = s->cfg;
@@ -829,10 +831,61 @@ extern "C" int or_sim_step(or_sim* s) {
     }
     s->A = all_size;
     ++s->gen;
+    if (cfg.compact_every) or_sim_compact(s);
     auto c4 = std::chrono::steady_clock::now();
     s->phase[4] = std::chrono::duration<double>(c3 - c2).count();
     s->phase[5] = std::chrono::duration<double>(c4 - c3).count();
     s->phase[6] = std::chrono::duration<double>(c1 - c0).count();
+    return 0;
+}
+
+// Column compaction (no reference counterpart; the reference reuses free
+// slots, QS/engine_parallel_pipeline.hpp:337-348): every slot on the fixed or
+// lost list of the current population is removed, the remaining (variable)
+// slots keep their order and are renumbered 0..V-1, rows are re-packed to
+// W = ceil(V/512)*8 words, positions, effects and ages follow their slots,
+// and the free lists become empty.  Genome traits and fitness are unchanged.
+extern "C" int or_sim_compact(or_sim* s) {
+    const uint32_t A = s->A, W = s->wc, H = s->H, T = s->T;
+    std::vector<uint8_t> drop(A, 0);
+    for (uint32_t j : s->fixed_list) drop[j] = 1;
+    for (uint32_t j : s->lost_list) drop[j] = 1;
+    std::vector<uint32_t> kept;
+    kept.reserve(A);
+    for (uint32_t j = 0; j < A; ++j)
+        if (!drop[j]) kept.push_back(j);
+    const uint32_t V = (uint32_t)kept.size();
+    const uint32_t W2 = ((V + 511) / 512) * 8;
+    std::vector<uint64_t> rows((size_t)H * W2, 0);
+    for (uint32_t r = 0; r < H; ++r) {
+        const uint64_t* in = s->child.data() + (size_t)r * W;
+        uint64_t* out = rows.data() + (size_t)r * W2;
+        for (uint32_t k = 0; k < V; ++k) {
+            const uint32_t j = kept[k];
+            if ((in[j / 64] >> (j % 64)) & 1) out[k / 64] |= (uint64_t)1 << (k % 64);
+        }
+    }
+    std::vector<double> pos(V), eff((size_t)V * T);
+    std::vector<uint32_t> age(V);
+    for (uint32_t k = 0; k < V; ++k) {
+        pos[k] = s->pos[kept[k]];
+        age[k] = s->age[kept[k]];
+        for (uint32_t t = 0; t < T; ++t) eff[(size_t)k * T + t] = s->eff[(size_t)kept[k] * T + t];
+    }
+    s->child.swap(rows);
+    s->wc = W2;
+    s->A = V;
+    s->pos.swap(pos);
+    s->eff.swap(eff);
+    s->age.swap(age);
+    s->fixed_list.clear();
+    s->lost_list.clear();
+    s->all_neutral = true;
+    for (double w : s->eff)
+        if (w != 0.0) {
+            s->all_neutral = false;
+            break;
+        }
     return 0;
 }
 

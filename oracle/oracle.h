@@ -65,6 +65,9 @@ int or_sim_set_population(or_sim* s, const uint64_t* rows, uint32_t words,
                           const double* fitness);
 void or_sim_set_threads(or_sim* s, int threads);
 int or_sim_step(or_sim* s);
+/* Remove every fixed or lost column of the current population (slots keep
+ * their order and are renumbered; the free lists become empty). */
+int or_sim_compact(or_sim* s);
 void or_sim_shape(const or_sim* s, clotho_shape* out);
 void or_sim_rows(const or_sim* s, uint64_t* out);
 void or_sim_alleles(const or_sim* s, double* positions, double* effects, uint32_t* ages);

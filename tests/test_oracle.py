@@ -354,3 +354,74 @@ def test_config0_dynamics_plausible():
     assert 1500 <= s.new_mutations <= 2500
     assert 8000 < s.variable_count < 20000
     assert s.words == ((s.alleles + 511) // 512) * 8
+
+
+# ---------------------------------------------------------------- compaction
+def _genome_alleles(sim, exclude_free=True):
+    """Per haplotype: sorted tuples (position, effects..., age) of its set
+    bits, skipping the fixed/lost slots when exclude_free."""
+    rows = sim.rows()
+    pos, eff, age = sim.alleles()
+    free = set()
+    if exclude_free:
+        fx, ls = sim.free_space()
+        free = set(int(j) for j in fx) | set(int(j) for j in ls)
+    bits = np.unpackbits(rows.view(np.uint8), axis=1, bitorder="little")[:, : pos.size]
+    out = []
+    for r in range(rows.shape[0]):
+        slots = [j for j in np.flatnonzero(bits[r]) if j not in free]
+        out.append(sorted((pos[j], *eff[j].tolist(), int(age[j])) for j in slots))
+    return out
+
+
+def test_compaction_preserves_population_as_allele_multisets():
+    """Every draw is keyed by gamete or mutation index, never by slot, so a run
+    that compacts after every generation carries, per genome, the same alleles
+    (position, effect, age) as the uncompacted run minus its fixed/lost
+    columns, and identical traits, fitness and gametes."""
+    base = dict(seed=21, traits=2, neutral_p=0.3, sigma=0.3, fitness=api.FIT_MV_STABILIZING,
+                fitness_scale=2.0, env_sigma=0.5, max_alleles=1 << 14)
+    a = oracle.OracleSim(api.make_config(120, 3.0, 2.0, **base))
+    b = oracle.OracleSim(api.make_config(120, 3.0, 2.0, compact=True, **base))
+    for g in range(12):
+        a.step()
+        b.step()
+        sa, sb = a.shape(), b.shape()
+        assert sb.free_count == 0 and sb.alleles == sa.variable_count
+        assert sb.words == ((sb.alleles + 511) // 512) * 8
+        assert np.array_equal(a.traits(), b.traits())
+        assert np.array_equal(a.fitness(), b.fitness())
+        for x, y in zip(a.gametes(), b.gametes()):
+            assert np.array_equal(x, y)
+        assert _genome_alleles(a) == _genome_alleles(b, exclude_free=False), f"generation {g}"
+
+
+def test_compaction_hand_built_case():
+    """Five slots, slot 1 fixed (all rows) and slot 3 lost (no row): the kept
+    slots 0, 2, 4 become 0, 1, 2 with their positions, effects and bits."""
+    cfg = api.make_config(2, 0.0, 0.0, seed=1, traits=1, neutral_p=0.0, max_alleles=512)
+    o = oracle.OracleSim(cfg)
+    # rows (H = 4), bits over slots 0..4
+    bits = np.array([[1, 1, 0, 0, 1],
+                     [0, 1, 1, 0, 0],
+                     [1, 1, 1, 0, 0],
+                     [0, 1, 0, 0, 1]], np.uint8)
+    rows = np.zeros((4, 8), np.uint64)
+    for r in range(4):
+        for j in range(5):
+            if bits[r, j]:
+                rows[r, 0] |= np.uint64(1) << np.uint64(j)
+    pos = np.array([0.5, 0.25, 0.75, 0.125, 0.0625])
+    eff = np.array([[1.0], [2.0], [3.0], [4.0], [5.0]])
+    o.set_population(rows, pos, eff)
+    fx, ls = o.free_space()
+    assert fx.tolist() == [1] and ls.tolist() == [3]
+    traits_before = o.traits().copy()
+    o.compact()
+    s = o.shape()
+    assert s.alleles == 3 and s.words == 8 and s.free_count == 0
+    p2, e2, _ = o.alleles()
+    assert p2.tolist() == [0.5, 0.75, 0.0625] and e2[:, 0].tolist() == [1.0, 3.0, 5.0]
+    want = [0b101, 0b010, 0b011, 0b100]
+    assert o.rows()[:, 0].tolist() == want and not o.rows()[:, 1:].any()
+    assert np.array_equal(o.traits(), traits_before)
