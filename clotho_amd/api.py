@@ -116,7 +116,7 @@ EXPORTS = [
     "clotho_autotune",
     "clotho_batch_create", "clotho_batch_destroy", "clotho_batch_last_error",
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
-    "clotho_batch_set_timing", "clotho_compact",
+    "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -165,6 +165,7 @@ def lib():
     L.clotho_trait_contract_synthetic.argtypes = [C.c_uint32, C.c_uint32, vp, C.c_uint64, vp,
                                                   C.c_uint32, C.c_int, C.c_int, C.c_int, vp,
                                                   C.POINTER(C.c_double), C.c_int]
+    L.clotho_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.clotho_batch_create.argtypes = [vp, C.c_uint32, C.c_int, C.POINTER(vp)]
     L.clotho_batch_destroy.argtypes = [vp]
     L.clotho_batch_destroy.restype = None
@@ -300,6 +301,11 @@ class Simulation:
                                                   ev.ctypes.data, mu.ctypes.data, h),
                     "download_gametes")
         return top, bot, ev, mu
+
+    def device_bytes(self):
+        out = C.c_uint64(0)
+        self._check(lib().clotho_device_bytes(self.ctx, C.byref(out)), "device_bytes")
+        return out.value
 
     def compact(self):
         self._check(lib().clotho_compact(self.ctx), "compact")

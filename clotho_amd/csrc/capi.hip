@@ -55,6 +55,7 @@ int dalloc(clotho_ctx* c, T** p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
     HIP_TRY(c, hipMalloc((void**)p, count * sizeof(T)));
+    c->device_bytes += count * sizeof(T);
     return CLOTHO_OK;
 }
 
@@ -613,6 +614,12 @@ int clotho_phase_commit(clotho_ctx* c) {
     c->cur = 1 - c->cur;
     return CLOTHO_OK;
 }
+int clotho_device_bytes(const clotho_ctx* c, uint64_t* bytes) {
+    if (!c || !bytes) return CLOTHO_ERR_ARG;
+    *bytes = c->device_bytes;
+    return CLOTHO_OK;
+}
+
 int clotho_compact(clotho_ctx* c) {
     if (!c) return CLOTHO_ERR_ARG;
     HIP_TRY(c, hipSetDevice(c->device));

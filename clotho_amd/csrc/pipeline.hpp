@@ -158,6 +158,7 @@ struct clotho_ctx {
     clotho::GraphPair graphs;
     std::string err;
     hipDeviceProp_t prop;
+    size_t device_bytes = 0;    // sum of the context's device allocations
 };
 
 namespace clotho {

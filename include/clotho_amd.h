@@ -161,6 +161,11 @@ int clotho_download_fitness(clotho_ctx* ctx, double* fitness, size_t cap);
 int clotho_download_gametes(clotho_ctx* ctx, uint32_t* top_row, uint32_t* bottom_row,
                             uint32_t* events, uint32_t* mutations, size_t cap);
 
+/* Device memory held by the context (both populations, allele table,
+ * free-space and selection buffers, scratch), in bytes.  No reference
+ * counterpart (the reference's footprint is its std::vector/new[] storage). */
+int clotho_device_bytes(const clotho_ctx* ctx, uint64_t* bytes);
+
 /* ---- one generation ---------------------------------------------------- */
 /* Enqueue n generations on the context stream and wait; returns the sticky
  * device error as CLOTHO_ERR_CAPACITY if one occurred.  use_graph != 0
