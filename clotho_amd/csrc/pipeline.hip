@@ -488,13 +488,17 @@ struct WordSort {
 // read back with v_readlane, so the row loop issues no scalar loads.  Even and
 // odd rows use separate register sets: the loads of row q + 2 are issued as
 // soon as row q is done and stay in flight while row q + 1 is classified (no
-// register rotation, hence no wait on loads still in flight).  Lane e holds
-// event key e.  Events are classified four at a time so that their LDS
-// lookups overlap; a group is padded with rank-64 entries (pm[64] is all
-// ones), and since #real + #padding = 4, pm[c0] ^ pm[c1] ^ pm[c2] ^ pm[c3]
-// equals the XOR of the complements of the real events.
+// register rotation, hence no wait on loads still in flight; three or four
+// sets measured no faster on the same device).  Lane e holds
+// event key e.  Events are classified kEventGroup at a time so that their
+// LDS lookups overlap; a group is padded with rank-64 entries (pm[64] is all
+// ones), and since #real + #padding = kEventGroup is even, the XOR of the
+// group's prefix masks equals the XOR of the complements of the real events.
 constexpr int kRecWaves = 16;
 constexpr int kRecBlock = 64 * kRecWaves;
+// events classified together; any even size keeps the padding parity
+// argument (#real + #padding even), and 2 pads Poisson(5) counts least
+constexpr int kEventGroup = 2;
 // dword 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
 // data format, offsets range-checked against num_records in bytes
 constexpr int kBufferWord3 = 0x00020000;
@@ -590,43 +594,46 @@ struct Recombine {
             auto process = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
                 const uint32_t ne = lane_read(g_ne, q);
                 uint64_t m = 0;
-                for (uint32_t e = 0; e < ne; e += 4) {
-                    uint32_t key[4], p[4];
-                    bool more[4];
-    #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
+                for (uint32_t e = 0; e < ne; e += kEventGroup) {
+                    uint32_t key[kEventGroup], p[kEventGroup];
+                    bool more[kEventGroup];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) {
                         const uint32_t x = e + i;
                         const uint32_t a = lane_read(lo, x & 63);
                         key[i] = (kWideEvents && x >= 64) ? lane_read(hi, x & 63) : a;
                     }
-                    uint32_t tb[4];
-    #pragma unroll
-                    for (int i = 0; i < 4; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
-    #pragma unroll
-                    for (int i = 0; i < 4; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
-                    uint32_t sk[4][4];
-    #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-    #pragma unroll
+                    uint32_t tb[kEventGroup];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                    uint32_t sk[kEventGroup][4];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i)
+#pragma unroll
                         for (int j = 0; j < 4; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
-    #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) {
                         const bool live = e + i < ne;
-                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) +
-                                           (sk[i][1] < key[i] ? 1u : 0u) +
+                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u) +
                                            (sk[i][2] < key[i] ? 1u : 0u);
                         more[i] = live && sk[i][3] < key[i];
                         p[i] = live ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
                     }
-                    if (more[0] | more[1] | more[2] | more[3]) {
-                        // four or more keys of a bucket below an event: rare
-    #pragma unroll
-                        for (int i = 0; i < 4; ++i)
+                    bool any_more = false;
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) any_more = any_more || more[i];
+                    if (any_more) {  // four or more keys of a bucket below an event: rare
+#pragma unroll
+                        for (int i = 0; i < kEventGroup; ++i)
                             if (more[i])
                                 while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
                     }
-                    m ^= (s_pm[p[0] * 64 + lane] ^ s_pm[p[1] * 64 + lane]) ^
-                         (s_pm[p[2] * 64 + lane] ^ s_pm[p[3] * 64 + lane]);
+                    uint64_t xm = 0;
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
+                    m ^= xm;
                 }
                 t &= keep;
                 b &= keep;

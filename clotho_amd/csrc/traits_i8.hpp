@@ -133,17 +133,28 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
             }
         }
     };
+    // B fragments of group G (16 KB) go straight from global memory into LDS
+    // buffer G & 1 (global_load_lds: no registers, completion counted by
+    // vmcnt); each wave moves four 1 KB pieces.
     auto stage_b = [&](uint32_t G) {
         if (kLdsB && G < NG) {
-            u32x4* dst = s_b + (G & 1) * 16 * 64;
-            const g_u32x4* src = (const g_u32x4*)(packed + (size_t)G * 16 * 64);
-            for (uint32_t i = threadIdx.x; i < 16 * 64; i += 256) dst[i] = src[i];
+            for (uint32_t i = wave; i < 16; i += 4) {
+                const g_u32x4* src = (const g_u32x4*)(packed + ((size_t)G * 16 + i) * 64) + lane;
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)src,
+                    (__attribute__((address_space(3))) void*)(s_b + (G & 1) * 16 * 64 + i * 64),
+                    16, 0, 0);
+            }
         }
     };
+    // vmcnt(0) only (expcnt and lgkmcnt at their maxima): the direct-to-LDS
+    // loads and the next group's rows have landed
+    auto wait_loads = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
     const g_u32x4* packed4 = (const g_u32x4*)packed;
     u32x4 x[RT][2];
     if (NG) load_rows(0, x);
     stage_b(0);
+    wait_loads();
     __syncthreads();
     for (uint32_t G = 0; G < NG; ++G) {
         u32x4 xn[RT][2];
@@ -176,6 +187,7 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
                         __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
             }
         }
+        wait_loads();
         __syncthreads();  // staged B of group G + 1 complete; buffer G & 1 free
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
