@@ -168,6 +168,11 @@ def main():
               enumerate(["generation", "draw", "recombine", "traits", "free_space", "fitness"])}
     log(f"[bench] per-generation ms (eager): {phases}")
 
+    # per-period population statistics (column and row margins, all rows)
+    _, _, stats_ms = sim.allele_counts(timed=True)
+    _, _, stats_ms = sim.allele_counts(timed=True)
+    log(f"[bench] population statistics (all rows): {stats_ms:.3f} ms")
+
     ms_per_step = elapsed / args.steps * 1e3
     value = h * args.steps / elapsed
     out = {
@@ -197,6 +202,8 @@ def main():
         "hbm_copy_gbps_measured": copy_gbps,
         "phase_ms": phases,
         "diploid_per_s": value / 2,
+        "population_stats": {"ms": stats_ms,
+                             "gbps": 8.0 * words * h / (stats_ms * 1e-3) / 1e9 if stats_ms else None},
     }
     print(json.dumps(out), flush=True)
 

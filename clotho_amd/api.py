@@ -117,6 +117,7 @@ EXPORTS = [
     "clotho_batch_create", "clotho_batch_destroy", "clotho_batch_last_error",
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
     "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
+    "clotho_sample_rows", "clotho_allele_counts",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -166,6 +167,10 @@ def lib():
                                                   C.c_uint32, C.c_int, C.c_int, C.c_int, vp,
                                                   C.POINTER(C.c_double), C.c_int]
     L.clotho_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.clotho_sample_rows.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     u32p]
+    L.clotho_allele_counts.argtypes = [vp, vp, C.c_uint32, u32p, C.c_size_t, u32p,
+                                       C.POINTER(C.c_double)]
     L.clotho_batch_create.argtypes = [vp, C.c_uint32, C.c_int, C.POINTER(vp)]
     L.clotho_batch_destroy.argtypes = [vp]
     L.clotho_batch_destroy.restype = None
@@ -307,6 +312,23 @@ class Simulation:
         self._check(lib().clotho_device_bytes(self.ctx, C.byref(out)), "device_bytes")
         return out.value
 
+    def allele_counts(self, rows=None, timed=False):
+        """(column_counts[alleles], row_counts) of the current population over
+        `rows` (all haplotypes when None)."""
+        s = self.shape()
+        n = s.haplotypes if rows is None else len(rows)
+        cols = np.zeros(max(s.alleles, 1), np.uint32)
+        rc = np.zeros(max(n, 1), np.uint32)
+        ms = C.c_double(0.0)
+        rp = None
+        if rows is not None:
+            rows = np.ascontiguousarray(rows, np.uint32)
+            rp = rows.ctypes.data
+        self._check(lib().clotho_allele_counts(self.ctx, rp, n, cols, cols.size, rc,
+                                               C.byref(ms)), "allele_counts")
+        out = (cols[: s.alleles], rc[:n])
+        return (out + (ms.value,)) if timed else out
+
     def compact(self):
         self._check(lib().clotho_compact(self.ctx), "compact")
 
@@ -323,6 +345,14 @@ class Simulation:
         out = np.zeros(8, np.float64)
         n = lib().clotho_timings(self.ctx, out, out.size)
         return n, out
+
+
+def sample_rows(seed, generation, replicate, population, k):
+    out = np.zeros(max(min(k, population), 1), np.uint32)
+    rc = lib().clotho_sample_rows(seed, generation, replicate, population, k, out)
+    if rc != 0:
+        raise ClothoError(f"sample_rows failed ({ERRORS.get(rc, rc)})")
+    return out[: min(k, population)]
 
 
 def poisson_table(lam, cap=128):

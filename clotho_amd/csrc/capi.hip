@@ -614,6 +614,82 @@ int clotho_phase_commit(clotho_ctx* c) {
     c->cur = 1 - c->cur;
     return CLOTHO_OK;
 }
+int clotho_sample_rows(uint64_t seed, uint32_t generation, uint32_t replicate, uint32_t population,
+                       uint32_t k, uint32_t* out) {
+    if (!out || population == 0) return CLOTHO_ERR_ARG;
+    if (k >= population) {  // every row, in order
+        for (uint32_t i = 0; i < population; ++i) out[i] = i;
+        return CLOTHO_OK;
+    }
+    std::vector<bool> taken(population, false);
+    uint32_t got = 0;
+    for (uint64_t d = 0; got < k; ++d) {
+        const Draw4 r = philox_draw(seed, generation, kStreamSample, replicate,
+                                    (uint32_t)(d / 2), (uint32_t)(d >> 33));
+        const uint64_t u = (d & 1) ? k53(r.x[2], r.x[3]) : k53(r.x[0], r.x[1]);
+        const uint32_t t = (uint32_t)(((unsigned __int128)u * population) >> 53);
+        if (taken[t]) continue;  // rejection, as random_sample_generator
+        taken[t] = true;
+        out[got++] = t;
+    }
+    return CLOTHO_OK;
+}
+
+int clotho_allele_counts(clotho_ctx* c, const uint32_t* rows, uint32_t n_rows,
+                         uint32_t* column_counts, size_t cap_columns, uint32_t* row_counts,
+                         double* kernel_ms) {
+    if (!c || !column_counts || !row_counts) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t W = st.wc, A = st.all_size, H = c->p.h;
+    const uint32_t n = rows ? n_rows : H;
+    if (cap_columns < A) return fail(c, CLOTHO_ERR_ARG, "allele_counts: column buffer too small");
+    if (rows)
+        for (uint32_t i = 0; i < n; ++i)
+            if (rows[i] >= H) return fail(c, CLOTHO_ERR_ARG, "allele_counts: row out of range");
+    const uint32_t ngroups = (W + 15) / 16, rg = (n + 63) / 64;
+    const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
+                                                      rg, 2048 / std::max<uint32_t>(1, ngroups)));
+    uint32_t *d_sample = nullptr, *d_cols = nullptr, *d_rows = nullptr, *d_cp = nullptr,
+             *d_rp = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        for (uint32_t* p : {d_sample, d_cols, d_rows, d_cp, d_rp})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    if (rows) step(hipMalloc((void**)&d_sample, 4ull * std::max<uint32_t>(n, 1)));
+    step(hipMalloc((void**)&d_cols, 4ull * std::max<uint32_t>(A, 1)));
+    step(hipMalloc((void**)&d_rows, 4ull * std::max<uint32_t>(n, 1)));
+    step(hipMalloc((void**)&d_cp, 4ull * std::max<size_t>((size_t)chunks * W * 64, 1)));
+    step(hipMalloc((void**)&d_rp, 4ull * std::max<size_t>((size_t)ngroups * n, 1)));
+    step(hipEventCreate(&e0));
+    step(hipEventCreate(&e1));
+    if (rows && e == hipSuccess)
+        step(hipMemcpyAsync(d_sample, rows, 4ull * n, hipMemcpyHostToDevice, c->stream));
+    step(hipEventRecord(e0, c->stream));
+    if (e == hipSuccess)
+        step(launch_allele_counts(c, c->b.rows[c->cur], W, A, d_sample, n, d_cols, d_rows, d_cp,
+                                  d_rp, chunks));
+    step(hipEventRecord(e1, c->stream));
+    if (A) step(hipMemcpyAsync(column_counts, d_cols, 4ull * A, hipMemcpyDeviceToHost, c->stream));
+    if (n) step(hipMemcpyAsync(row_counts, d_rows, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+    step(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (e == hipSuccess) step(hipEventElapsedTime(&ms, e0, e1));
+    cleanup();
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    if (kernel_ms) *kernel_ms = ms;
+    return CLOTHO_OK;
+}
+
 int clotho_device_bytes(const clotho_ctx* c, uint64_t* bytes) {
     if (!c || !bytes) return CLOTHO_ERR_ARG;
     *bytes = c->device_bytes;

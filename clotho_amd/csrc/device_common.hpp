@@ -16,6 +16,8 @@
 namespace clotho {
 
 enum : uint32_t { S_PARENTS = 0, S_GAMETE = 1, S_MUTCOUNT = 2, S_MUTATTR = 3, S_ENV = 4 };
+// stream 7 draws synthetic genotypes; stream 8 draws statistics row samples
+constexpr uint32_t kStreamSample = 8;
 
 struct Draw4 {
     uint32_t x[4];

@@ -1057,6 +1057,92 @@ __global__ void k_compact_state(Buffers B) {
 }
 
 // ---------------------------------------------------------------------------
+// Population statistics: column and row margins over a row sample
+// (frequency_evaluator<population_space<row_block_alignment>>::operator(),
+// include/clotho/data_spaces/analysis/allele_frequency/frequency_evaluator_population_space_row_alignment.hpp:36-61).
+// Block = 16 waves over 16 consecutive words (one 128-byte line per row) of
+// groups of 64 sampled rows, lane = row.  Row margins: per-word popcounts
+// summed across the 16 waves in LDS.  Column margins: the wave's 64 x 64 bit
+// block (lane = row, bit = column) is transposed across lanes in six
+// butterfly stages, after which lane = column and a popcount counts that
+// column's carriers among the 64 rows.  Partials per (row chunk, column) and
+// (word group, row) are summed by two small kernels (no atomics).
+// ---------------------------------------------------------------------------
+constexpr int kStatWaves = 16;
+
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, uint32_t lane) {
+    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                               0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const uint32_t sft = 32u >> i;
+        const uint64_t y = __shfl_xor(x, (int)sft, 64);
+        const uint64_t M = masks[i];
+        // rows with bit sft clear keep their low-half columns and take the
+        // partner's low-half columns into the high half; the partner the reverse
+        x = (lane & sft) ? ((x & ~M) | ((y & ~M) >> sft)) : ((x & M) | ((y & M) << sft));
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(64 * kStatWaves) k_allele_counts(
+    const uint64_t* __restrict__ rows_in, uint32_t W, const uint32_t* __restrict__ sample_in,
+    uint32_t n, uint32_t groups_per_chunk, uint32_t* __restrict__ col_part_in,
+    uint32_t* __restrict__ row_part_in) {
+    __shared__ uint32_t s_rc[kStatWaves][64];
+    const auto rows = gbl(rows_in);
+    const auto col_part = gbl(col_part_in);
+    const auto row_part = gbl(row_part_in);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t ngroups = (W + kStatWaves - 1) / kStatWaves;
+    const uint32_t wg = blockIdx.x % ngroups, chunk = blockIdx.x / ngroups;
+    const uint32_t w = wg * kStatWaves + wave;
+    const uint32_t rg_total = (n + 63) / 64;
+    const uint32_t g0 = chunk * groups_per_chunk, g1 = min(rg_total, g0 + groups_per_chunk);
+    uint32_t col = 0;
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t k = g * 64 + lane;
+        uint64_t x = 0;
+        if (k < n && w < W) {
+            const uint32_t r = sample_in ? gbl(sample_in)[k] : k;
+            x = rows[(size_t)r * W + w];
+        }
+        s_rc[wave][lane] = (uint32_t)__popcll(x);
+        col += (uint32_t)__popcll(transpose64(x, lane));
+        __syncthreads();
+        if (wave == 0 && k < n) {
+            uint32_t rc = 0;
+            for (int q = 0; q < kStatWaves; ++q) rc += s_rc[q][lane];
+            row_part[(size_t)wg * n + k] = rc;
+        }
+        __syncthreads();
+    }
+    if (w < W) col_part[(size_t)chunk * W * 64 + (size_t)w * 64 + lane] = col;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum_columns(const uint32_t* __restrict__ part,
+                                                        uint32_t chunks, uint32_t W,
+                                                        uint32_t A, uint32_t* __restrict__ out) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= A) return;
+    uint32_t s = 0;
+    for (uint32_t c = 0; c < chunks; ++c) s += gbl(part)[(size_t)c * W * 64 + j];
+    gbl(out)[j] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum_rows(const uint32_t* __restrict__ part,
+                                                     uint32_t ngroups, uint32_t n,
+                                                     uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    uint32_t s = 0;
+    for (uint32_t g = 0; g < ngroups; ++g) s += gbl(part)[(size_t)g * n + k];
+    gbl(out)[k] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Fitness: GeneralFitness over individuals, z = genome(2i) + genome(2i+1)
 // ---------------------------------------------------------------------------
 __device__ inline double fitness_of(const Params& P, const double* z) {
@@ -1445,6 +1531,31 @@ hipError_t launch_compact(clotho_ctx* c, int src) {
     TRY_LAUNCH(hipGetLastError());
     hipLaunchKernelGGL(k_compact_state, dim3(1), dim3(1), 0, c->stream, c->b);
     return hipGetLastError();
+}
+
+hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W, uint32_t A,
+                                const uint32_t* d_sample, uint32_t n, uint32_t* d_cols,
+                                uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,
+                                uint32_t chunks) {
+    const uint32_t ngroups = (W + kStatWaves - 1) / kStatWaves;
+    const uint32_t rg = (n + 63) / 64;
+    const uint32_t per = (rg + chunks - 1) / chunks;
+    if (W && n) {
+        hipLaunchKernelGGL(k_allele_counts, dim3(ngroups * chunks), dim3(64 * kStatWaves), 0,
+                           c->stream, rows, W, d_sample, n, per, d_col_part, d_row_part);
+        TRY_LAUNCH(hipGetLastError());
+    }
+    if (A) {
+        hipLaunchKernelGGL(k_sum_columns, dim3((A + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, d_col_part, W && n ? chunks : 0u, W, A, d_cols);
+        TRY_LAUNCH(hipGetLastError());
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_sum_rows, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           c->stream, d_row_part, W ? ngroups : 0u, n, d_rows);
+        TRY_LAUNCH(hipGetLastError());
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {

@@ -173,6 +173,12 @@ hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
 // Column compaction of population [src] into [1 - src] (fixed and lost
 // columns removed, slots renumbered in order); the caller flips cur.
 hipError_t launch_compact(clotho_ctx* c, int src);
+// Column and row margins of a row sample (d_sample == nullptr: rows 0..n-1);
+// partial buffers: d_col_part [chunks][W*64], d_row_part [ceil(W/16)][n].
+hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W, uint32_t A,
+                                const uint32_t* d_sample, uint32_t n, uint32_t* d_cols,
+                                uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,
+                                uint32_t chunks);
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
 hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
                              uint32_t words);

@@ -4,7 +4,7 @@
 // simulate() per generation, report free/variable/fixed counts and the
 // per-phase performance map.  Prints one JSON object.
 //
-//   engine_demo [--registry] [key=value ...] [--generations G] [--graph]
+//   engine_demo [--registry] [key=value ...] [--generations G] [--graph] [--stats K]
 //
 // --registry lists the fitness_toolkit generators and the parsed
 // configuration without touching a GPU.
@@ -32,12 +32,15 @@ int main(int argc, char** argv) {
     config.put("alleles.capacity", 32768.0);
     unsigned generations = 10;
     bool graph = false, registry = false;
+    long stats_sample = -1;  // --stats K: allele-frequency statistics over K rows (0 = all)
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--registry") {
             registry = true;
         } else if (a == "--graph") {
             graph = true;
+        } else if (a == "--stats" && i + 1 < argc) {
+            stats_sample = std::stol(argv[++i]);
         } else if (a == "--generations" && i + 1 < argc) {
             generations = (unsigned)std::stoul(argv[++i]);
         } else if (a.find('=') != std::string::npos) {
@@ -75,10 +78,31 @@ int main(int argc, char** argv) {
         }
     }
     auto* fs = engine.getFreeSpace();
+    std::string stats = "null";
+    if (stats_sample >= 0) {
+        clotho_amd::allele_frequency af;
+        if (stats_sample == 0)
+            af.evaluate(*engine.getChildPopulation());
+        else
+            af.evaluate(*engine.getChildPopulation(), (uint64_t)config.get("random_number.seed", 0.0),
+                        s.generation, (uint32_t)stats_sample);
+        const auto fd = af.frequency_distribution();
+        uint64_t fh = 1469598103934665603ull;  // FNV-1a over the spectrum
+        for (uint32_t v : fd)
+            for (int b = 0; b < 4; ++b) {
+                fh ^= (v >> (8 * b)) & 0xFF;
+                fh *= 1099511628211ull;
+            }
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), "\"%016llx\"", (unsigned long long)fh);
+        stats = "{\"stats\": " + af.stats_json() + ", \"frequency_distribution_fnv1a\": " + buf +
+                "}";
+    }
     std::printf("{\"generation\": %u, \"words\": %u, \"alleles\": %u, \"free\": %u, "
-                "\"fixed\": %u, \"variable\": %u, \"rows_fnv1a\": \"%016llx\"}\n",
+                "\"fixed\": %u, \"variable\": %u, \"rows_fnv1a\": \"%016llx\", "
+                "\"allele_frequency\": %s}\n",
                 s.generation, s.words, s.alleles, (unsigned)fs->free_size(),
                 (unsigned)fs->fixed_size(), (unsigned)fs->variable_count(),
-                (unsigned long long)h);
+                (unsigned long long)h, stats.c_str());
     return 0;
 }

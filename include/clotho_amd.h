@@ -161,6 +161,27 @@ int clotho_download_fitness(clotho_ctx* ctx, double* fitness, size_t cap);
 int clotho_download_gametes(clotho_ctx* ctx, uint32_t* top_row, uint32_t* bottom_row,
                             uint32_t* events, uint32_t* mutations, size_t cap);
 
+/* ---- population statistics -------------------------------------------- */
+/* Row sample for the per-period statistics (random_sample_generator,
+ * include/clotho/data_spaces/generators/random_sample_generator.hpp:133-160):
+ * k >= population gives every row 0..population-1 in order; otherwise k
+ * distinct rows in draw order, each draw uniform on [0, population) from
+ * Philox stream 8 (seed, generation, replicate) and rejected when already
+ * taken.  Host-only; out holds min(k, population) entries. */
+int clotho_sample_rows(uint64_t seed, uint32_t generation, uint32_t replicate,
+                       uint32_t population, uint32_t k, uint32_t* out);
+/* Column and row margins of the current population over a row subset
+ * (frequency_evaluator<population_space<row_block_alignment>>::operator(),
+ * include/clotho/data_spaces/analysis/allele_frequency/frequency_evaluator_population_space_row_alignment.hpp:36-61):
+ * column_counts[j] = number of rows of the subset carrying slot j, for every
+ * slot j < alleles (cap_columns >= alleles); row_counts[i] = alleles carried
+ * by rows[i].  rows == NULL takes every haplotype (n_rows ignored, row_counts
+ * holds H entries).  kernel_ms (may be NULL) receives the device time of the
+ * counting kernels.  Synchronous. */
+int clotho_allele_counts(clotho_ctx* ctx, const uint32_t* rows, uint32_t n_rows,
+                         uint32_t* column_counts, size_t cap_columns, uint32_t* row_counts,
+                         double* kernel_ms);
+
 /* Device memory held by the context (both populations, allele table,
  * free-space and selection buffers, scratch), in bytes.  No reference
  * counterpart (the reference's footprint is its std::vector/new[] storage). */

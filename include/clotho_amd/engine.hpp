@@ -26,8 +26,10 @@
 #ifndef CLOTHO_AMD_ENGINE_HPP_
 #define CLOTHO_AMD_ENGINE_HPP_
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <memory>
@@ -427,6 +429,99 @@ private:
 
 // ---------------------------------------------------------------------------
 // Engine<RNG, Real, Block, Size, mi355x_pipeline>
+// ---------------------------------------------------------------------------
+// Per-period population statistics: allele_frequency + margin_details
+// (include/clotho/data_spaces/analysis/allele_frequency/population_space_row_allele_frequency.hpp:31-60,
+//  include/clotho/data_spaces/analysis/allele_frequency/margin_details.hpp:50-104)
+// over a row sample drawn like random_sample_generator
+// (include/clotho/data_spaces/generators/random_sample_generator.hpp:133-160).
+// The margins come from one device pass (clotho_allele_counts); the summaries
+// are computed here.  Differences from the reference: the median is exact
+// (the reference reports Boost's P-square estimate), the variance is the
+// population variance (as Boost's), and frequency_distribution has n + 1
+// entries (the reference sizes it n, so a column carried by every sampled row
+// would index past its end).
+class allele_frequency {
+public:
+    // sample_size >= H: every haplotype; otherwise a sample drawn from Philox
+    // stream 8 keyed by (seed, generation, replicate)
+    void evaluate(device_population& pop, uint64_t seed, uint32_t generation,
+                  uint32_t sample_size) {
+        const clotho_shape s = pop.shape();
+        std::vector<uint32_t> sample(std::min(sample_size, s.haplotypes));
+        check(clotho_sample_rows(seed, generation, pop.config().replicate, s.haplotypes,
+                                 sample_size, sample.data()),
+              pop.ctx(), "sample_rows");
+        run(pop, s, sample.data(), (uint32_t)sample.size());
+    }
+    void evaluate(device_population& pop) {
+        const clotho_shape s = pop.shape();
+        run(pop, s, nullptr, s.haplotypes);
+    }
+
+    const std::vector<uint32_t>& column_margin() const { return m_cols; }
+    const std::vector<uint32_t>& row_margin() const { return m_rows; }
+    double kernel_ms() const { return m_ms; }
+
+    // frequency_distribution[v] = number of slots carried by v sampled rows
+    std::vector<uint32_t> frequency_distribution() const {
+        std::vector<uint32_t> f(m_n + 1, 0);
+        for (uint32_t v : m_cols) ++f[v];
+        return f;
+    }
+
+    struct summary {
+        double min = 0, max = 0, mean = 0, median = 0, variance = 0;
+        size_t total = 0;
+    };
+    static summary summarize(std::vector<uint32_t> v) {
+        summary r;
+        r.total = v.size();
+        if (v.empty()) return r;
+        std::sort(v.begin(), v.end());
+        double sum = 0, sq = 0;
+        for (uint32_t x : v) {
+            sum += x;
+            sq += (double)x * x;
+        }
+        r.min = v.front();
+        r.max = v.back();
+        r.mean = sum / v.size();
+        r.variance = sq / v.size() - r.mean * r.mean;
+        const size_t h = v.size() / 2;
+        r.median = (v.size() % 2) ? v[h] : 0.5 * ((double)v[h - 1] + v[h]);
+        return r;
+    }
+
+    // the reference's stats.* keys (recordResults) as JSON; the full
+    // distribution arrays are available through the accessors above
+    std::string stats_json() const {
+        const summary c = summarize(m_cols), r = summarize(m_rows);
+        char buf[1024];
+        std::snprintf(buf, sizeof(buf),
+                      "{\"sequences_per_allele\": {\"min\": %.17g, \"max\": %.17g, "
+                      "\"mean\": %.17g, \"median\": %.17g, \"variance\": %.17g, \"total\": %zu}, "
+                      "\"alleles_per_sequence\": {\"min\": %.17g, \"max\": %.17g, "
+                      "\"mean\": %.17g, \"median\": %.17g, \"variance\": %.17g, \"total\": %zu}}",
+                      c.min, c.max, c.mean, c.median, c.variance, c.total, r.min, r.max, r.mean,
+                      r.median, r.variance, r.total);
+        return buf;
+    }
+
+private:
+    void run(device_population& pop, const clotho_shape& s, const uint32_t* rows, uint32_t n) {
+        m_n = n;
+        m_cols.assign(s.alleles, 0);
+        m_rows.assign(n, 0);
+        check(clotho_allele_counts(pop.ctx(), rows, n, m_cols.data(), m_cols.size(),
+                                   m_rows.data(), &m_ms),
+              pop.ctx(), "allele_counts");
+    }
+    std::vector<uint32_t> m_cols, m_rows;
+    uint32_t m_n = 0;
+    double m_ms = 0;
+};
+
 // ---------------------------------------------------------------------------
 struct mi355x_pipeline {};
 

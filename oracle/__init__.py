@@ -74,6 +74,7 @@ def lib():
     L.or_sim_step.argtypes = [C.c_void_p]
     L.or_sim_step.restype = C.c_int
     L.or_sim_compact.argtypes = [C.c_void_p]
+    L.or_allele_counts.argtypes = [u64p, C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, u32p]
     L.or_sim_compact.restype = C.c_int
     L.or_sim_shape.argtypes = [C.c_void_p, C.c_void_p]
     L.or_sim_rows.argtypes = [C.c_void_p, u64p]
@@ -184,6 +185,17 @@ def select(fitness, seed, generation, replicate=0):
     out = np.zeros(2 * fitness.size, np.uint32)
     lib().or_select(fitness, fitness.size, seed, generation, replicate, out)
     return out
+
+
+def allele_counts(rows, alleles, sample=None):
+    """(column counts [alleles], row counts) over rows[sample]."""
+    rows = np.ascontiguousarray(rows, np.uint64)
+    h, w = rows.shape
+    sample = np.arange(h, dtype=np.uint32) if sample is None else np.ascontiguousarray(sample, np.uint32)
+    cols = np.zeros(max(alleles, 1), np.uint32)
+    rc = np.zeros(max(sample.size, 1), np.uint32)
+    lib().or_allele_counts(rows, w, alleles, sample, sample.size, cols, rc)
+    return cols[:alleles], rc[: sample.size]
 
 
 def fitness_value(cfg, n, z):

@@ -676,6 +676,29 @@ void offspring_batch(or_sim* s, const Selector& sel, const std::vector<uint32_t>
 
 }  // namespace
 
+// Column and row margins over a row subset: the reference's set-bit walk
+// (frequency_evaluator<population_space<row_block_alignment>>::operator(),
+// include/clotho/data_spaces/analysis/allele_frequency/frequency_evaluator_population_space_row_alignment.hpp:36-61).
+extern "C" void or_allele_counts(const uint64_t* rows, uint32_t W, uint32_t A,
+                                 const uint32_t* sample, uint32_t n, uint32_t* column_counts,
+                                 uint32_t* row_counts) {
+    std::fill(column_counts, column_counts + A, 0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t* row = rows + (size_t)sample[i] * W;
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < W; ++w) {
+            uint64_t b = row[w];
+            while (b) {
+                const uint32_t j = 64u * w + (uint32_t)ctz64(b);
+                b &= b - 1;
+                if (j < A) ++column_counts[j];
+                ++m;
+            }
+        }
+        row_counts[i] = m;
+    }
+}
+
 extern "C" int or_sim_compact(or_sim* s);
 
 // Engine<parallel_pipeline>::simulate (examples/QTLSim/engine_parallel_pipeline.hpp:172-239).

@@ -68,6 +68,9 @@ int or_sim_step(or_sim* s);
 /* Remove every fixed or lost column of the current population (slots keep
  * their order and are renumbered; the free lists become empty). */
 int or_sim_compact(or_sim* s);
+/* Column and row margins of rows[sample[i]] (i < n), W words per row, slots < A. */
+void or_allele_counts(const uint64_t* rows, uint32_t W, uint32_t A, const uint32_t* sample,
+                      uint32_t n, uint32_t* column_counts, uint32_t* row_counts);
 void or_sim_shape(const or_sim* s, clotho_shape* out);
 void or_sim_rows(const or_sim* s, uint64_t* out);
 void or_sim_alleles(const or_sim* s, double* positions, double* effects, uint32_t* ages);

@@ -425,3 +425,37 @@ def test_compaction_hand_built_case():
     want = [0b101, 0b010, 0b011, 0b100]
     assert o.rows()[:, 0].tolist() == want and not o.rows()[:, 1:].any()
     assert np.array_equal(o.traits(), traits_before)
+
+
+# ---------------------------------------------------------------- statistics
+def test_allele_counts_brute_force():
+    rng = np.random.default_rng(12)
+    for h, alleles in ((1, 1), (7, 70), (130, 1000), (64, 512)):
+        words = ((alleles + 511) // 512) * 8
+        bits = rng.uniform(0, 1, (h, words * 64)) < rng.uniform(0, 1, words * 64)[None, :]
+        bits[:, alleles:] = False
+        rows = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64)
+        cols, rc = oracle.allele_counts(rows, alleles)
+        assert np.array_equal(cols, bits[:, :alleles].sum(0))
+        assert np.array_equal(rc, bits.sum(1))
+        sample = rng.choice(h, size=max(1, h // 3), replace=False).astype(np.uint32)
+        cols, rc = oracle.allele_counts(rows, alleles, sample)
+        assert np.array_equal(cols, bits[sample, :alleles].sum(0))
+        assert np.array_equal(rc, bits[sample].sum(1))
+
+
+def test_sample_rows_properties():
+    """The product's host sampler (random_sample_generator semantics): k >=
+    population gives every row in order; otherwise k distinct in-range rows,
+    deterministic in (seed, generation, replicate), roughly uniform."""
+    assert api.sample_rows(1, 1, 0, 10, 10).tolist() == list(range(10))
+    assert api.sample_rows(1, 1, 0, 10, 25).tolist() == list(range(10))
+    s = api.sample_rows(42, 5, 3, 1000, 300)
+    assert s.size == 300 and len(set(s.tolist())) == 300 and s.max() < 1000
+    assert np.array_equal(s, api.sample_rows(42, 5, 3, 1000, 300))
+    assert not np.array_equal(s, api.sample_rows(42, 6, 3, 1000, 300))
+    hits = np.zeros(50, np.int64)
+    for g in range(400):
+        hits[api.sample_rows(7, g, 0, 50, 10)] += 1
+    # each row is drawn with probability 10/50 per sample: 80 expected of 400
+    assert hits.min() > 45 and hits.max() < 120
