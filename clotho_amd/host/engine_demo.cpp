@@ -5,9 +5,11 @@
 // per-phase performance map.  Prints one JSON object.
 //
 //   engine_demo [--registry] [key=value ...] [--generations G] [--graph] [--stats K]
+//               [--log-prefix P]
 //
 // --registry lists the fitness_toolkit generators and the parsed
 // configuration without touching a GPU.
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -15,6 +17,111 @@
 #include <string>
 
 #include "clotho_amd/engine.hpp"
+#include "clotho_amd/log_writer.hpp"
+
+// The reference driver's generation loop with logging (QS/qtlsim_main.cpp:
+// 122-192): .config once, a .status record every log.period generations
+// (allele_frequency of all rows, sample.<i> for log.sampling.<i>.size, and
+// population_data with the free space, allele space and fitness), and a
+// .performance record with the per-generation simulate times, their start
+// and stop stamps, the statistics times and the free/variable/fixed counts.
+template <class Engine>
+void run_logged(Engine& engine, clotho_amd::property_tree& config, unsigned generations,
+                const std::string& prefix) {
+    using clock = std::chrono::steady_clock;
+    auto ns = [](clock::time_point t) {
+        return (long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                   t.time_since_epoch())
+            .count();
+    };
+    clotho_amd::json_tree cfg;
+    for (const auto& kv : config.items()) cfg.put(kv.first, kv.second);
+    clotho_amd::log_writer(prefix, ".config").write(cfg);
+    clotho_amd::log_writer status(prefix, ".status");
+    const unsigned period0 = (unsigned)config.get("log.period", (double)generations);
+    unsigned period = std::min(period0 ? period0 : generations, generations);
+    std::vector<long long> sim, starts, stops, stats;
+    std::vector<unsigned> free_c, var_c, fixed_c;
+    const auto run0 = clock::now();
+    for (unsigned left = generations; left-- > 0;) {
+        const auto t0 = clock::now();
+        engine.simulate();
+        const auto t1 = clock::now();
+        sim.push_back(std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count());
+        starts.push_back(ns(t0));
+        stops.push_back(ns(t1));
+        auto* fs = engine.getFreeSpace();
+        free_c.push_back((unsigned)fs->free_size());
+        var_c.push_back((unsigned)fs->variable_count());
+        fixed_c.push_back((unsigned)fs->fixed_size());
+        if (!--period) {
+            const auto s0 = clock::now();
+            clotho_amd::json_tree log;
+            auto* pop = engine.getChildPopulation();
+            const clotho_shape sh = pop->shape();
+            auto record = [&](clotho_amd::allele_frequency& af, const std::string& at) {
+                const auto& cm = af.column_margin();
+                log.at(at).add_value_array("distribution", cm.begin(), cm.end());
+                const auto fd = af.frequency_distribution();
+                log.at(at).add_value_array("frequency_distribution", fd.begin(), fd.end());
+                const auto c = clotho_amd::allele_frequency::summarize(cm);
+                const auto r = clotho_amd::allele_frequency::summarize(af.row_margin());
+                const char* names[2] = {"sequences_per_allele", "alleles_per_sequence"};
+                const clotho_amd::allele_frequency::summary* sm[2] = {&c, &r};
+                for (int q = 0; q < 2; ++q) {
+                    const std::string b = at + ".stats." + names[q] + ".";
+                    log.put(b + "min", sm[q]->min);
+                    log.put(b + "max", sm[q]->max);
+                    log.put(b + "mean", sm[q]->mean);
+                    log.put(b + "median", sm[q]->median);
+                    log.put(b + "variance", sm[q]->variance);
+                    log.put(b + "total", sm[q]->total);
+                }
+            };
+            clotho_amd::allele_frequency all;
+            all.evaluate(*pop);
+            record(all, "allele_frequency");
+            for (unsigned i = 0;; ++i) {
+                const double k = config.get("log.sampling." + std::to_string(i) + ".size", -1.0);
+                if (k < 0) break;
+                if (k == 0) continue;
+                clotho_amd::allele_frequency af;
+                af.evaluate(*pop, (uint64_t)config.get("random_number.seed", 0.0), sh.generation,
+                            (uint32_t)k);
+                record(af, "sample." + std::to_string(i) + ".allele_frequency");
+            }
+            log.at("population_data.free_space")
+                .add_value_array("free", fs->free_begin(), fs->free_end());
+            log.put("population_data.free_space.free_size", fs->free_size());
+            log.put("population_data.free_space.variable_size", fs->variable_count());
+            auto* alleles = engine.getAlleleSpace();
+            clotho_amd::json_tree& as = log.at("population_data.allele_space");
+            for (size_t j = 0; j < alleles->positions.size(); ++j) {
+                clotho_amd::json_tree a;
+                a.put("position", alleles->positions[j]);
+                a.put("age", alleles->ages[j]);
+                as.push_back(a);
+            }
+            const auto& fit = engine.getFitness();
+            log.at("population_data.child").add_value_array("fitness", fit.begin(), fit.end());
+            status.write(log);
+            stats.push_back(std::chrono::duration_cast<std::chrono::nanoseconds>(clock::now() - s0)
+                                .count());
+            period = std::min(period0 ? period0 : generations, left);
+        }
+    }
+    clotho_amd::json_tree perf;
+    perf.put("performance.runtime",
+             std::chrono::duration_cast<std::chrono::nanoseconds>(clock::now() - run0).count());
+    perf.at("performance").add_value_array("simulate", sim.begin(), sim.end());
+    perf.at("performance.generations").add_value_array("start", starts.begin(), starts.end());
+    perf.at("performance.generations").add_value_array("stop", stops.begin(), stops.end());
+    perf.at("performance").add_value_array("stats", stats.begin(), stats.end());
+    perf.at("memory").add_value_array("free_count", free_c.begin(), free_c.end());
+    perf.at("memory").add_value_array("variable_count", var_c.begin(), var_c.end());
+    perf.at("memory").add_value_array("fixed_count", fixed_c.begin(), fixed_c.end());
+    clotho_amd::log_writer(prefix, ".performance").write(perf);
+}
 
 int main(int argc, char** argv) {
     clotho_amd::property_tree config;
@@ -33,12 +140,17 @@ int main(int argc, char** argv) {
     unsigned generations = 10;
     bool graph = false, registry = false;
     long stats_sample = -1;  // --stats K: allele-frequency statistics over K rows (0 = all)
+    std::string log_prefix;  // --log-prefix P: the reference's .config/.status/.performance files
+    bool logging = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "--registry") {
             registry = true;
         } else if (a == "--graph") {
             graph = true;
+        } else if (a == "--log-prefix" && i + 1 < argc) {
+            log_prefix = argv[++i];
+            logging = true;
         } else if (a == "--stats" && i + 1 < argc) {
             stats_sample = std::stol(argv[++i]);
         } else if (a == "--generations" && i + 1 < argc) {
@@ -63,7 +175,9 @@ int main(int argc, char** argv) {
                                clotho_amd::mi355x_pipeline>
         engine_type;
     engine_type engine(&rng, config);
-    if (graph) {
+    if (logging) {
+        run_logged(engine, config, generations, log_prefix);
+    } else if (graph) {
         engine.simulate(generations, true);
     } else {
         for (unsigned g = 0; g < generations; ++g) engine.simulate();

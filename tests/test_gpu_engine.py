@@ -38,3 +38,47 @@ def test_engine_adapter_matches_oracle(graph):
     assert out["generation"] == s.generation and out["alleles"] == s.alleles
     assert out["free"] == s.free_count and out["fixed"] == s.fixed_count
     assert out["rows_fnv1a"] == fnv1a(orc.rows())
+
+
+def test_engine_demo_log_files(tmp_path):
+    """The demo's .config/.status/.performance records (the reference
+    driver's loop) carry the statistics, free space, alleles and per-generation
+    counts of the oracle's run."""
+    prefix = str(tmp_path / "run")
+    args = [build.DEMO, "population_growth.params.B=300", "mutation.mu=2",
+            "recombination.rho=1.5", "random_number.seed=4", "log.period=3",
+            "log.sampling.0.size=100", "--generations", "6", "--log-prefix", prefix]
+    subprocess.check_call(args, stdout=subprocess.DEVNULL)
+    cfg_log = json.load(open(prefix + ".0.config.json"))
+    assert cfg_log["mutation"]["mu"] == "2" and cfg_log["log"]["period"] == "3"
+    cfg = api.make_config(300, 2.0, 1.5, seed=4, traits=1, neutral_p=1.0, max_alleles=32768)
+    orc = oracle.OracleSim(cfg)
+    free_counts = []
+    for k in range(2):
+        for _ in range(3):
+            orc.step()
+            free_counts.append(orc.shape().free_count)
+        st = json.load(open(f"{prefix}.{k}.status.json"))
+        s = orc.shape()
+        cols, rc = oracle.allele_counts(orc.rows(), s.alleles)
+        af = st["allele_frequency"]
+        assert [int(x) for x in af["distribution"]] == cols.tolist()
+        fd = np.bincount(cols, minlength=s.haplotypes + 1)
+        assert [int(x) for x in af["frequency_distribution"]] == fd.tolist()
+        assert float(af["stats"]["alleles_per_sequence"]["mean"]) == pytest.approx(rc.mean())
+        sample = api.sample_rows(4, s.generation, 0, s.haplotypes, 100)
+        scols, src = oracle.allele_counts(orc.rows(), s.alleles, sample)
+        sa = st["sample"]["0"]["allele_frequency"]
+        assert [int(x) for x in sa["distribution"]] == scols.tolist()
+        pd = st["population_data"]
+        assert int(pd["free_space"]["free_size"]) == s.free_count
+        fx, ls = orc.free_space()
+        assert [int(x) for x in pd["free_space"]["free"]] == fx.tolist() + ls.tolist()
+        pos, _, age = orc.alleles()
+        assert [float(a["position"]) for a in pd["allele_space"]] == pos.tolist()
+        assert [int(a["age"]) for a in pd["allele_space"]] == age.tolist()
+        assert [float(x) for x in pd["child"]["fitness"]] == orc.fitness().tolist()
+    perf = json.load(open(prefix + ".0.performance.json"))
+    assert len(perf["performance"]["simulate"]) == 6
+    assert len(perf["performance"]["stats"]) == 2
+    assert [int(x) for x in perf["memory"]["free_count"]] == free_counts
