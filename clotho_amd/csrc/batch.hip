@@ -61,6 +61,7 @@ int sync_views(clotho_batch* b) {
         std::memset(&v[r], 0, sizeof(RepView));
         v[r].P = b->reps[r]->p;
         v[r].B = b->reps[r]->b;
+        v[r].B.by_parent = nullptr;  // replicates recombine in child order
     }
     if (b->d_views && b->host_views.size() == R &&
         std::memcmp(v.data(), b->host_views.data(), sizeof(RepView) * R) == 0)

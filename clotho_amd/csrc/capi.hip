@@ -67,7 +67,8 @@ void free_buffers(clotho_ctx* c) {
                     B.sorted,     B.pm,        B.digits,    B.bucket,
                     B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
                     B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
-                    B.cdf_rho,    B.cdf_mu,    B.state};
+                    B.cdf_rho,    B.cdf_mu,    B.state,     B.by_parent,  B.parent_cnt,
+                    B.parent_bsum};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     std::memset(&B, 0, sizeof(B));
@@ -279,6 +280,9 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.bottom, P.h);
     rc |= dalloc(c, &B.n_events, P.h);
     rc |= dalloc(c, &B.events, (size_t)P.h * P.ev_cap);
+    rc |= dalloc(c, &B.by_parent, P.h);
+    rc |= dalloc(c, &B.parent_cnt, P.n);
+    rc |= dalloc(c, &B.parent_bsum, sel_blocks);
     rc |= dalloc(c, &B.var_part, (size_t)(P.max_chunks + 1) * P.w_cap);
     rc |= dalloc(c, &B.fix_part, (size_t)P.max_chunks * P.w_cap);
     rc |= dalloc(c, &B.word_cnt, P.w_cap);
@@ -311,6 +315,12 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     if (ok) {
         std::vector<double> ones(P.n, 1.0);
         ok = hipMemcpy(B.fitness[0], ones.data(), 8 * P.n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (ok) {  // child order and empty gametes until the first draw
+        std::vector<uint4> rec(P.h);
+        for (uint32_t o = 0; o < P.h; ++o) rec[o] = make_uint4(o, 0, 0, 0);
+        ok = hipMemcpy(B.by_parent, rec.data(), sizeof(uint4) * (size_t)P.h,
+                       hipMemcpyHostToDevice) == hipSuccess;
     }
     GenState st;
     std::memset(&st, 0, sizeof(st));

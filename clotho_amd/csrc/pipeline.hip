@@ -106,6 +106,7 @@ struct MutCounts {
         const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
         if (o < P.h) local[o] = inc - c;
         if (threadIdx.x == 0) bsum[blk] = total;
+        if (B.by_parent && o < P.n) gbl(B.parent_cnt)[o] = 0;  // counted by the gamete draws
     }
 };
 
@@ -403,11 +404,94 @@ struct Gametes {
         top_out[o] = swap ? 2 * p + 1 : 2 * p;
         bot_out[o] = swap ? 2 * p : 2 * p + 1;
         nev_out[o] = ne;
+        if (B.by_parent && p < P.n)
+            __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         const auto ev = events + (size_t)o * P.ev_cap;
         for (uint32_t e = 0; e < ne; e += 4) {
             const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
             for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
         }
+    }
+};
+
+// Parent order of the child rows: a counting sort of the gametes by parent
+// individual (counts from the gamete draws, block-local exclusive scan, scan
+// of the block totals, scatter).  Only the processing order of the
+// recombination depends on it; which of two siblings comes first is decided by
+// the scatter's atomics and changes nothing in the output.
+struct ParentScan {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const auto cnt = gbl(B.parent_cnt);
+        __shared__ uint32_t lds[kWaves];
+        const uint32_t i = blk * kBlock + threadIdx.x;
+        const uint32_t c = i < P.n ? cnt[i] : 0u;
+        uint32_t total;
+        const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
+        if (i < P.n) cnt[i] = inc - c;
+        if (threadIdx.x == 0) gbl(B.parent_bsum)[blk] = total;
+    }
+};
+
+struct ParentBlockscan {
+    static constexpr int kThreads = 1024;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return 1u; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        (void)blk;
+        const auto bsum = gbl(B.parent_bsum);
+        __shared__ uint32_t s_wave[16];
+        __shared__ uint32_t s_carry;
+        const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        const uint32_t nb = (P.n + kBlock - 1) / kBlock;
+        if (tid == 0) s_carry = 0;
+        __syncthreads();
+        for (uint32_t base = 0; base < nb; base += 1024) {
+            const uint32_t idx = base + tid;
+            const uint32_t v = idx < nb ? bsum[idx] : 0u;
+            uint32_t x = v;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if ((int)lane >= d) x += y;
+            }
+            if (lane == 63) s_wave[wave] = x;
+            __syncthreads();
+            uint32_t off = s_carry, tot = 0;
+            for (uint32_t q = 0; q < 16; ++q) {
+                if (q < wave) off += s_wave[q];
+                tot += s_wave[q];
+            }
+            if (idx < nb) bsum[idx] = off + x - v;
+            __syncthreads();
+            if (tid == 0) s_carry += tot;
+            __syncthreads();
+        }
+    }
+};
+
+struct ParentScatter {
+    static constexpr int kThreads = kBlock;
+    __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        (void)sel;
+        const uint32_t o = blk * kBlock + threadIdx.x;
+        if (o >= P.h) return;
+        const uint32_t top = gbl(B.top)[o], p = top >> 1;
+        if (p >= P.n) return;
+        const uint32_t pos = gbl(B.parent_bsum)[p / kBlock] +
+                             __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t k0 = gbl(B.mut_local)[o] + gbl(B.mut_bsum)[o / kBlock];
+        const uint32_t nx = gbl(B.n_events)[o] | (gbl(B.mut_count)[o] << 8);
+        if (pos < P.h)
+            *(__attribute__((address_space(1))) i8::u32x4*)(B.by_parent + pos) =
+                i8::u32x4{o, top, nx, k0};
     }
 };
 
@@ -499,6 +583,9 @@ constexpr int kRecBlock = 64 * kRecWaves;
 // events classified together; any even size keeps the padding parity
 // argument (#real + #padding even), and 2 pads Poisson(5) counts least
 constexpr int kEventGroup = 2;
+// children of one parent individual produced from one load of its row pair
+// (parent-ordered gametes; the batched launches keep child order and take 1)
+constexpr int kSibsOrdered = 4;
 // dword 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
 // data format, offsets range-checked against num_records in bytes
 constexpr int kBufferWord3 = 0x00020000;
@@ -512,7 +599,7 @@ __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
 // that drops the lanes past the row), so the number of vector memory
 // operations in flight is the same on every path and the waits before a row's
 // compute cover only that row's loads.
-template <bool kWideEvents>
+template <bool kWideEvents, int kSibs>
 struct Recombine {
     static constexpr int kThreads = kRecBlock;
     __host__ __device__ static uint32_t blocks(const Params& P) { return P.tiles * P.chunks; }
@@ -528,7 +615,6 @@ struct Recombine {
         const auto free_list = gbl(B.free_list);
         const auto fixed_bits = gbl(B.fixed_bits);
         const auto top_in = gbl(B.top);
-        const auto bot_in = gbl(B.bottom);
         const auto nev_in = gbl(B.n_events);
         const auto events = gbl(B.events);
         const auto count = gbl(B.mut_count);
@@ -536,6 +622,8 @@ struct Recombine {
         const auto boff = gbl(B.mut_bsum);
         const auto var_part = gbl(B.var_part);
         const auto fix_part = gbl(B.fix_part);
+        const auto by_parent = (const __attribute__((address_space(1))) i8::u32x4*)B.by_parent;
+        const bool ordered = B.by_parent != nullptr;
         const uint32_t bx = blk % P.tiles, by = blk / P.tiles;
         __shared__ uint32_t s_bucket[64 * 64];
         __shared__ uint32_t s_sorted[68 * 64];
@@ -564,35 +652,92 @@ struct Recombine {
         const uint64_t keep = inpar ? ~fixed_bits[w] : 0;  // also clears columns >= wp
         __syncthreads();
 
+        // positions [c0, c1) of this row chunk.  With sibling items a wave
+        // takes the contiguous segment [s0, s1), so the children of one parent
+        // meet in one wave; without, the waves interleave (wave w takes
+        // c0 + w + 16 k), so a block writes consecutive child rows together.
         const uint32_t rows_per_chunk = (P.h + P.chunks - 1) / P.chunks;
-        const uint32_t r0 = by * rows_per_chunk + wave;
-        const uint32_t r1 = min(P.h, by * rows_per_chunk + rows_per_chunk);
-        const uint32_t nrows = r1 > r0 ? (r1 - r0 + kRecWaves - 1) / kRecWaves : 0;
+        const uint32_t c0 = min(P.h, by * rows_per_chunk);
+        const uint32_t c1 = min(P.h, c0 + rows_per_chunk);
+        uint32_t first, stride, npos;
+        if (kSibs > 1) {
+            const uint32_t seg = (c1 - c0 + kRecWaves - 1) / kRecWaves;
+            first = min(c1, c0 + wave * seg);
+            stride = 1;
+            npos = min(c1, first + seg) - first;
+        } else {
+            first = c0 + wave;
+            stride = kRecWaves;
+            npos = c1 > first ? (c1 - first + kRecWaves - 1) / kRecWaves : 0;
+        }
         uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-        for (uint32_t j0 = 0; j0 < nrows; j0 += 64) {
-            const uint32_t nb = min(64u, nrows - j0);
-            // gamete records of rows j0 .. j0 + nb - 1, lane q = row j0 + q
-            auto row_of = [&](uint32_t q) { return r0 + kRecWaves * (j0 + q); };
-            const uint32_t ol = row_of(min(lane, nb - 1));
-            const uint32_t g_top = top_in[ol];
-            const uint32_t g_bot = bot_in[ol];
-            const uint32_t g_ne = nev_in[ol];
-            const uint32_t g_nm = count[ol];
-            const uint32_t g_k0 = local[ol] + boff[ol / kBlock];
-
-            // rows past the batch reload its last row (in bounds, never processed)
-            auto load_row = [&](uint32_t q, uint64_t& t, uint64_t& b, uint32_t& lo, uint32_t& hi) {
-                const uint32_t qq = min(q, nb - 1);
-                const auto ev = events + (size_t)row_of(qq) * ev_cap;
-                lo = ev[min(lane, ev_cap - 1)];
-                if (kWideEvents) hi = ev[min(lane + 64, ev_cap - 1)];
-                t = parent[(size_t)lane_read(g_top, qq) * wp + col];
-                b = parent[(size_t)lane_read(g_bot, qq) * wp + col];
+        for (uint32_t j0 = 0; j0 < npos; j0 += 64) {
+            const uint32_t nb = min(64u, npos - j0);
+            // gamete record of position j0 + lane (the last one repeated past
+            // the batch): child row, top row (bottom = top ^ 1), events |
+            // mutations << 8, first mutation
+            const uint32_t pos = first + stride * (j0 + min(lane, nb - 1));
+            uint32_t g_row, g_top, g_nx, g_k0;
+            if (ordered) {
+                const i8::u32x4 r = by_parent[pos];
+                g_row = r.x;
+                g_top = r.y;
+                g_nx = r.z;
+                g_k0 = r.w;
+            } else {
+                g_row = pos;
+                g_top = top_in[pos];
+                g_nx = nev_in[pos] | (count[pos] << 8);
+                g_k0 = local[pos] + boff[pos / kBlock];
+            }
+            // items: runs of at most kSibs consecutive positions whose gametes
+            // share the parent individual; one item loads the parent pair once
+            uint64_t items = 0;
+            if (kSibs > 1) {
+                const uint32_t parent_id = g_top >> 1;
+                const uint32_t prev_id = __shfl_up(parent_id, 1, 64);
+                const uint64_t starts = __ballot(lane == 0 || lane >= nb || parent_id != prev_id);
+                const uint64_t upto = lane == 63 ? ~(uint64_t)0 : (((uint64_t)2 << lane) - 1);
+                const uint32_t run0 = 63u - (uint32_t)__builtin_clzll(starts & upto);
+                items = __ballot(lane < nb && (lane - run0) % kSibs == 0);
+            }
+            uint32_t next_q = 0;  // kSibs == 1: every position is an item
+            struct Item {
+                uint32_t q, len;  // first position (lane) and number of children
+                uint64_t t, b;    // parent rows top-of-first-child and its partner
+                uint32_t lo[kSibs], hi[kSibs];
+            };
+            auto next_item = [&](Item& it) {
+                if (kSibs == 1) {
+                    it.q = next_q < nb ? next_q++ : 64u;
+                    it.len = 1;
+                } else if (items) {
+                    it.q = (uint32_t)__builtin_ctzll(items);
+                    items &= items - 1;
+                    it.len = (items ? (uint32_t)__builtin_ctzll(items) : nb) - it.q;
+                } else {
+                    it.q = 64;
+                    it.len = 0;
+                }
+            };
+            // loads of an item (issued for every item slot, indices clamped in bounds)
+            auto load_item = [&](Item& it) {
+                const uint32_t qq = min(it.q, nb - 1);
+                const uint32_t top = lane_read(g_top, qq);
+                it.t = parent[(size_t)top * wp + col];
+                it.b = parent[(size_t)(top ^ 1u) * wp + col];
+#pragma unroll
+                for (int k = 0; k < kSibs; ++k) {
+                    const auto ev = events + (size_t)lane_read(g_row, min(qq + k, nb - 1)) * ev_cap;
+                    it.lo[k] = ev[min(lane, ev_cap - 1)];
+                    if (kWideEvents) it.hi[k] = ev[min(lane + 64, ev_cap - 1)];
+                }
             };
 
-            auto process = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
-                const uint32_t ne = lane_read(g_ne, q);
+            auto child_row = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
+                const uint32_t nx = lane_read(g_nx, q);
+                const uint32_t ne = nx & 0xFFu;
                 uint64_t m = 0;
                 for (uint32_t e = 0; e < ne; e += kEventGroup) {
                     uint32_t key[kEventGroup], p[kEventGroup];
@@ -639,7 +784,7 @@ struct Recombine {
                 b &= keep;
                 uint64_t c = (t & ~m) | (b & m);
                 // new mutations of this row (free columns: zero in every parent row)
-                const uint32_t nm = lane_read(g_nm, q);
+                const uint32_t nm = nx >> 8;
                 if (nm) {
                     const uint32_t k0 = lane_read(g_k0, q);
                     for (uint32_t k = k0; k < k0 + nm; ++k) {
@@ -649,7 +794,7 @@ struct Recombine {
                 }
                 {
                     const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
-                        child + (size_t)row_of(q) * wc, 0, (int)(wc * 8u), kBufferWord3);
+                        child + (size_t)lane_read(g_row, q) * wc, 0, (int)(wc * 8u), kBufferWord3);
                     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
                     const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
                     __builtin_amdgcn_raw_buffer_store_b64(v, row, valid ? w * 8u : 0x80000000u, 0,
@@ -659,15 +804,33 @@ struct Recombine {
                 acc_and &= c;
             };
 
-            uint64_t tA = 0, bA = 0, tB = 0, bB = 0;
-            uint32_t loA = 0, hiA = 0, loB = 0, hiB = 0;
-            load_row(0, tA, bA, loA, hiA);
-            load_row(1, tB, bB, loB, hiB);
-            for (uint32_t q = 0; q < nb; q += 2) {
-                process(q, tA, bA, loA, hiA);
-                load_row(q + 2, tA, bA, loA, hiA);
-                if (q + 1 < nb) process(q + 1, tB, bB, loB, hiB);
-                load_row(q + 3, tB, bB, loB, hiB);
+            // each child takes the loaded pair in its own strand order
+            auto process_item = [&](const Item& it) {
+                const uint32_t top0 = lane_read(g_top, it.q);
+#pragma unroll
+                for (int k = 0; k < kSibs; ++k) {
+                    if ((uint32_t)k < it.len) {
+                        const uint32_t q = it.q + k;
+                        const bool flip = lane_read(g_top, q) != top0;
+                        child_row(q, flip ? it.b : it.t, flip ? it.t : it.b, it.lo[k],
+                                  kWideEvents ? it.hi[k] : 0u);
+                    }
+                }
+            };
+
+            Item iA, iB;
+            next_item(iA);
+            load_item(iA);
+            next_item(iB);
+            load_item(iB);
+            while (iA.q < 64) {
+                process_item(iA);
+                next_item(iA);
+                load_item(iA);
+                if (iB.q >= 64) break;
+                process_item(iB);
+                next_item(iB);
+                load_item(iB);
             }
         }
         __syncthreads();  // every wave is past its last prefix-mask read
@@ -1280,8 +1443,19 @@ struct TraitsI8 {
 // replicate's Params and Buffers read from device memory; a barrier separates
 // consecutive virtual blocks so that shared memory is reused safely.
 // ---------------------------------------------------------------------------
+template <class Op>
+struct MinWaves {
+    static constexpr int value = 1;
+};
+template <bool W, int K>
+struct MinWaves<Recombine<W, K>> {
+    static constexpr int value = 8;
+};
+
 #define CLOTHO_SINGLE(name, Op)                                                                 \
-    __global__ void __launch_bounds__(Op::kThreads) name(Params P, Buffers B, uint32_t sel) {  \
+    __global__ void __launch_bounds__(Op::kThreads)                                            \
+        __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value)))                               \
+        name(Params P, Buffers B, uint32_t sel) {                                               \
         Op::run(P, B, sel, blockIdx.x);                                                         \
     }
 
@@ -1292,9 +1466,14 @@ CLOTHO_SINGLE(k_mut_attrs, MutAttrs)
 CLOTHO_SINGLE(k_sel_scan, SelScan)
 CLOTHO_SINGLE(k_sel_blockscan, SelBlockscan)
 CLOTHO_SINGLE(k_gametes, Gametes)
+CLOTHO_SINGLE(k_parent_scan, ParentScan)
+CLOTHO_SINGLE(k_parent_blockscan, ParentBlockscan)
+CLOTHO_SINGLE(k_parent_scatter, ParentScatter)
 CLOTHO_SINGLE(k_word_sort, WordSort)
-using RecombineNarrow = Recombine<false>;  // at most 64 events per gamete
-using RecombineWide = Recombine<true>;
+using RecombineNarrow = Recombine<false, kSibsOrdered>;  // at most 64 events per gamete
+using RecombineWide = Recombine<true, 2>;  // two sets of 128 event keys stay in registers
+using BatchRecombineNarrow = Recombine<false, 1>;
+using BatchRecombineWide = Recombine<true, 1>;
 CLOTHO_SINGLE(k_recombine, RecombineNarrow)
 CLOTHO_SINGLE(k_recombine_wide, RecombineWide)
 CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
@@ -1315,15 +1494,6 @@ CLOTHO_SINGLE(k_commit, Commit)
 // Occupancy floor of a batched kernel: the recombination kernel needs two
 // resident 1024-thread blocks per CU (8 waves per SIMD, <= 64 VGPRs) to keep
 // enough parent loads in flight.
-template <class Op>
-struct MinWaves {
-    static constexpr int value = 1;
-};
-template <bool W>
-struct MinWaves<Recombine<W>> {
-    static constexpr int value = 8;
-};
-
 template <class Op>
 __global__ void __launch_bounds__(Op::kThreads)
     __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value))) k_batched(const RepView* __restrict__ views,
@@ -1385,6 +1555,11 @@ hipError_t launch_draw(clotho_ctx* c) {
     TRY_LAUNCH(launch_single<SelScan>(k_sel_scan, c, par));
     TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
     TRY_LAUNCH(launch_single<Gametes>(k_gametes, c, par));
+    if (c->b.by_parent) {
+        TRY_LAUNCH(launch_single<ParentScan>(k_parent_scan, c, par));
+        TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
+        TRY_LAUNCH(launch_single<ParentScatter>(k_parent_scatter, c, par));
+    }
     return hipSuccess;
 }
 
@@ -1486,9 +1661,9 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
     TRY_LAUNCH(mark(1));
     TRY_LAUNCH(launch_batched<WordSort>(L, kOpWordSort, par, s));
     if (L.wide_events)
-        TRY_LAUNCH(launch_batched<RecombineWide>(L, kOpRecombine, par, s));
+        TRY_LAUNCH(launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s));
     else
-        TRY_LAUNCH(launch_batched<RecombineNarrow>(L, kOpRecombine, par, s));
+        TRY_LAUNCH(launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s));
     TRY_LAUNCH(mark(2));
     if (L.traits) {
         const uint32_t chi = 1 - par;

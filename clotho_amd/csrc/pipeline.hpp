@@ -83,6 +83,13 @@ struct Buffers {
     uint32_t* bottom;         // [h]
     uint32_t* n_events;       // [h]
     uint32_t* events;         // [h][ev_cap] event keys
+    // Gamete records in parent order (the children of one parent individual
+    // adjacent): {child row, top row, n_events | mutations << 8, first
+    // mutation}.  The recombination loads a parent row pair once for up to
+    // kSibs of its children; nullptr = child order (the batched launches).
+    uint4* by_parent;         // [h]
+    uint32_t* parent_cnt;     // [n] children per parent -> exclusive in-block offsets
+    uint32_t* parent_bsum;    // [ceil(n/256)] block totals -> exclusive offsets
     uint64_t* var_part;       // [chunks][w_cap] OR partials
     uint64_t* fix_part;       // [chunks][w_cap] AND partials
     uint32_t* word_cnt;       // [w_cap] (fixed count << 16 | lost count) per word
