@@ -46,7 +46,8 @@ class Config(C.Structure):
         ("effect_quantum_log2", C.c_int32),
         ("max_alleles", C.c_uint32),
         ("compact_every", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("max_individuals", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -74,7 +75,7 @@ class Shape(C.Structure):
 
 def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, neutral_p=0.5,
                 mean=0.0, sigma=1.0, fitness_scale=1.0, env_sigma=0.0, quantum_log2=-24,
-                max_alleles=1 << 20, replicate=0, compact=False):
+                max_alleles=1 << 20, replicate=0, compact=False, max_individuals=0):
     """Build a Config; defaults follow the reference's parameter defaults
     (sequence_bias.p 0.5, neutral.p 0.5, traits.mean 0, traits.sigma 1)."""
     c = Config()
@@ -94,6 +95,7 @@ def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, n
     c.effect_quantum_log2 = quantum_log2
     c.max_alleles = max_alleles
     c.compact_every = 1 if compact else 0
+    c.max_individuals = max_individuals
     return c
 
 
@@ -117,7 +119,7 @@ EXPORTS = [
     "clotho_batch_create", "clotho_batch_destroy", "clotho_batch_last_error",
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
     "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
-    "clotho_sample_rows", "clotho_allele_counts",
+    "clotho_sample_rows", "clotho_allele_counts", "clotho_set_population_size",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -151,6 +153,7 @@ def lib():
     L.clotho_download_fitness.argtypes = [vp, vp, C.c_size_t]
     L.clotho_download_gametes.argtypes = [vp, vp, vp, vp, vp, C.c_size_t]
     L.clotho_run.argtypes = [vp, C.c_uint32, C.c_int]
+    L.clotho_set_population_size.argtypes = [vp, C.c_uint32]
     for n in ("clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
               "clotho_phase_traits",
               "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit",
@@ -331,6 +334,11 @@ class Simulation:
 
     def compact(self):
         self._check(lib().clotho_compact(self.ctx), "compact")
+
+    def set_population_size(self, n):
+        """Offspring individuals of every following generation (the pN of a
+        population-growth model); at most max(N, max_individuals)."""
+        self._check(lib().clotho_set_population_size(self.ctx, int(n)), "set_population_size")
 
     def autotune(self):
         """Pick the recombination row-chunk count by timing (state unchanged)."""

@@ -192,6 +192,10 @@ int clotho_batch_run(clotho_batch* b, uint32_t n, int mode, double* elapsed_ms) 
             b->err = "batch_run: a replicate was advanced outside the batch";
             return CLOTHO_ERR_STATE;
         }
+        if ((c->next_n && c->next_n != c->p.n) || c->p.n_par != c->p.n) {
+            b->err = "batch_run: replicate population sizes are fixed inside a batch";
+            return CLOTHO_ERR_ARG;
+        }
         BATCH_TRY(b, hipStreamSynchronize(c->stream));  // uploads done
     }
     int rc = sync_views(b);

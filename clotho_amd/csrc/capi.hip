@@ -99,6 +99,49 @@ int write_state(clotho_ctx* c, const GenState& st) {
     return CLOTHO_OK;
 }
 
+// ifitness_generator::generate(N) (src/clotho/fitness/normal_fitness_generator.cpp:36-52,
+// src/clotho/fitness/quadratic_fitness_generator.cpp:42-54): constants for the
+// P.n individuals being evaluated.
+void set_fitness_constants(Params& P, const clotho_config& cfg) {
+    P.fitness_kind = cfg.fitness_kind;
+    P.fit_coeff = 1.0;
+    P.fit_inv = 1.0;
+    if (cfg.fitness_kind == CLOTHO_FIT_GAUSSIAN) {
+        const double sigma = std::sqrt(4.0 * (double)P.n * cfg.mu);
+        P.fit_coeff = (1.0 / sigma) * 0.398942280401432677939946059934;
+        P.fit_inv = 1.0 / (2.0 * sigma * sigma);
+    } else if (cfg.fitness_kind == CLOTHO_FIT_QUADRATIC) {
+        const double s = cfg.fitness_scale * std::sqrt(4.0 * (double)P.n * cfg.mu);
+        P.fit_inv = 1.0 / s;
+    } else if (cfg.fitness_kind == CLOTHO_FIT_GAUSSIAN_FIXED ||
+               cfg.fitness_kind == CLOTHO_FIT_MV_STABILIZING) {
+        P.fit_inv = 1.0 / (2.0 * cfg.fitness_scale * cfg.fitness_scale);
+    }
+}
+
+// Sizes of the generation about to be enqueued: the current population is the
+// parent, the offspring population takes the requested size
+// (Engine::simulate, QS/engine_parallel_pipeline.hpp:181-197).  Launch
+// arguments change with the sizes, so captured graphs are dropped.
+void begin_generation(clotho_ctx* c) {
+    Params& P = c->p;
+    const uint32_t n = c->next_n ? c->next_n : P.n;
+    if (P.n_par == P.n && n == P.n) return;
+    P.n_par = P.n;
+    P.n = n;
+    P.h = 2 * n;
+    P.mut_blocks = (P.h + kScanBlock - 1) / kScanBlock;
+    set_fitness_constants(P, c->cfg);
+    destroy_graphs(c);
+}
+
+// After a generation the offspring population is the current one.
+void end_generation(clotho_ctx* c) {
+    if (c->p.n_par == c->p.n) return;
+    c->p.n_par = c->p.n;
+    destroy_graphs(c);
+}
+
 int enqueue_generation(clotho_ctx* c, std::vector<hipEvent_t>* marks) {
     auto mark = [&](int i) -> hipError_t {
         if (!marks) return hipSuccess;
@@ -169,6 +212,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     if (!cfg || !out) return CLOTHO_ERR_ARG;
     *out = nullptr;
     if (cfg->n_individuals == 0 || cfg->n_individuals > (1u << 28)) return CLOTHO_ERR_ARG;
+    if (cfg->max_individuals > (1u << 28)) return CLOTHO_ERR_ARG;
     if (cfg->trait_count > (uint32_t)kMaxTraits) return CLOTHO_ERR_ARG;
     if (!(cfg->bias >= 0.0 && cfg->bias <= 1.0)) return CLOTHO_ERR_ARG;
     if (!(cfg->neutral_p >= 0.0 && cfg->neutral_p <= 1.0)) return CLOTHO_ERR_ARG;
@@ -212,6 +256,9 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     P.replicate = cfg->replicate;
     P.n = cfg->n_individuals;
     P.h = 2 * P.n;
+    P.n_par = P.n;
+    P.n_cap = std::max(P.n, cfg->max_individuals);
+    const uint32_t h_cap = 2 * P.n_cap;
     P.traits = cfg->trait_count;
     P.s_cap = ((cfg->max_alleles + 511) / 512) * 512;
     P.w_cap = P.s_cap / 64;
@@ -225,44 +272,31 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     P.env_sigma = cfg->env_sigma;
     P.eff_scale = std::ldexp(1.0, -cfg->effect_quantum_log2);
     P.eff_quantum = std::ldexp(1.0, cfg->effect_quantum_log2);
-    P.fitness_kind = cfg->fitness_kind;
-    P.fit_coeff = 1.0;
-    P.fit_inv = 1.0;
-    // ifitness_generator::generate(N): constants for N individuals
-    if (cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN) {
-        const double sigma = std::sqrt(4.0 * (double)P.n * cfg->mu);
-        P.fit_coeff = (1.0 / sigma) * 0.398942280401432677939946059934;
-        P.fit_inv = 1.0 / (2.0 * sigma * sigma);
-    } else if (cfg->fitness_kind == CLOTHO_FIT_QUADRATIC) {
-        const double s = cfg->fitness_scale * std::sqrt(4.0 * (double)P.n * cfg->mu);
-        P.fit_inv = 1.0 / s;
-    } else if (cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN_FIXED ||
-               cfg->fitness_kind == CLOTHO_FIT_MV_STABILIZING) {
-        P.fit_inv = 1.0 / (2.0 * cfg->fitness_scale * cfg->fitness_scale);
-    }
+    set_fitness_constants(P, *cfg);
     P.tiles = (P.w_cap + 63) / 64;
     {
         // default launch shape: ~4096 recombination blocks, but at least ~32
         // rows per wave (a block's order-statistic tables are loaded once per
         // block); clotho_autotune may pick another row-chunk count up to max_chunks
         P.max_chunks = std::max<uint32_t>(1, std::min<uint32_t>(
-                                                 (8192 + P.tiles - 1) / P.tiles, P.h / 8));
+                                                 (8192 + P.tiles - 1) / P.tiles, h_cap / 8));
         uint32_t ch = (4096 + P.tiles - 1) / P.tiles;
         ch = std::min<uint32_t>(ch, std::max<uint32_t>(1, (P.h + 511) / 512));
         P.chunks = std::min<uint32_t>(P.max_chunks, std::max<uint32_t>(1, ch));
     }
     P.mut_blocks = (P.h + kScanBlock - 1) / kScanBlock;
     P.word_blocks = (P.w_cap + 63) / 64;  // 64-word free-space blocks
-    const uint32_t sel_blocks = (P.n + kScanBlock - 1) / kScanBlock;
+    const uint32_t sel_blocks = (P.n_cap + kScanBlock - 1) / kScanBlock;  // at capacity
+    const uint32_t mut_blocks_cap = (h_cap + kScanBlock - 1) / kScanBlock;
     const uint32_t T1 = std::max<uint32_t>(1, P.traits);
 
     Buffers& B = c->b;
     std::memset(&B, 0, sizeof(B));
     int rc = CLOTHO_OK;
     for (int k = 0; k < 2 && rc == CLOTHO_OK; ++k) {
-        rc |= dalloc(c, &B.rows[k], (size_t)P.h * P.w_cap);
-        rc |= dalloc(c, &B.traits[k], (size_t)P.h * T1);
-        rc |= dalloc(c, &B.fitness[k], P.n);
+        rc |= dalloc(c, &B.rows[k], (size_t)h_cap * P.w_cap);
+        rc |= dalloc(c, &B.traits[k], (size_t)h_cap * T1);
+        rc |= dalloc(c, &B.fitness[k], P.n_cap);
     }
     rc |= dalloc(c, &B.keys, P.s_cap);
     rc |= dalloc(c, &B.effects, (size_t)P.s_cap * T1);
@@ -273,21 +307,21 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.pm, (size_t)P.w_cap * 65);
     rc |= dalloc(c, &B.bucket, (size_t)P.w_cap * 64);
     rc |= dalloc(c, &B.digits, i8_packed_elems(std::max<uint32_t>(1, P.traits), P.w_cap));
-    rc |= dalloc(c, &B.mut_count, P.h);
-    rc |= dalloc(c, &B.mut_local, P.h);
-    rc |= dalloc(c, &B.mut_bsum, P.mut_blocks);
-    rc |= dalloc(c, &B.top, P.h);
-    rc |= dalloc(c, &B.bottom, P.h);
-    rc |= dalloc(c, &B.n_events, P.h);
-    rc |= dalloc(c, &B.events, (size_t)P.h * P.ev_cap);
-    rc |= dalloc(c, &B.by_parent, P.h);
-    rc |= dalloc(c, &B.parent_cnt, P.n);
+    rc |= dalloc(c, &B.mut_count, h_cap);
+    rc |= dalloc(c, &B.mut_local, h_cap);
+    rc |= dalloc(c, &B.mut_bsum, mut_blocks_cap);
+    rc |= dalloc(c, &B.top, h_cap);
+    rc |= dalloc(c, &B.bottom, h_cap);
+    rc |= dalloc(c, &B.n_events, h_cap);
+    rc |= dalloc(c, &B.events, (size_t)h_cap * P.ev_cap);
+    rc |= dalloc(c, &B.by_parent, h_cap);
+    rc |= dalloc(c, &B.parent_cnt, P.n_cap);
     rc |= dalloc(c, &B.parent_bsum, sel_blocks);
     rc |= dalloc(c, &B.var_part, (size_t)(P.max_chunks + 1) * P.w_cap);
     rc |= dalloc(c, &B.fix_part, (size_t)P.max_chunks * P.w_cap);
     rc |= dalloc(c, &B.word_cnt, P.w_cap);
     rc |= dalloc(c, &B.word_bsum, 2 * P.word_blocks);
-    rc |= dalloc(c, &B.cum_local, P.n);
+    rc |= dalloc(c, &B.cum_local, P.n_cap);
     rc |= dalloc(c, &B.cum_block, sel_blocks);
     rc |= dalloc(c, &B.fit_stats, 2 * sel_blocks);
     rc |= dalloc(c, &B.cdf_rho, kMaxCdf);
@@ -301,33 +335,32 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
               hipMemcpy(B.cdf_mu, cdf_mu, sizeof(double) * nm, hipMemcpyHostToDevice) ==
                   hipSuccess;
     for (int k = 0; k < 2 && ok; ++k)
-        ok = hipMemset(B.traits[k], 0, sizeof(int64_t) * P.h * T1) == hipSuccess;
+        ok = hipMemset(B.traits[k], 0, sizeof(int64_t) * h_cap * T1) == hipSuccess;
     ok = ok && hipMemset(B.keys, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
          hipMemset(B.effects, 0, sizeof(int32_t) * P.s_cap * T1) == hipSuccess &&
          hipMemset(B.ages, 0, sizeof(uint32_t) * P.s_cap) == hipSuccess &&
          hipMemset(B.fixed_bits, 0, sizeof(uint64_t) * P.w_cap) == hipSuccess &&
-         hipMemset(B.top, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
-         hipMemset(B.bottom, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
-         hipMemset(B.n_events, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
-         hipMemset(B.mut_count, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
-         hipMemset(B.mut_local, 0, sizeof(uint32_t) * P.h) == hipSuccess &&
-         hipMemset(B.mut_bsum, 0, sizeof(uint32_t) * P.mut_blocks) == hipSuccess;
+         hipMemset(B.top, 0, sizeof(uint32_t) * h_cap) == hipSuccess &&
+         hipMemset(B.bottom, 0, sizeof(uint32_t) * h_cap) == hipSuccess &&
+         hipMemset(B.n_events, 0, sizeof(uint32_t) * h_cap) == hipSuccess &&
+         hipMemset(B.mut_count, 0, sizeof(uint32_t) * h_cap) == hipSuccess &&
+         hipMemset(B.mut_local, 0, sizeof(uint32_t) * h_cap) == hipSuccess &&
+         hipMemset(B.mut_bsum, 0, sizeof(uint32_t) * mut_blocks_cap) == hipSuccess;
     if (ok) {
-        std::vector<double> ones(P.n, 1.0);
-        ok = hipMemcpy(B.fitness[0], ones.data(), 8 * P.n, hipMemcpyHostToDevice) == hipSuccess;
+        std::vector<double> ones(P.n_cap, 1.0);
+        ok = hipMemcpy(B.fitness[0], ones.data(), 8 * (size_t)P.n_cap, hipMemcpyHostToDevice) ==
+             hipSuccess;
     }
     if (ok) {  // child order and empty gametes until the first draw
-        std::vector<uint4> rec(P.h);
-        for (uint32_t o = 0; o < P.h; ++o) rec[o] = make_uint4(o, 0, 0, 0);
-        ok = hipMemcpy(B.by_parent, rec.data(), sizeof(uint4) * (size_t)P.h,
+        std::vector<uint4> rec(h_cap);
+        for (uint32_t o = 0; o < h_cap; ++o) rec[o] = make_uint4(o, 0, 0, 0);
+        ok = hipMemcpy(B.by_parent, rec.data(), sizeof(uint4) * (size_t)h_cap,
                        hipMemcpyHostToDevice) == hipSuccess;
     }
     GenState st;
     std::memset(&st, 0, sizeof(st));
     st.gen = 1;  // Engine::init leaves m_generation at 1 before the first simulate
-    st.n = P.n;
     st.fit_constant = 1;
-    st.sel_blocks = sel_blocks;
     ok = ok && hipMemcpy(B.state, &st, sizeof(st), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) return bail(CLOTHO_ERR_HIP);
     c->cur = 0;
@@ -595,6 +628,7 @@ int clotho_download_gametes(clotho_ctx* c, uint32_t* top, uint32_t* bottom, uint
 
 int clotho_phase_draw(clotho_ctx* c) {
     if (!c) return CLOTHO_ERR_ARG;
+    begin_generation(c);
     HIP_TRY(c, launch_draw(c));
     return CLOTHO_OK;
 }
@@ -622,6 +656,18 @@ int clotho_phase_commit(clotho_ctx* c) {
     if (!c) return CLOTHO_ERR_ARG;
     HIP_TRY(c, launch_commit(c));
     c->cur = 1 - c->cur;
+    end_generation(c);
+    return CLOTHO_OK;
+}
+
+int clotho_set_population_size(clotho_ctx* c, uint32_t n) {
+    if (!c) return CLOTHO_ERR_ARG;
+    if (n == 0) return fail(c, CLOTHO_ERR_ARG, "population size must be positive");
+    if (n > c->p.n_cap)
+        return fail(c, CLOTHO_ERR_CAPACITY,
+                    "population size " + std::to_string(n) + " exceeds the capacity of " +
+                        std::to_string(c->p.n_cap) + " individuals (max_individuals)");
+    c->next_n = n;
     return CLOTHO_OK;
 }
 int clotho_sample_rows(uint64_t seed, uint32_t generation, uint32_t replicate, uint32_t population,
@@ -737,7 +783,10 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
     for (double& v : tm.ms) v = 0.0;
     tm.generations = 0;
     std::vector<hipEvent_t> marks;
+    // a generation that changes the population size runs with eager launches
+    if (c->next_n && c->next_n != c->p.n) use_graph = 0;
     if (use_graph) {
+        end_generation(c);  // steady sizes: parent and offspring of equal size
         int rc = capture_graphs(c);
         if (rc) return rc;
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -769,9 +818,11 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
                 for (hipEvent_t& e : all.back()) HIP_TRY(c, hipEventCreate(&e));
                 mk = &all.back();
             }
+            begin_generation(c);
             int rc = enqueue_generation(c, mk);
             if (rc) return rc;
             c->cur = after_generation(c, c->cur);
+            end_generation(c);
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         if (tm.enabled) {

@@ -44,6 +44,11 @@ __device__ __forceinline__ uint32_t uniform_read(Ptr p) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)*p);
 }
 
+// Scan blocks over the parent population's fitness (selection set-up).
+__host__ __device__ inline uint32_t parent_blocks(const Params& P) {
+    return (P.n_par + kBlock - 1) / kBlock;
+}
+
 
 template <class Ptr>
 __device__ inline uint32_t poisson_draw(Ptr cdf, int n, double u) {
@@ -106,14 +111,14 @@ struct MutCounts {
         const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
         if (o < P.h) local[o] = inc - c;
         if (threadIdx.x == 0) bsum[blk] = total;
-        if (B.by_parent && o < P.n) gbl(B.parent_cnt)[o] = 0;  // counted by the gamete draws
     }
 };
 
-// Per-block max and all-equal flag of the parent fitness (selection set-up).
+// Per-block max and all-equal flag of the parent fitness (selection set-up);
+// also clears the per-parent child counts of the parent order.
 struct FitnessStats {
     static constexpr int kThreads = kBlock;
-    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __host__ __device__ static uint32_t blocks(const Params& P) { return parent_blocks(P); }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
@@ -125,10 +130,11 @@ struct FitnessStats {
         const double f0 = fit[0];
         double m = 0.0;
         int eq = 1;
-        if (i < P.n) {
+        if (i < P.n_par) {
             const double f = fit[i];
             m = f;
             eq = (f == f0) ? 1 : 0;
+            if (B.by_parent) gbl(B.parent_cnt)[i] = 0;  // counted by the gamete draws
         }
         smax[threadIdx.x] = m;
         seq[threadIdx.x] = eq;
@@ -190,7 +196,7 @@ struct Prepare {
         // selection: constant fitness -> uniform; else quantisation exponent
         double m = 0.0;
         int eq = 1;
-        for (uint32_t b = tid; b < st->sel_blocks; b += 1024) {
+        for (uint32_t b = tid; b < parent_blocks(P); b += 1024) {
             m = fmax(m, stats[2 * b]);
             eq &= stats[2 * b + 1] != 0.0 ? 1 : 0;
         }
@@ -224,7 +230,7 @@ struct Prepare {
             st->fit_constant = constant ? 1u : 0u;
             if (!constant) {
                 int L = 0;
-                while (((uint64_t)1 << L) < P.n) ++L;
+                while (((uint64_t)1 << L) < P.n_par) ++L;
                 st->sel_shift = 52 - L - (det_ilogb(mx) + 1);
             }
         }
@@ -281,7 +287,7 @@ struct MutAttrs {
 // scan; every partial sum is an integer below 2^52, exact in FP64.
 struct SelScan {
     static constexpr int kThreads = kBlock;
-    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __host__ __device__ static uint32_t blocks(const Params& P) { return parent_blocks(P); }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
@@ -294,13 +300,13 @@ struct SelScan {
         const uint32_t i = blk * kBlock + threadIdx.x;
         const int sh = st->sel_shift;
         double q = 0.0;
-        if (i < P.n) {
+        if (i < P.n_par) {
             const double f = fit[i];
             q = floor(det_ldexp(f > 0.0 ? f : 0.0, sh));
         }
         double total;
         const double inc = block_inclusive_scan<double>(q, lds, total);
-        if (i < P.n) cum_local[i] = inc;
+        if (i < P.n_par) cum_local[i] = inc;
         if (threadIdx.x == 0) cum_block[blk] = total;
     }
 };
@@ -318,7 +324,7 @@ struct SelBlockscan {
         __shared__ double s_carry;
         if (st->fit_constant) return;
         const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-        const uint32_t nb = st->sel_blocks;
+        const uint32_t nb = parent_blocks(P);
         if (tid == 0) s_carry = 0.0;
         __syncthreads();
         for (uint32_t base = 0; base < nb; base += 1024) {
@@ -349,10 +355,10 @@ struct SelBlockscan {
 template <class StPtr, class FPtr>
 __device__ inline uint32_t pick_parent(const Params& P, StPtr st, uint64_t k, FPtr cum_local,
                                        FPtr cum_block) {
-    if (st->fit_constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n);
+    if (st->fit_constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n_par);
     const double target = (double)__umul64hi(k << 11, st->sel_total);
     // first block whose inclusive total exceeds target
-    uint32_t lo = 0, hi = st->sel_blocks;
+    uint32_t lo = 0, hi = parent_blocks(P);
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (cum_block[mid] > target)
@@ -363,7 +369,7 @@ __device__ inline uint32_t pick_parent(const Params& P, StPtr st, uint64_t k, FP
     const uint32_t b = lo;
     const double base = b ? cum_block[b - 1] : 0.0;
     const double rel = target - base;
-    uint32_t a = b * kBlock, e = min((b + 1) * kBlock, P.n);
+    uint32_t a = b * kBlock, e = min((b + 1) * kBlock, P.n_par);
     while (a < e) {
         const uint32_t mid = (a + e) >> 1;
         if (cum_local[mid] > rel)
@@ -404,7 +410,7 @@ struct Gametes {
         top_out[o] = swap ? 2 * p + 1 : 2 * p;
         bot_out[o] = swap ? 2 * p : 2 * p + 1;
         nev_out[o] = ne;
-        if (B.by_parent && p < P.n)
+        if (B.by_parent && p < P.n_par)
             __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         const auto ev = events + (size_t)o * P.ev_cap;
@@ -422,17 +428,17 @@ struct Gametes {
 // the scatter's atomics and changes nothing in the output.
 struct ParentScan {
     static constexpr int kThreads = kBlock;
-    __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
+    __host__ __device__ static uint32_t blocks(const Params& P) { return parent_blocks(P); }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
         const auto cnt = gbl(B.parent_cnt);
         __shared__ uint32_t lds[kWaves];
         const uint32_t i = blk * kBlock + threadIdx.x;
-        const uint32_t c = i < P.n ? cnt[i] : 0u;
+        const uint32_t c = i < P.n_par ? cnt[i] : 0u;
         uint32_t total;
         const uint32_t inc = block_inclusive_scan<uint32_t>(c, lds, total);
-        if (i < P.n) cnt[i] = inc - c;
+        if (i < P.n_par) cnt[i] = inc - c;
         if (threadIdx.x == 0) gbl(B.parent_bsum)[blk] = total;
     }
 };
@@ -448,7 +454,7 @@ struct ParentBlockscan {
         __shared__ uint32_t s_wave[16];
         __shared__ uint32_t s_carry;
         const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-        const uint32_t nb = (P.n + kBlock - 1) / kBlock;
+        const uint32_t nb = parent_blocks(P);
         if (tid == 0) s_carry = 0;
         __syncthreads();
         for (uint32_t base = 0; base < nb; base += 1024) {
@@ -483,7 +489,7 @@ struct ParentScatter {
         const uint32_t o = blk * kBlock + threadIdx.x;
         if (o >= P.h) return;
         const uint32_t top = gbl(B.top)[o], p = top >> 1;
-        if (p >= P.n) return;
+        if (p >= P.n_par) return;
         const uint32_t pos = gbl(B.parent_bsum)[p / kBlock] +
                              __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);

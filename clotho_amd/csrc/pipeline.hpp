@@ -21,7 +21,6 @@ constexpr double kEffectLimit = 1073741823.0;  // 2^30 - 1
 // hipGraph replays successive generations without host involvement.
 struct GenState {
     uint32_t gen;          // stream generation counter (reference m_generation)
-    uint32_t n;            // individuals N
     uint32_t alleles;      // allele-table size of the parent population (A)
     uint32_t wp;           // parent words per row
     uint32_t free_count;   // F = fixed + lost of the parent
@@ -33,7 +32,6 @@ struct GenState {
     int32_t sel_shift;     // fitness quantisation exponent
     uint32_t error;        // sticky: 1 = allele capacity exceeded
     uint32_t error_gen;    // generation of the first error
-    uint32_t sel_blocks;   // number of fitness scan blocks
     uint64_t sel_total;    // sum of quantised fitness
     uint32_t kept;         // variable columns kept by the last compaction plan
 };
@@ -42,8 +40,12 @@ struct GenState {
 struct Params {
     uint64_t seed;
     uint32_t replicate;
-    uint32_t n;            // individuals
+    uint32_t n;            // individuals of the current population (the child
+                           // while a generation is produced)
     uint32_t h;            // haplotypes = 2n
+    uint32_t n_par;        // individuals of the parent population of the
+                           // generation being produced
+    uint32_t n_cap;        // individual capacity of the buffers
     uint32_t traits;       // T
     uint32_t s_cap;        // allele capacity
     uint32_t w_cap;        // word capacity per row
@@ -166,6 +168,7 @@ struct clotho_ctx {
     std::string err;
     hipDeviceProp_t prop;
     size_t device_bytes = 0;    // sum of the context's device allocations
+    uint32_t next_n = 0;        // offspring individuals of following generations (0: keep)
 };
 
 namespace clotho {
@@ -194,6 +197,8 @@ void destroy_graphs(clotho_ctx* c);
 int read_state(clotho_ctx* c, GenState* st);
 int write_state(clotho_ctx* c, const GenState& st);
 int capture_graphs(clotho_ctx* c);  // one generation per parity, on the context stream
+void begin_generation(clotho_ctx* c);  // parent = current, offspring at the requested size
+void end_generation(clotho_ctx* c);    // offspring population becomes current
 int after_generation(const clotho_ctx* c, int cur);  // population index after a generation
 // Batched replicates (all kernels of one generation for every replicate).
 void batch_op_blocks(const Params& P, uint32_t* out);  // kOpCount block counts

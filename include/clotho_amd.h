@@ -84,7 +84,8 @@ extern "C" {
 typedef struct clotho_config {
     uint64_t seed;            /* random_number.seed (Philox key)                         */
     uint32_t replicate;       /* replicate id, part of every stream counter              */
-    uint32_t n_individuals;   /* N, diploid individuals (constant population size)       */
+    uint32_t n_individuals;   /* N, diploid founder individuals (the population size
+                                 until clotho_set_population_size changes it)            */
     uint32_t trait_count;     /* traits.N                                                */
     uint32_t fitness_kind;    /* CLOTHO_FIT_*                                            */
     double mu;                /* mutation.mu, expected new mutations per gamete          */
@@ -100,7 +101,8 @@ typedef struct clotho_config {
     uint32_t compact_every;   /* 1: compact fixed and lost columns away after every
                                  generation (clotho_compact); 0: keep the reference's
                                  slot reuse                                              */
-    uint32_t reserved[3];
+    uint32_t max_individuals; /* individual capacity for population growth (0: N)        */
+    uint32_t reserved[2];
 } clotho_config;
 
 /* Shape and bookkeeping of the current (child) population after a generation. */
@@ -193,6 +195,16 @@ int clotho_device_bytes(const clotho_ctx* ctx, uint64_t* bytes);
  * replays a captured hipGraph of the generation instead of launching kernels. */
 int clotho_run(clotho_ctx* ctx, uint32_t n_generations, int use_graph);
 int clotho_generation(clotho_ctx* ctx);
+
+/* Individuals of the offspring population of every following generation: the
+ * pN of the reference's population-growth model, pN = growth(parent N,
+ * generation) (QS/engine_parallel_pipeline.hpp:181-197;
+ * src/clotho/genetics/linear_population_growth.cpp:23-25,
+ * src/clotho/genetics/predefined_population_growth.cpp:25-27).  Parents are
+ * drawn from the current population whatever its size.  n must not exceed the
+ * capacity max(n_individuals, max_individuals) (CLOTHO_ERR_CAPACITY).  A run
+ * whose first generation changes the size uses eager launches. */
+int clotho_set_population_size(clotho_ctx* ctx, uint32_t n);
 
 /* Phase-level entry points, in generation order (clotho_generation = these). */
 int clotho_phase_draw(clotho_ctx* ctx);

@@ -71,6 +71,8 @@ def lib():
                                         C.c_void_p]
     L.or_sim_set_population.restype = C.c_int
     L.or_sim_set_threads.argtypes = [C.c_void_p, C.c_int]
+    L.or_sim_set_population_size.argtypes = [C.c_void_p, C.c_uint32]
+    L.or_sim_set_population_size.restype = C.c_int
     L.or_sim_step.argtypes = [C.c_void_p]
     L.or_sim_step.restype = C.c_int
     L.or_sim_compact.argtypes = [C.c_void_p]
@@ -242,6 +244,10 @@ class OracleSim:
                                          effects.reshape(-1), positions.size, fp)
         if rc != 0:
             raise ValueError("oracle: bad population")
+
+    def set_population_size(self, n):
+        if lib().or_sim_set_population_size(self.h, int(n)) != 0:
+            raise ValueError("oracle: population size must be positive")
 
     def step(self):
         rc = lib().or_sim_step(self.h)

@@ -520,6 +520,7 @@ extern "C" double or_fitness_value(const clotho_config* cfg, uint32_t n, const d
 struct or_sim {
     clotho_config cfg;
     uint32_t N = 0, H = 0, T = 0;
+    uint32_t next_N = 0;  // offspring individuals of the following generations (0: N)
     uint32_t gen = 1;  // Engine::init increments m_generation to 1 before the first simulate
     std::vector<uint64_t> parent, child;
     uint32_t wp = 0, wc = 0;
@@ -701,9 +702,25 @@ extern "C" void or_allele_counts(const uint64_t* rows, uint32_t W, uint32_t A,
 
 extern "C" int or_sim_compact(or_sim* s);
 
+// Offspring population size of the following generations: the pN the
+// reference's population_growth model returns (pN = growth(parent N,
+// generation), examples/QTLSim/engine_parallel_pipeline.hpp:181-184).
+extern "C" int or_sim_set_population_size(or_sim* s, uint32_t n) {
+    if (!s || n == 0) return CLOTHO_ERR_ARG;
+    s->next_N = n;
+    return 0;
+}
+
 // Engine<parallel_pipeline>::simulate (examples/QTLSim/engine_parallel_pipeline.hpp:172-239).
 extern "C" int or_sim_step(or_sim* s) {
     const clotho_config& cfg = s->cfg;
+    // parent individuals, then the child population grows to pN (:181-197)
+    const uint32_t H_par = s->H;
+    const uint32_t N_par = s->N;
+    if (s->next_N) {
+        s->N = s->next_N;
+        s->H = 2 * s->N;
+    }
     const uint32_t N = s->N, H = s->H, T = s->T;
     auto c0 = std::chrono::steady_clock::now();
 
@@ -715,7 +732,7 @@ extern "C" int or_sim_step(or_sim* s) {
     //    DS/population_space/population_space_row_block_alignment.hpp:133-153)
     for (uint32_t j : s->fixed_list) {
         const uint64_t keep = ~((uint64_t)1 << (j % 64));
-        for (uint32_t r = 0; r < H; ++r) s->parent[(size_t)r * s->wp + j / 64] &= keep;
+        for (uint32_t r = 0; r < H_par; ++r) s->parent[(size_t)r * s->wp + j / 64] &= keep;
     }
     const uint32_t F = (uint32_t)(s->fixed_list.size() + s->lost_list.size());
     std::vector<uint32_t> free_slots(s->fixed_list);
@@ -776,7 +793,7 @@ extern "C" int or_sim_step(or_sim* s) {
 
     // 7: selection from the parent fitness
     Selector sel;
-    sel.build(s->fitness.data(), N, s->fitness_constant);
+    sel.build(s->fitness.data(), N_par, s->fitness_constant);
 
     s->top_row.assign(H, 0);
     s->bottom_row.assign(H, 0);

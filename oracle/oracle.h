@@ -64,6 +64,8 @@ int or_sim_set_population(or_sim* s, const uint64_t* rows, uint32_t words,
                           const double* positions, const double* effects, uint32_t alleles,
                           const double* fitness);
 void or_sim_set_threads(or_sim* s, int threads);
+/* Individuals of the offspring population of every following generation. */
+int or_sim_set_population_size(or_sim* s, uint32_t n);
 int or_sim_step(or_sim* s);
 /* Remove every fixed or lost column of the current population (slots keep
  * their order and are renumbered; the free lists become empty). */

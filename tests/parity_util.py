@@ -8,7 +8,7 @@ from clotho_amd import api
 
 def compare_state(sim, orc, where=""):
     gs, os_ = sim.shape(), orc.shape()
-    for key in ("generation", "haplotypes", "words", "alleles", "new_mutations", "fixed_count",
+    for key in ("generation", "n_individuals", "haplotypes", "words", "alleles", "new_mutations", "fixed_count",
                 "lost_count", "free_count", "variable_count", "all_neutral", "fitness_constant"):
         assert getattr(gs, key) == getattr(os_, key), f"{where}: shape.{key} {getattr(gs, key)} != {getattr(os_, key)}"
     assert gs.error == 0

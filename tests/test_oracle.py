@@ -459,3 +459,35 @@ def test_sample_rows_properties():
         hits[api.sample_rows(7, g, 0, 50, 10)] += 1
     # each row is drawn with probability 10/50 per sample: 80 expected of 400
     assert hits.min() > 45 and hits.max() < 120
+
+
+def test_oracle_population_size_changes():
+    """Population growth in the oracle: an explicit size equal to N changes
+    nothing; after a size change the child has 2 n rows and n fitness values,
+    its gametes draw parents from the previous population only, and bottom is
+    always the other haplotype of the top row's individual."""
+    cfg = api.make_config(100, 1.5, 2.0, seed=9, traits=1, neutral_p=0.5, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 13,
+                          max_individuals=300)
+    a, b = oracle.OracleSim(cfg), oracle.OracleSim(cfg)
+    b.set_population_size(100)
+    for _ in range(3):
+        a.step()
+        b.step()
+    assert np.array_equal(a.rows(), b.rows())
+    assert np.array_equal(a.fitness(), b.fitness())
+    n_par = 100
+    for n in (250, 40, 40, 1, 300):
+        a.set_population_size(n)
+        a.step()
+        s = a.shape()
+        assert s.n_individuals == n and s.haplotypes == 2 * n
+        top, bottom, _, _ = a.gametes()
+        assert top.size == 2 * n and int(top.max()) < 2 * n_par
+        assert np.array_equal(top ^ 1, bottom)
+        assert a.rows().shape[0] == 2 * n and a.fitness().size == n
+        # every slot of the allele table is fixed, lost or segregating in the child
+        assert s.free_count + s.variable_count == s.alleles
+        n_par = n
+    with pytest.raises(ValueError):
+        a.set_population_size(0)
