@@ -4,11 +4,20 @@
 // simulate() per generation, report free/variable/fixed counts and the
 // per-phase performance map.  Prints one JSON object.
 //
-//   engine_demo [--registry] [key=value ...] [--generations G] [--graph] [--stats K]
-//               [--log-prefix P]
+//   engine_demo [-c|--config FILE.json] [key=value ...] [--generations G] [--graph]
+//               [--stats K] [-p|--prefix|--log-prefix P] [--registry [--schedule K]]
+//               [--print-config]
 //
-// --registry lists the fitness_toolkit generators and the parsed
-// configuration without touching a GPU.
+// -c reads a JSON configuration the way the reference driver does
+// (QS/qtlsim_common.cpp:17-23, "configuration" block optional); key=value
+// pairs given after it override single keys (a value starting with '[' or
+// '{' is read as JSON).  --registry lists the fitness and population-growth
+// toolkits and the parsed sizes (--schedule K adds the population sizes of
+// the founders and generations 1..K) without touching a GPU; --print-config
+// prints the resolved configuration with every default and the toolkit
+// defaults, as the reference's driver does when run without a configuration
+// (QS/main.cpp:113-127).
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -18,6 +27,59 @@
 
 #include "clotho_amd/engine.hpp"
 #include "clotho_amd/log_writer.hpp"
+
+// The flat configuration as the reference writes it: nested objects, with
+// "<key>.<i>" elements gathered into arrays in index order.
+inline clotho_amd::json_tree config_json(const clotho_amd::property_tree& config,
+                                         const std::string& root) {
+    auto split = [](const std::string& k) {
+        std::vector<std::string> parts;
+        size_t b = 0;
+        while (true) {
+            const size_t e = k.find('.', b);
+            parts.push_back(k.substr(b, e == std::string::npos ? e : e - b));
+            if (e == std::string::npos) return parts;
+            b = e + 1;
+        }
+    };
+    auto numeric = [](const std::string& p) {
+        return !p.empty() && p.find_first_not_of("0123456789") == std::string::npos;
+    };
+    std::vector<std::vector<std::string>> keys;
+    std::vector<std::string> values;
+    for (const auto& kv : config.items()) {
+        keys.push_back(split(kv.first));
+        values.push_back(kv.second);
+    }
+    std::vector<size_t> order(keys.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const auto &x = keys[a], &y = keys[b];
+        for (size_t i = 0; i < std::min(x.size(), y.size()); ++i) {
+            if (x[i] == y[i]) continue;
+            if (numeric(x[i]) && numeric(y[i])) return std::stoul(x[i]) < std::stoul(y[i]);
+            return x[i] < y[i];
+        }
+        return x.size() < y.size();
+    });
+    clotho_amd::json_tree out;
+    for (size_t i : order) {
+        clotho_amd::json_tree* t = root.empty() ? &out : &out.child(root);
+        for (const std::string& p : keys[i]) t = numeric(p) ? &t->element(std::stoul(p)) : &t->child(p);
+        t->set_value(values[i]);
+    }
+    return out;
+}
+
+// Sample size of log.sampling_size entry i (qtl_logging_parameter,
+// examples/qtl/qtl_logging_parameter.hpp:38-62: an object's "size", default
+// 100, or a bare number); -1 past the last entry.  "log.sampling.<i>.size" is
+// accepted as well.
+inline double sample_size(const clotho_amd::property_tree& config, unsigned i) {
+    const std::string k = "log.sampling_size." + std::to_string(i);
+    if (config.has_child(k)) return config.has(k) ? config.get(k, 0.0) : config.get(k + ".size", 100.0);
+    return config.get("log.sampling." + std::to_string(i) + ".size", -1.0);
+}
 
 // The reference driver's generation loop with logging (QS/qtlsim_main.cpp:
 // 122-192): .config once, a .status record every log.period generations
@@ -34,9 +96,7 @@ void run_logged(Engine& engine, clotho_amd::property_tree& config, unsigned gene
                    t.time_since_epoch())
             .count();
     };
-    clotho_amd::json_tree cfg;
-    for (const auto& kv : config.items()) cfg.put(kv.first, kv.second);
-    clotho_amd::log_writer(prefix, ".config").write(cfg);
+    clotho_amd::log_writer(prefix, ".config").write(config_json(config, "configuration"));
     clotho_amd::log_writer status(prefix, ".status");
     const unsigned period0 = (unsigned)config.get("log.period", (double)generations);
     unsigned period = std::min(period0 ? period0 : generations, generations);
@@ -82,7 +142,7 @@ void run_logged(Engine& engine, clotho_amd::property_tree& config, unsigned gene
             all.evaluate(*pop);
             record(all, "allele_frequency");
             for (unsigned i = 0;; ++i) {
-                const double k = config.get("log.sampling." + std::to_string(i) + ".size", -1.0);
+                const double k = sample_size(config, i);
                 if (k < 0) break;
                 if (k == 0) continue;
                 clotho_amd::allele_frequency af;
@@ -138,7 +198,8 @@ int main(int argc, char** argv) {
     config.put("population_growth.params.B", 1000.0);
     config.put("alleles.capacity", 32768.0);
     unsigned generations = 10;
-    bool graph = false, registry = false;
+    bool graph = false, registry = false, print_config = false;
+    long schedule = -1;  // --schedule K with --registry
     long stats_sample = -1;  // --stats K: allele-frequency statistics over K rows (0 = all)
     std::string log_prefix;  // --log-prefix P: the reference's .config/.status/.performance files
     bool logging = false;
@@ -146,9 +207,17 @@ int main(int argc, char** argv) {
         const std::string a = argv[i];
         if (a == "--registry") {
             registry = true;
+        } else if (a == "--print-config") {
+            print_config = true;
+        } else if (a == "--schedule" && i + 1 < argc) {
+            schedule = std::stol(argv[++i]);
         } else if (a == "--graph") {
             graph = true;
-        } else if (a == "--log-prefix" && i + 1 < argc) {
+        } else if ((a == "-c" || a == "--config") && i + 1 < argc) {
+            clotho_amd::read_json(argv[++i], config);
+            if (config.has("generations.size"))
+                generations = (unsigned)config.get("generations.size", (double)generations);
+        } else if ((a == "--log-prefix" || a == "-p" || a == "--prefix") && i + 1 < argc) {
             log_prefix = argv[++i];
             logging = true;
         } else if (a == "--stats" && i + 1 < argc) {
@@ -156,18 +225,50 @@ int main(int argc, char** argv) {
         } else if (a == "--generations" && i + 1 < argc) {
             generations = (unsigned)std::stoul(argv[++i]);
         } else if (a.find('=') != std::string::npos) {
-            config.put(a.substr(0, a.find('=')), a.substr(a.find('=') + 1));
+            const std::string key = a.substr(0, a.find('=')), value = a.substr(a.find('=') + 1);
+            if (!value.empty() && (value[0] == '[' || value[0] == '{')) {
+                config.erase(key);
+                clotho_amd::read_json_text(value, config, key);
+            } else {
+                config.put(key, value);
+            }
+            if (key == "generations.size" || key == "configuration.generations.size")
+                generations = (unsigned)config.get("generations.size", (double)generations);
         }
+    }
+    config.put("generations.size", (double)generations);
+    if (print_config) {
+        (void)clotho_amd::make_config(config);  // completes every parameter block
+        clotho_amd::fitness_toolkit::getInstance()->tool_configurations(config);
+        clotho_amd::population_growth_toolkit::getInstance()->tool_configurations(config);
+        clotho_amd::log_writer("", "").write(config_json(config, "configuration"));
+        return 0;
     }
     if (registry) {
         std::printf("{\"fitness_tools\": [");
         const auto names = clotho_amd::fitness_toolkit::getInstance()->tool_names();
         for (size_t i = 0; i < names.size(); ++i)
             std::printf("%s\"%s\"", i ? ", " : "", names[i].c_str());
+        std::printf("], \"population_growth_tools\": [");
+        const auto gnames = clotho_amd::population_growth_toolkit::getInstance()->names();
+        for (size_t i = 0; i < gnames.size(); ++i)
+            std::printf("%s\"%s\"", i ? ", " : "", gnames[i].c_str());
         const clotho_config c = clotho_amd::make_config(config);
-        std::printf("], \"n\": %u, \"mu\": %.17g, \"rho\": %.17g, \"fitness_kind\": %u, "
-                    "\"seed\": %llu}\n",
-                    c.n_individuals, c.mu, c.rho, c.fitness_kind, (unsigned long long)c.seed);
+        std::printf("], \"n\": %u, \"max_individuals\": %u, \"mu\": %.17g, \"rho\": %.17g, "
+                    "\"fitness_kind\": %u, \"seed\": %llu",
+                    c.n_individuals, c.max_individuals, c.mu, c.rho, c.fitness_kind,
+                    (unsigned long long)c.seed);
+        if (schedule >= 0) {
+            auto growth = clotho_amd::population_growth_toolkit::getInstance()->get_tool(config);
+            std::printf(", \"schedule\": [");
+            if (growth) {
+                const auto sizes = clotho_amd::growth_schedule(*growth, (uint32_t)schedule);
+                for (size_t i = 0; i < sizes.size(); ++i)
+                    std::printf("%s%u", i ? ", " : "", sizes[i]);
+            }
+            std::printf("]");
+        }
+        std::printf("}\n");
         return 0;
     }
     std::mt19937 rng(0);

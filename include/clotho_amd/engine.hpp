@@ -5,7 +5,10 @@
 // Reference families mirrored (DS/ = include/clotho/data_spaces/,
 // QS/ = examples/QTLSim/):
 //   property_tree          boost::property_tree::ptree keys used by the path
-//                          (src/clotho/utility/clotho_strings.cpp:16-47)
+//                          (src/clotho/utility/clotho_strings.cpp:16-47), in config.hpp
+//   population_growth_toolkit, ipopulation_growth
+//                          src/clotho/genetics/population_growth_toolkit.cpp:36-62,
+//                          in population_growth.hpp
 //   ifitness, ifitness_generator, fitness_toolkit
 //                          include/clotho/fitness/ifitness.hpp:21-38,
 //                          include/clotho/fitness/ifitness_generator.hpp:22-44,
@@ -40,40 +43,10 @@
 #include <vector>
 
 #include "../clotho_amd.h"
+#include "config.hpp"
+#include "population_growth.hpp"
 
 namespace clotho_amd {
-
-// ---------------------------------------------------------------------------
-// configuration: the dotted keys the reference reads from its JSON property
-// tree ("configuration." prefix optional), with its defaults.
-// ---------------------------------------------------------------------------
-class property_tree {
-public:
-    void put(const std::string& key, const std::string& value) { m_kv[strip(key)] = value; }
-    void put(const std::string& key, double value) {
-        std::ostringstream os;
-        os.precision(17);
-        os << value;
-        m_kv[strip(key)] = os.str();
-    }
-    bool has(const std::string& key) const { return m_kv.count(strip(key)) != 0; }
-    std::string get(const std::string& key, const std::string& dflt) const {
-        auto it = m_kv.find(strip(key));
-        return it == m_kv.end() ? dflt : it->second;
-    }
-    double get(const std::string& key, double dflt) const {
-        auto it = m_kv.find(strip(key));
-        return it == m_kv.end() ? dflt : std::stod(it->second);
-    }
-    const std::map<std::string, std::string>& items() const { return m_kv; }
-
-private:
-    static std::string strip(const std::string& k) {
-        const std::string p = "configuration.";
-        return k.compare(0, p.size(), p) == 0 ? k.substr(p.size()) : k;
-    }
-    std::map<std::string, std::string> m_kv;
-};
 
 inline void check(int rc, const clotho_ctx* ctx, const char* what) {
     if (rc != CLOTHO_OK) {
@@ -156,11 +129,33 @@ public:
         return &inst;
     }
     void register_tool(std::shared_ptr<ifitness_generator> gen) { m_tools[gen->name()] = gen; }
+    // The generator named by fitness.name; an absent block is written back
+    // empty (src/clotho/fitness/fitness_toolkit.cpp:50-69).
     std::shared_ptr<ifitness_generator> get_tool(property_tree& config) const {
+        if (!config.has_child("fitness")) {
+            config.put("fitness.name", std::string(""));
+            config.put("fitness.params", std::string(""));
+            return nullptr;
+        }
         const std::string n = config.get("fitness.name", std::string(""));
         auto it = m_tools.find(n);
         if (it == m_tools.end()) return nullptr;
         return it->second->create(config);
+    }
+    // fitness.toolkit.<name>.{name, params, depends}: every generator with the
+    // parameters it reads and the keys of other blocks it depends on
+    // (fitness_toolkit::tool_configurations, src/clotho/fitness/fitness_toolkit.cpp:20-48)
+    void tool_configurations(property_tree& config) const {
+        for (const auto& kv : m_tools) {
+            const std::string p = "fitness.toolkit." + kv.first;
+            config.put(p + ".name", kv.first);
+            const uint32_t k = kv.second->device_kind();
+            if (k == CLOTHO_FIT_QUADRATIC || k == CLOTHO_FIT_GAUSSIAN_FIXED ||
+                k == CLOTHO_FIT_MV_STABILIZING)
+                config.put(p + ".params.scale", 1.0);
+            if (k == CLOTHO_FIT_GAUSSIAN || k == CLOTHO_FIT_QUADRATIC)
+                config.put(p + ".depends.mutation.mu", 1e-4);
+        }
     }
     std::vector<std::string> tool_names() const {
         std::vector<std::string> v;
@@ -228,21 +223,33 @@ private:
 // ---------------------------------------------------------------------------
 // device population: owns the C ABI context
 // ---------------------------------------------------------------------------
-// Population size from population_growth: the reference starts from
-// pN = growth(0, gen) (QS/engine_parallel_pipeline.hpp:150-153); "linear" is
-// pN' = A pN + B (src/clotho/genetics/linear_population_growth.cpp:23-25), so
-// A = 0 gives the constant size B, the only schedule the device pipeline runs.
-inline uint32_t population_size(const property_tree& config) {
-    if (config.has("population_growth.size"))
-        return (uint32_t)config.get("population_growth.size", 0.0);
-    const std::string name = config.get("population_growth.name", std::string("linear"));
-    if (name != "linear")
-        throw std::runtime_error("population_growth." + name + " is not supported (constant N only)");
-    const double A = config.get("population_growth.params.A", config.get("population_growth.A", 1.0));
-    const double B = config.get("population_growth.params.B", config.get("population_growth.B", 0.0));
-    if (A != 0.0 || !(B >= 1.0))
-        throw std::runtime_error("population_growth.linear must have A = 0 and B >= 1 (constant N)");
-    return (uint32_t)B;
+// Founder size and individual capacity from the population-growth model:
+// the reference starts from pN = growth(0, 0) (QS/engine_parallel_pipeline.hpp:150-153).
+// The capacity covers the model's sizes over generations.size generations
+// (or population_growth.capacity when given).  Without a model the size is
+// population_growth.size, constant.
+struct population_sizes {
+    uint32_t founders = 0, capacity = 0;
+};
+
+inline population_sizes population_size(property_tree& config) {
+    population_sizes out;
+    auto growth = population_growth_toolkit::getInstance()->get_tool(config);
+    if (growth) {
+        const uint32_t horizon = (uint32_t)config.get("generations.size", 10000.0);
+        const std::vector<uint32_t> sizes = growth_schedule(*growth, horizon);
+        out.founders = sizes[0];
+        for (uint32_t v : sizes) out.capacity = std::max(out.capacity, v);
+    } else {
+        out.founders = (uint32_t)config.get("population_growth.size", 0.0);
+        out.capacity = out.founders;
+    }
+    if (config.has("population_growth.capacity"))
+        out.capacity = std::max(out.founders, (uint32_t)config.get("population_growth.capacity", 0.0));
+    if (out.founders == 0)
+        throw std::runtime_error(
+            "population_growth must give a positive founder size (e.g. linear with B >= 1)");
+    return out;
 }
 
 inline int device_of(const property_tree& config) { return (int)config.get("device", 0.0); }
@@ -251,7 +258,9 @@ inline clotho_config make_config(property_tree& config) {
     clotho_config c{};
     c.seed = (uint64_t)config.get("random_number.seed", 0.0);
     c.replicate = (uint32_t)config.get("replicate", 0.0);
-    c.n_individuals = population_size(config);
+    const population_sizes ps = population_size(config);
+    c.n_individuals = ps.founders;
+    c.max_individuals = ps.capacity;
     c.trait_count = (uint32_t)config.get("traits.N", 1.0);
     c.mu = config.get("mutation.mu", 1e-4);
     c.rho = config.get("recombination.rho", 1e-4);
@@ -543,11 +552,20 @@ public:
           m_free(1),
           m_xover(rng, &m_alleles, config.get("recombination.rho", 1e-4),
                   config.get("recombination.sequence_bias.p", 0.5)),
+          m_growth(population_growth_toolkit::getInstance()->get_tool(config)),
           m_generation(1) {}
+
+    // pN of the coming generation: growth(parent N, generation)
+    // (QS/engine_parallel_pipeline.hpp:181-184); the parent size without a model
+    uint32_t next_population_size() {
+        const uint32_t parent = m_pop.getIndividualCount();
+        return m_growth ? (*m_growth)(parent, (uint32_t)m_generation) : parent;
+    }
 
     // one generation: draw, recombine+mutate, traits+fitness, free space, commit
     void simulate() {
         clotho_ctx* c = m_pop.ctx();
+        check(clotho_set_population_size(c, next_population_size()), c, "set_population_size");
         check(clotho_phase_draw(c), c, "phase_draw");
         m_xover(m_pop);
         m_fit(m_pop);
@@ -555,10 +573,25 @@ public:
         check(clotho_phase_commit(c), c, "phase_commit");
         ++m_generation;
     }
-    // many generations without host round trips (hipGraph replay when asked)
+    // many generations without host round trips (hipGraph replay when asked);
+    // a growth model splits them into runs of equal population size
     void simulate(unsigned int generations, bool graph) {
-        check(clotho_run(m_pop.ctx(), generations, graph ? 1 : 0), m_pop.ctx(), "clotho_run");
-        m_generation += generations;
+        clotho_ctx* c = m_pop.ctx();
+        uint32_t parent = m_pop.getIndividualCount();
+        while (generations) {
+            const uint32_t n =
+                m_growth ? (*m_growth)(parent, (uint32_t)m_generation) : parent;
+            unsigned int run = 1;
+            while (m_growth && run < generations &&
+                   (*m_growth)(n, (uint32_t)(m_generation + run)) == n)
+                ++run;
+            if (!m_growth) run = generations;
+            check(clotho_set_population_size(c, n), c, "set_population_size");
+            check(clotho_run(c, run, graph ? 1 : 0), c, "clotho_run");
+            m_generation += run;
+            generations -= run;
+            parent = n;
+        }
     }
     // pick the recombination launch shape for the resident population
     unsigned int autotune() {
@@ -602,6 +635,7 @@ private:
     GeneralFitness m_fit;
     free_space_type m_free;
     crossover_type m_xover;
+    std::shared_ptr<ipopulation_growth> m_growth;
     size_t m_generation;
 };
 

@@ -47,6 +47,19 @@ public:
         put(path, number(v));
     }
     void put_child(const std::string& path, const json_tree& t) { at(path) = t; }
+    // element i of this node as an array (""-keyed children), created on demand
+    json_tree& element(size_t i) {
+        while (m_children.size() <= i) m_children.emplace_back("", json_tree());
+        return m_children[i].second;
+    }
+    // key of this node's child, created on demand (no dotted-path splitting)
+    json_tree& child(const std::string& key) {
+        for (auto& kv : m_children)
+            if (kv.first == key) return kv.second;
+        m_children.emplace_back(key, json_tree());
+        return m_children.back().second;
+    }
+    void set_value(const std::string& v) { m_value = v; }
     void push_back(const json_tree& t) { m_children.emplace_back("", t); }
     template <class T>
     void push_value(T v) {

@@ -49,7 +49,8 @@ def test_engine_demo_log_files(tmp_path):
             "recombination.rho=1.5", "random_number.seed=4", "log.period=3",
             "log.sampling.0.size=100", "--generations", "6", "--log-prefix", prefix]
     subprocess.check_call(args, stdout=subprocess.DEVNULL)
-    cfg_log = json.load(open(prefix + ".0.config.json"))
+    # the configuration block as the reference driver writes it (QS/qtlsim_main.cpp:120-121)
+    cfg_log = json.load(open(prefix + ".0.config.json"))["configuration"]
     assert cfg_log["mutation"]["mu"] == "2" and cfg_log["log"]["period"] == "3"
     cfg = api.make_config(300, 2.0, 1.5, seed=4, traits=1, neutral_p=1.0, max_alleles=32768)
     orc = oracle.OracleSim(cfg)
@@ -82,3 +83,33 @@ def test_engine_demo_log_files(tmp_path):
     assert len(perf["performance"]["simulate"]) == 6
     assert len(perf["performance"]["stats"]) == 2
     assert [int(x) for x in perf["memory"]["free_count"]] == free_counts
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_engine_demo_population_growth(tmp_path, graph):
+    """A JSON configuration with predefined population growth through the C++
+    driver: each generation's offspring count is sizes[g mod 4]
+    (predefined_population_growth) and the run matches the oracle driven with
+    the same sizes.  With --graph the runs of equal size replay graphs."""
+    sizes = [300, 500, 200, 200]
+    conf = {"configuration": {
+        "mutation": {"mu": "2"}, "recombination": {"rho": "1.5"},
+        "random_number": {"seed": "6"}, "neutral": {"p": "1"},
+        "fitness": {"name": "constant"}, "alleles": {"capacity": "32768"},
+        "population_growth": {"name": "predefined", "params": {"sizes": [str(v) for v in sizes]}},
+        "generations": {"size": "6"}}}
+    path = tmp_path / "growth.json"
+    path.write_text(json.dumps(conf))
+    args = [build.DEMO, "-c", str(path)] + (["--graph"] if graph else [])
+    out = json.loads(subprocess.run(args, check=True, capture_output=True, text=True).stdout)
+    cfg = api.make_config(300, 2.0, 1.5, seed=6, traits=1, neutral_p=1.0, max_alleles=32768,
+                          max_individuals=500)
+    orc = oracle.OracleSim(cfg)
+    for g in range(1, 7):
+        orc.set_population_size(sizes[g % 4])
+        orc.step()
+    s = orc.shape()
+    assert s.n_individuals == 200
+    assert out["generation"] == s.generation and out["alleles"] == s.alleles
+    assert out["free"] == s.free_count and out["fixed"] == s.fixed_count
+    assert out["rows_fnv1a"] == fnv1a(orc.rows())

@@ -65,6 +65,7 @@ def test_engine_adapter_registry_and_config():
     assert set(d["fitness_tools"]) == {"constant", "gaussian", "quadratic", "gaussian_fixed",
                                        "mv_stabilizing"}
     assert d["n"] == 250 and d["mu"] == 0.5 and d["fitness_kind"] == api.FIT_QUADRATIC
-    r = subprocess.run([demo, "--registry", "population_growth.params.A=1.1"],
+    # a model without a positive founder size is rejected
+    r = subprocess.run([demo, "--registry", "population_growth.params.B=0"],
                        capture_output=True, text=True)
-    assert r.returncode != 0 and "constant N" in r.stderr
+    assert r.returncode != 0 and "founder size" in r.stderr
