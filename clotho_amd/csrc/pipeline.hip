@@ -572,18 +572,22 @@ struct WordSort {
 //   child = (t & ~mask) | (b & mask), then XOR of this row's new mutations
 // OR/AND of the child words accumulate in registers (free_space_accumulator).
 //
-// Latency structure: a wave owns rows r0 + wave + kRecWaves j.  The gamete
-// records of 64 such rows (parents, event and mutation counts, mutation
-// offset) are loaded with one vector load each, lane q holding row j0 + q, and
-// read back with v_readlane, so the row loop issues no scalar loads.  Even and
-// odd rows use separate register sets: the loads of row q + 2 are issued as
-// soon as row q is done and stay in flight while row q + 1 is classified (no
-// register rotation, hence no wait on loads still in flight; three or four
-// sets measured no faster on the same device).  Lane e holds
-// event key e.  Events are classified kEventGroup at a time so that their
-// LDS lookups overlap; a group is padded with rank-64 entries (pm[64] is all
-// ones), and since #real + #padding = kEventGroup is even, the XOR of the
-// group's prefix masks equals the XOR of the complements of the real events.
+// Latency structure: with parent-ordered gamete records a wave owns a
+// contiguous segment of positions (without them, positions c0 + wave +
+// kRecWaves j).  The records of 64 positions (child row, top row, event and
+// mutation counts, mutation offset) are loaded with one vector load, lane q
+// holding position j0 + q, and read back with v_readlane, so the row loop
+// issues no scalar loads.  Runs of positions with one parent individual are
+// cut into items of at most kSibs children: an item loads the parent pair
+// once and the event keys of each child (lane e holds key e).  Items
+// alternate between two register sets: the loads of item k + 2 are issued as
+// soon as item k is done and stay in flight while item k + 1 is classified
+// (no register rotation, hence no wait on loads still in flight; three or
+// four sets measured no faster on the same device).  Events are classified
+// kEventGroup at a time so that their LDS lookups overlap; a group is padded
+// with rank-64 entries (pm[64] is all ones), and since #real + #padding =
+// kEventGroup is even, the XOR of the group's prefix masks equals the XOR of
+// the complements of the real events.
 constexpr int kRecWaves = 16;
 constexpr int kRecBlock = 64 * kRecWaves;
 // events classified together; any even size keeps the padding parity
@@ -743,10 +747,13 @@ struct Recombine {
                 }
             };
 
-            auto child_row = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi) {
+            // A child whose top row is the item's bottom row takes the
+            // complemented mask: (b & ~m) | (t & m) = (t & ~~m) | (b & ~m).
+            auto child_row = [&](uint32_t q, uint64_t t, uint64_t b, uint32_t lo, uint32_t hi,
+                                 bool flip) {
                 const uint32_t nx = lane_read(g_nx, q);
                 const uint32_t ne = nx & 0xFFu;
-                uint64_t m = 0;
+                uint64_t m = flip ? ~(uint64_t)0 : 0;
                 for (uint32_t e = 0; e < ne; e += kEventGroup) {
                     uint32_t key[kEventGroup], p[kEventGroup];
                     bool more[kEventGroup];
@@ -822,8 +829,7 @@ struct Recombine {
                     if ((uint32_t)k < it.len) {
                         const uint32_t q = it.q + k;
                         const bool flip = lane_read(g_top, q) != top0;
-                        child_row(q, flip ? b0 : t0, flip ? t0 : b0, it.lo[k],
-                                  kWideEvents ? it.hi[k] : 0u);
+                        child_row(q, t0, b0, it.lo[k], kWideEvents ? it.hi[k] : 0u, flip);
                     }
                 }
             };
