@@ -611,6 +611,158 @@ __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
 // that drops the lanes past the row), so the number of vector memory
 // operations in flight is the same on every path and the waits before a row's
 // compute cover only that row's loads.
+// Narrow rows (W <= 32 words, child order, at most 64 events per gamete):
+// a wave produces 64 / FW rows per step, lane = (row f, word lane mod FW), so
+// a replicate of 8-word rows keeps every lane busy instead of one in eight.
+// The LDS tables hold the tile's FW words repeated across the 64 columns, so
+// lane l still reads its own bank column.  For events j FW .. j FW + FW - 1
+// each lane holds key j FW + (l mod FW) of its row (the first FW keys are
+// loaded with the parent words, later ones only when some row of the step
+// has that many events) and takes key x of its row from lane
+// f FW + x mod FW; rows of a step share the event loop up to the largest
+// event count, lookups past a row's own count take rank 64 (pm[64] = all
+// ones), and since every lane does the same even number of lookups the XOR
+// of the prefix masks is still the XOR of the complements of the real events.
+template <uint32_t FW>
+__device__ __forceinline__ void recombine_folded(const Params& P, const Buffers& B, uint32_t sel,
+                                                 uint32_t by, uint32_t* s_bucket,
+                                                 uint32_t* s_sorted, uint64_t* s_pm) {
+    constexpr uint32_t NF = 64 / FW;   // rows per wave step
+    const auto st = gbl(B.state);
+    const auto parent = gbl(B.rows[sel]);
+    const auto child = gbl(B.rows[1 - sel]);
+    const auto sorted = gbl(B.sorted);
+    const auto pm = gbl(B.pm);
+    const auto bucket = gbl(B.bucket);
+    const auto free_list = gbl(B.free_list);
+    const auto fixed_bits = gbl(B.fixed_bits);
+    const auto top_in = gbl(B.top);
+    const auto nev_in = gbl(B.n_events);
+    const auto events = gbl(B.events);
+    const auto count = gbl(B.mut_count);
+    const auto local = gbl(B.mut_local);
+    const auto boff = gbl(B.mut_bsum);
+    const uint32_t wc = uniform_read(&st->wc), wp = uniform_read(&st->wp);
+    const uint32_t A = uniform_read(&st->alleles), F = uniform_read(&st->free_count),
+                   S = uniform_read(&st->all_size);
+    const uint32_t ev_cap = P.ev_cap;  // <= 64 on this path
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t f = lane / FW, w = lane % FW;
+    const bool valid = w < wc;
+    const bool inpar = valid && w < wp;
+    const uint32_t col = wp ? min(w, wp - 1) : 0;
+
+    for (uint32_t k = wave; k < 68; k += kRecWaves)
+        s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
+    for (uint32_t q = wave; q < 64; q += kRecWaves)
+        s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
+    for (uint32_t c = wave; c < 65; c += kRecWaves)
+        s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
+    const uint64_t keep = inpar ? ~fixed_bits[w] : 0;
+    __syncthreads();
+
+    const uint32_t rows_per_chunk = (P.h + P.chunks - 1) / P.chunks;
+    const uint32_t c0 = min(P.h, by * rows_per_chunk);
+    const uint32_t c1 = min(P.h, c0 + rows_per_chunk);
+    const uint32_t nsteps = (c1 - c0 + NF - 1) / NF;
+    uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
+
+    auto wave_max = [](uint32_t v) {
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, (int)d, 64));
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+    };
+
+    for (uint32_t step = wave; step < nsteps; step += kRecWaves) {
+        const uint32_t row = c0 + step * NF + f;
+        const bool live = row < c1;
+        const uint32_t r = live ? row : c0;  // in bounds (c0 < c1 here)
+        const uint32_t top = top_in[r];
+        const uint32_t ne = live ? nev_in[r] : 0u;
+        const uint32_t nm = live ? count[r] : 0u;
+        const uint32_t k0 = local[r] + boff[r / kBlock];
+        const uint64_t t = parent[(size_t)top * wp + col] & keep;
+        const uint64_t b = parent[(size_t)(top ^ 1u) * wp + col] & keep;
+        const auto ev = events + (size_t)r * ev_cap;
+        uint32_t kv = ev[min(w, ev_cap - 1)];
+
+        const uint32_t max_ne = wave_max(ne);
+        uint64_t m = 0;
+        for (uint32_t j = 0; j * FW < max_ne; ++j) {
+            if (j) kv = ev[min(j * FW + w, ev_cap - 1)];
+            for (uint32_t x = 0; x < FW && j * FW + x < max_ne; x += kEventGroup) {
+                uint32_t key[kEventGroup], p[kEventGroup];
+                bool more[kEventGroup];
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i)
+                    key[i] = (uint32_t)__shfl((int)kv, (int)(f * FW + ((x + i) & (FW - 1))), 64);
+                uint32_t tb[kEventGroup];
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                uint32_t sk[kEventGroup][4];
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sk[i][q] = s_sorted[(p[i] + q) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i) {
+                    const bool real = j * FW + x + i < ne;
+                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u) +
+                                       (sk[i][2] < key[i] ? 1u : 0u);
+                    more[i] = real && sk[i][3] < key[i];
+                    p[i] = real ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
+                }
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i)
+                    if (more[i])
+                        while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
+                uint64_t xm = 0;
+#pragma unroll
+                for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
+                m ^= xm;
+            }
+        }
+        uint64_t c = (t & ~m) | (b & m);
+        const uint32_t max_nm = wave_max(nm);
+        for (uint32_t k = 0; k < max_nm; ++k) {
+            if (k < nm) {
+                const uint32_t kk = k0 + k;
+                const uint32_t slot = kk < F ? free_list[kk] : A + (kk - F);
+                if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+            }
+        }
+        if (live && valid) child[(size_t)row * wc + w] = c;
+        if (live) {
+            acc_or |= c;
+            acc_and &= c;
+        }
+    }
+    // rows of one word live in lanes w, w + FW, ...: fold them, then the waves
+#pragma unroll
+    for (uint32_t d = FW; d < 64; d <<= 1) {
+        acc_or |= (uint64_t)__shfl_xor((long long)acc_or, (int)d, 64);
+        acc_and &= (uint64_t)__shfl_xor((long long)acc_and, (int)d, 64);
+    }
+    __syncthreads();  // every wave is past its last prefix-mask read
+    uint64_t* s_or = s_pm;
+    uint64_t* s_and = s_pm + kRecWaves * 64;
+    s_or[wave * 64 + lane] = acc_or;
+    s_and[wave * 64 + lane] = acc_and;
+    __syncthreads();
+    if (wave == 0 && lane < FW && valid) {
+        uint64_t vo = s_or[lane], va = s_and[lane];
+        for (int q = 1; q < kRecWaves; ++q) {
+            vo |= s_or[q * 64 + lane];
+            va &= s_and[q * 64 + lane];
+        }
+        gbl(B.var_part)[(size_t)by * P.w_cap + w] = vo;
+        gbl(B.fix_part)[(size_t)by * P.w_cap + w] = va;
+    }
+}
+
 template <bool kWideEvents, int kSibs>
 struct Recombine {
     static constexpr int kThreads = kRecBlock;
@@ -644,6 +796,12 @@ struct Recombine {
         const uint32_t wc = uniform_read(&st->wc);
         const uint32_t w0 = bx * 64;
         if (w0 >= wc) return;  // uniform for the block
+        if constexpr (kSibs == 1 && !kWideEvents) {
+            // narrow rows: several rows per wave (tile 0 is the only live tile)
+            if (wc <= 8) return recombine_folded<8>(P, B, sel, by, s_bucket, s_sorted, s_pm);
+            if (wc <= 16) return recombine_folded<16>(P, B, sel, by, s_bucket, s_sorted, s_pm);
+            if (wc <= 32) return recombine_folded<32>(P, B, sel, by, s_bucket, s_sorted, s_pm);
+        }
         const uint32_t wp = uniform_read(&st->wp);
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1510,18 +1668,27 @@ CLOTHO_SINGLE(k_commit, Commit)
 // Occupancy floor of a batched kernel: the recombination kernel needs two
 // resident 1024-thread blocks per CU (8 waves per SIMD, <= 64 VGPRs) to keep
 // enough parent loads in flight.
+constexpr uint32_t kBatchOffsLds = 1024;  // replicate offsets kept in LDS up to this R
+
 template <class Op>
 __global__ void __launch_bounds__(Op::kThreads)
     __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value))) k_batched(const RepView* __restrict__ views,
                                                           const uint32_t* __restrict__ offs,
                                                           uint32_t R, uint32_t sel) {
+    // the replicate offsets in LDS when they fit: every virtual block
+    // searches them, and a search in global memory is a chain of dependent loads
+    __shared__ uint32_t s_offs[kBatchOffsLds + 1];
+    const bool in_lds = R <= kBatchOffsLds;
+    if (in_lds)
+        for (uint32_t i = threadIdx.x; i <= R; i += Op::kThreads) s_offs[i] = offs[i];
+    __syncthreads();
     const uint32_t total = offs[R];
     for (uint32_t v = blockIdx.x; v < total; v += gridDim.x) {
         // replicate: the largest r < R with offs[r] <= v (empty ranges are skipped)
         uint32_t lo = 0, hi = R;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (offs[mid] <= v)
+            if ((in_lds ? s_offs[mid] : offs[mid]) <= v)
                 lo = mid;
             else
                 hi = mid;
@@ -1655,7 +1822,12 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
     const uint32_t total = L.total[op];
     if (!total) return hipSuccess;
     const uint32_t per_cu = Op::kThreads >= 1024 ? 2u : 8u;
-    const uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
+    // An odd grid: virtual block v runs on block v mod grid, and replicate
+    // ranges are multiples of the column-tile count (a power of two), so an
+    // even grid would hand every live tile-0 block to the same quarter of
+    // the grid while the rest only meet tiles past the live width.
+    uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
+    if (grid > 1 && !(grid & 1u)) --grid;
     hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::kThreads), 0, s, L.views,
                        L.offs + (size_t)op * (L.R + 1), L.R, sel);
     return hipGetLastError();
