@@ -82,20 +82,23 @@ __device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff_in
 }
 
 // Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
-// With one N tile (T <= 4) the block stages each group's B fragments (16 KB)
-// in LDS, double buffered, so the four waves share one copy; the next
-// group's rows are loaded before the current group is multiplied.  Bits
-// expand to int8 0/1 through a 256-entry LDS table (byte -> 8 bytes).
+// The block stages the B fragments in LDS, shared by its four waves, in
+// stages of 16 KB: stage s of group G holds q-blocks s QS .. s QS + QS - 1
+// (QS = 16 / NT) of all NT tiles, double buffered and moved by direct
+// global-to-LDS loads while the previous stage is multiplied.  A group's rows
+// are loaded at its first stage for the next group.  Bits expand to int8 0/1
+// through a 256-entry LDS table (byte -> 8 bytes).
 template <int RT, int NT>
 __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows_in, uint32_t h,
                                                 const uint32_t* p_words, uint32_t T,
                                                 const uint4* __restrict__ packed_in,
                                                 int64_t* __restrict__ traits_in, uint32_t block) {
+    static_assert(NT == 1 || NT == 2 || NT == 4 || NT == 8, "N tiles: 1, 2, 4 or 8");
+    constexpr int QS = 16 / NT;  // q-blocks per stage
     const auto rows = as_global(rows_in);
     const auto packed = as_global(packed_in);
     const auto traits = as_global(traits_in);
-    constexpr bool kLdsB = (NT == 1);
-    __shared__ u32x4 s_b[kLdsB ? 2 * 16 * 64 : 1];
+    __shared__ u32x4 s_b[2 * 16 * 64];
     __shared__ uint2 s_lut[256];
     const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
     const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
@@ -133,16 +136,19 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
             }
         }
     };
-    // B fragments of group G (16 KB) go straight from global memory into LDS
-    // buffer G & 1 (global_load_lds: no registers, completion counted by
-    // vmcnt); each wave moves four 1 KB pieces.
-    auto stage_b = [&](uint32_t G) {
-        if (kLdsB && G < NG) {
+    // stage k = (group k / NT, part k mod NT) into buffer k & 1: piece i =
+    // (tile i / QS, q-block part QS + i mod QS), 1 KB each, four per wave
+    const uint32_t n_stages = NG * NT;
+    auto stage_b = [&](uint32_t k) {
+        if (k < n_stages) {
+            const uint32_t G = k / NT, q0 = (k % NT) * QS;
             for (uint32_t i = wave; i < 16; i += 4) {
-                const g_u32x4* src = (const g_u32x4*)(packed + ((size_t)G * 16 + i) * 64) + lane;
+                const uint32_t nt = i / QS, q = q0 + i % QS;
+                const g_u32x4* src =
+                    (const g_u32x4*)(packed + (((size_t)nt * NG + G) * 16 + q) * 64) + lane;
                 __builtin_amdgcn_global_load_lds(
                     (const __attribute__((address_space(1))) void*)src,
-                    (__attribute__((address_space(3))) void*)(s_b + (G & 1) * 16 * 64 + i * 64),
+                    (__attribute__((address_space(3))) void*)(s_b + (k & 1) * 16 * 64 + i * 64),
                     16, 0, 0);
             }
         }
@@ -150,7 +156,6 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
     // vmcnt(0) only (expcnt and lgkmcnt at their maxima): the direct-to-LDS
     // loads and the next group's rows have landed
     auto wait_loads = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
-    const g_u32x4* packed4 = (const g_u32x4*)packed;
     u32x4 x[RT][2];
     if (NG) load_rows(0, x);
     stage_b(0);
@@ -159,36 +164,37 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
     for (uint32_t G = 0; G < NG; ++G) {
         u32x4 xn[RT][2];
         if (G + 1 < NG) load_rows(G + 1, xn);
-        stage_b(G + 1);
 #pragma unroll
-        for (uint32_t q = 0; q < 16; ++q) {
-            i8x4 a[RT];
+        for (int s = 0; s < NT; ++s) {
+            const uint32_t k = G * NT + s;
+            stage_b(k + 1);
 #pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                const u32x4 v = x[r][q >> 3];
-                const uint32_t dw = ((q >> 1) & 3) == 0 ? v.x
-                                  : ((q >> 1) & 3) == 1 ? v.y
-                                  : ((q >> 1) & 3) == 2 ? v.z : v.w;
-                const uint32_t b16 = (q & 1) ? (dw >> 16) : (dw & 0xFFFFu);
-                const uint2 lo = s_lut[b16 & 0xFFu], hi = s_lut[b16 >> 8];
-                a[r] = i8x4{(int)lo.x, (int)lo.y, (int)hi.x, (int)hi.y};
+            for (int qq = 0; qq < QS; ++qq) {
+                const int q = s * QS + qq;
+                i8x4 a[RT];
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    const u32x4 v = x[r][q >> 3];
+                    const uint32_t dw = ((q >> 1) & 3) == 0 ? v.x
+                                      : ((q >> 1) & 3) == 1 ? v.y
+                                      : ((q >> 1) & 3) == 2 ? v.z : v.w;
+                    const uint32_t b16 = (q & 1) ? (dw >> 16) : (dw & 0xFFFFu);
+                    const uint2 lo = s_lut[b16 & 0xFFu], hi = s_lut[b16 >> 8];
+                    a[r] = i8x4{(int)lo.x, (int)lo.y, (int)hi.x, (int)hi.y};
+                }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const u32x4 bq = s_b[(k & 1) * 16 * 64 + (nt * QS + qq) * 64 + lane];
+                    const i8x4 bv = i8x4{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
+#pragma unroll
+                    for (int r = 0; r < RT; ++r)
+                        acc[r][nt] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
+                }
             }
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                u32x4 bq;
-                if constexpr (kLdsB)
-                    bq = s_b[(G & 1) * 16 * 64 + q * 64 + lane];
-                else
-                    bq = packed4[(((size_t)nt * NG + G) * 16 + q) * 64 + lane];
-                const i8x4 bv = i8x4{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
-#pragma unroll
-                for (int r = 0; r < RT; ++r)
-                    acc[r][nt] =
-                        __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
-            }
+            wait_loads();
+            __syncthreads();  // stage k + 1 complete; buffer k & 1 free
         }
-        wait_loads();
-        __syncthreads();  // staged B of group G + 1 complete; buffer G & 1 free
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             x[r][0] = xn[r][0];

@@ -127,3 +127,12 @@ def test_autotune_keeps_parity():
         orc.step()
         assert sim.autotune() >= 1
         compare_state(sim, orc, f"autotuned generation {g + 1}")
+
+
+@pytest.mark.parametrize("traits", [7, 16, 29])
+def test_many_traits(traits):
+    # 2, 4 and 8 MFMA N tiles (four traits per tile), several K groups per row
+    cfg = api.make_config(160, 3.0, 2.0, seed=60 + traits, traits=traits, neutral_p=0.2,
+                          sigma=0.3, fitness=api.FIT_MV_STABILIZING, fitness_scale=3.0,
+                          max_alleles=1 << 13)
+    run_pair(cfg, 5, check_every=1, population=_uploaded_population(160, 2500, traits, traits))
