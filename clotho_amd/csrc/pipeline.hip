@@ -1289,16 +1289,84 @@ __global__ void __launch_bounds__(kBlock) k_compact_alleles(Params P, Buffers B)
     for (uint32_t t = 0; t < P.traits; ++t) tmp[(size_t)n * P.traits + t] = eff[(size_t)j * P.traits + t];
 }
 
-// Rows: block = one 64-word tile of the compacted row x a chunk of rows; each
-// wave packs one row at a time.  The input words whose kept bits fall in the
-// tile are [wa, wb) (dst is non-decreasing); every input word contributes its
-// packed bits to at most two output words of the tile, OR-ed into a per-wave
-// LDS row (no global atomics).
+// Rows: block = one 64-word tile of the compacted row x a chunk of rows.  The
+// input words whose kept bits fall in the tile are [wa, wb) (dst is
+// non-decreasing); every input word contributes its packed bits to at most
+// two output words of the tile, OR-ed into a per-wave LDS row (no global
+// atomics).  When the tile draws on at most 64 input words (few columns
+// removed) lane l keeps the plan of input word wa + l in registers (of wa + l
+// and wa + 64 + l up to 128 input words) and a wave packs kCompactRows rows
+// per step, their loads in flight together; otherwise a wave packs one row
+// at a time, reading the plan per row.
 constexpr int kCompactWaves = 4;
+constexpr int kCompactRows = 8;
+
+// The planned path of k_compact_rows: lane l holds the plan (keep mask,
+// compress masks, destination bit) of input words wa + l + 64 i, i < NWL, and
+// a wave packs kCompactRows rows per step starting at row o_first, stepping
+// by kCompactWaves * kCompactRows.
+template <int NWL, class In, class Out, class Keep, class Dst, class Mv>
+__device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out out, Keep keepm,
+                                                     Dst dst, Mv mvg,
+                                                     unsigned long long (*s_row)[64], uint32_t W,
+                                                     uint32_t W2, uint32_t wa, uint32_t wb,
+                                                     uint32_t lo_bit, uint32_t o_first,
+                                                     uint32_t r1) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t u = lo_bit / 64 + lane;
+    uint32_t wr[NWL], n[NWL];
+    uint64_t keep[NWL], mv[NWL][6];
+    int d[NWL];
+#pragma unroll
+    for (int i = 0; i < NWL; ++i) {
+        const uint32_t w = wa + 64 * i + lane;
+        const bool active = w < wb;
+        wr[i] = active ? w : wa;  // in-bounds word for the unconditional loads
+        keep[i] = active ? keepm[wr[i]] : 0;
+        n[i] = (uint32_t)__popcll(keep[i]);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) mv[i][j] = mvg[(size_t)j * P.w_cap + wr[i]];
+        d[i] = (int)dst[wr[i]] - (int)lo_bit;  // tile-relative bit of the packed word
+    }
+    for (uint32_t o0 = o_first; o0 < r1; o0 += kCompactWaves * kCompactRows) {
+        uint64_t xin[kCompactRows][NWL];
+#pragma unroll
+        for (int k = 0; k < kCompactRows; ++k)
+#pragma unroll
+            for (int i = 0; i < NWL; ++i) xin[k][i] = in[(size_t)min(o0 + k, r1 - 1) * W + wr[i]];
+#pragma unroll
+        for (int k = 0; k < kCompactRows; ++k) s_row[k][lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int i = 0; i < NWL; ++i) {
+            if (!n[i]) continue;
+#pragma unroll
+            for (int k = 0; k < kCompactRows; ++k) {
+                const uint64_t x = compress_bits(xin[k][i] & keep[i], mv[i]);
+                if (d[i] >= 0) {
+                    const uint32_t u0 = (uint32_t)d[i] >> 6, s = (uint32_t)d[i] & 63;
+                    if (u0 < 64) atomicOr(&s_row[k][u0], (unsigned long long)(x << s));
+                    if (s && s + n[i] > 64 && u0 + 1 < 64)
+                        atomicOr(&s_row[k][u0 + 1], (unsigned long long)(x >> (64 - s)));
+                } else if (-d[i] < (int)n[i]) {
+                    atomicOr(&s_row[k][0], (unsigned long long)(x >> (-d[i])));
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int k = 0; k < kCompactRows; ++k)
+            if (o0 + k < r1 && u < W2) out[(size_t)(o0 + k) * W2 + u] = s_row[k][lane];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
 
 __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, Buffers B,
                                                                      uint32_t src) {
-    __shared__ unsigned long long s_out[kCompactWaves][64];
+    __shared__ unsigned long long s_out[kCompactWaves][kCompactRows][64];
     const auto st = gbl(B.state);
     const uint32_t V = uniform_read(&st->kept), W = uniform_read(&st->wp);
     const uint32_t W2 = ((V + 511) / 512) * 8;
@@ -1328,8 +1396,19 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
     const uint32_t wb = a;
     const uint32_t rows_per_chunk = (P.h + gridDim.x / P.tiles - 1) / (gridDim.x / P.tiles);
     const uint32_t r0 = chunk * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
+    const uint32_t u = tile * 64 + lane;
+    if (wb - wa <= 64) {
+        compact_rows_planned<1>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, wa, wb, lo_bit,
+                                r0 + wave * kCompactRows, r1);
+        return;
+    }
+    if (wb - wa <= 128) {
+        compact_rows_planned<2>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, wa, wb, lo_bit,
+                                r0 + wave * kCompactRows, r1);
+        return;
+    }
     for (uint32_t o = r0 + wave; o < r1; o += kCompactWaves) {
-        s_out[wave][lane] = 0;
+        s_out[wave][0][lane] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         for (uint32_t base = wa; base < wb; base += 64) {
@@ -1345,17 +1424,16 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
             const int d = (int)dst[w] - (int)lo_bit;  // tile-relative bit of the packed word
             if (d >= 0) {
                 const uint32_t u0 = (uint32_t)d >> 6, s = (uint32_t)d & 63;
-                if (u0 < 64) atomicOr(&s_out[wave][u0], (unsigned long long)(x << s));
+                if (u0 < 64) atomicOr(&s_out[wave][0][u0], (unsigned long long)(x << s));
                 if (s && s + n > 64 && u0 + 1 < 64)
-                    atomicOr(&s_out[wave][u0 + 1], (unsigned long long)(x >> (64 - s)));
+                    atomicOr(&s_out[wave][0][u0 + 1], (unsigned long long)(x >> (64 - s)));
             } else if (-d < (int)n) {
-                atomicOr(&s_out[wave][0], (unsigned long long)(x >> (-d)));
+                atomicOr(&s_out[wave][0][0], (unsigned long long)(x >> (-d)));
             }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const uint32_t u = tile * 64 + lane;
-        if (u < W2) out[(size_t)o * W2 + u] = s_out[wave][lane];
+        if (u < W2) out[(size_t)o * W2 + u] = s_out[wave][0][lane];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }

@@ -99,3 +99,35 @@ def test_compaction_invariant_at_bench_shape():
     tb = sorted(zip(pb, ab, eb[:, 0], cb))
     assert ta == tb
     assert a.rows().shape[0] == h
+
+
+@pytest.mark.parametrize("removed", [0.01, 0.5])
+def test_compact_row_paths(removed):
+    """Four tiles of input words.  With 1% of the columns removed every output
+    tile draws on at most 64 input words (plan in registers, rows in groups);
+    with half removed the tiles draw on about 128 (plan read per row)."""
+    rng = np.random.default_rng(int(removed * 100) + 11)
+    n, alleles = 150, 16000
+    words = ((alleles + 511) // 512) * 8
+    freqs = rng.uniform(0.05, 0.95, alleles)
+    gone = rng.uniform(0, 1, alleles) < removed
+    bits = rng.uniform(0, 1, (2 * n, words * 64)) < np.pad(freqs, (0, words * 64 - alleles))[None, :]
+    bits[:, alleles:] = False
+    lost = np.flatnonzero(gone)
+    bits[:, lost[::2]] = False   # lost columns
+    bits[:, lost[1::2]] = True   # fixed columns
+    rows = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64)
+    pos = np.floor(rng.uniform(0, 1, alleles) * 2**32) / 2**32
+    eff = np.round(rng.normal(0, 0.2, (alleles, 1)) * 2**24) / 2**24
+    cfg = api.make_config(n, 1.0, 2.0, seed=5, traits=1, neutral_p=0.2, sigma=0.2,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 15)
+    sim, orc = api.Simulation(cfg), oracle.OracleSim(cfg)
+    sim.set_population(rows, pos, eff)
+    orc.set_population(rows, pos, eff)
+    sim.compact()
+    orc.compact()
+    compare_state(sim, orc, f"after compact ({removed:.0%} removed)")
+    sim.run(2)
+    orc.step()
+    orc.step()
+    compare_state(sim, orc, "two generations later")
