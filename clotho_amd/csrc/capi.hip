@@ -885,24 +885,36 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
     // Each timed recombination is followed by the traits pass over its output,
     // as in a generation, so the caches hold what they hold in the pipeline;
     // two passes over the candidates, each candidate keeping its faster pass.
+    // Launches longer than 2 ms are timed once, in one pass.
     constexpr int kReps = 3;
     hipEvent_t ev[2 * kReps] = {};
     for (hipEvent_t& e : ev) HIP_TRY(c, hipEventCreate(&e));
+    int reps = kReps, passes = 2;
+    {
+        HIP_TRY(c, launch_recombine(c));
+        HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+        HIP_TRY(c, launch_recombine(c));
+        HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+        HIP_TRY(c, hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+        if (ms > 2.0f) reps = passes = 1;
+    }
     std::vector<float> best_ms(cand.size(), 1e30f);
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         for (size_t i = 0; i < cand.size(); ++i) {
             c->p.chunks = cand[i];
             HIP_TRY(c, launch_recombine(c));  // warm-up
             HIP_TRY(c, launch_traits(c));
-            for (int r = 0; r < kReps; ++r) {
+            for (int r = 0; r < reps; ++r) {
                 HIP_TRY(c, hipEventRecord(ev[2 * r], c->stream));
                 HIP_TRY(c, launch_recombine(c));
                 HIP_TRY(c, hipEventRecord(ev[2 * r + 1], c->stream));
                 HIP_TRY(c, launch_traits(c));
             }
-            HIP_TRY(c, hipEventSynchronize(ev[2 * kReps - 1]));
+            HIP_TRY(c, hipEventSynchronize(ev[2 * reps - 1]));
             float tot = 0.f;
-            for (int r = 0; r < kReps; ++r) {
+            for (int r = 0; r < reps; ++r) {
                 float ms = 0.f;
                 HIP_TRY(c, hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
                 tot += ms;
