@@ -47,7 +47,8 @@ class Config(C.Structure):
         ("max_alleles", C.c_uint32),
         ("compact_every", C.c_uint32),
         ("max_individuals", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("intermediates", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -75,7 +76,8 @@ class Shape(C.Structure):
 
 def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, neutral_p=0.5,
                 mean=0.0, sigma=1.0, fitness_scale=1.0, env_sigma=0.0, quantum_log2=-24,
-                max_alleles=1 << 20, replicate=0, compact=False, max_individuals=0):
+                max_alleles=1 << 20, replicate=0, compact=False, max_individuals=0,
+                intermediates=0):
     """Build a Config; defaults follow the reference's parameter defaults
     (sequence_bias.p 0.5, neutral.p 0.5, traits.mean 0, traits.sigma 1)."""
     c = Config()
@@ -96,6 +98,7 @@ def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, n
     c.max_alleles = max_alleles
     c.compact_every = 1 if compact else 0
     c.max_individuals = max_individuals
+    c.intermediates = intermediates
     return c
 
 

@@ -99,6 +99,20 @@ int write_state(clotho_ctx* c, const GenState& st) {
     return CLOTHO_OK;
 }
 
+// Free space of an uploaded population: the analysis runs as at the end of
+// an intermediate batch, and the next generation starts a batch.
+int analyse_upload(clotho_ctx* c, GenState st) {
+    st.inter = c->p.inter_k;
+    st.free_base = 0;
+    int rc = write_state(c, st);
+    if (rc) return rc;
+    HIP_TRY(c, launch_population_analysis(c));
+    rc = read_state(c, &st);
+    if (rc) return rc;
+    st.inter = 0;
+    return write_state(c, st);
+}
+
 // ifitness_generator::generate(N) (src/clotho/fitness/normal_fitness_generator.cpp:36-52,
 // src/clotho/fitness/quadratic_fitness_generator.cpp:42-54): constants for the
 // P.n individuals being evaluated.
@@ -213,6 +227,8 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     *out = nullptr;
     if (cfg->n_individuals == 0 || cfg->n_individuals > (1u << 28)) return CLOTHO_ERR_ARG;
     if (cfg->max_individuals > (1u << 28)) return CLOTHO_ERR_ARG;
+    if (cfg->intermediates > (1u << 16) || (cfg->intermediates && cfg->compact_every))
+        return CLOTHO_ERR_ARG;
     if (cfg->trait_count > (uint32_t)kMaxTraits) return CLOTHO_ERR_ARG;
     if (!(cfg->bias >= 0.0 && cfg->bias <= 1.0)) return CLOTHO_ERR_ARG;
     if (!(cfg->neutral_p >= 0.0 && cfg->neutral_p <= 1.0)) return CLOTHO_ERR_ARG;
@@ -258,6 +274,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     P.h = 2 * P.n;
     P.n_par = P.n;
     P.n_cap = std::max(P.n, cfg->max_individuals);
+    P.inter_k = cfg->intermediates;
     const uint32_t h_cap = 2 * P.n_cap;
     P.traits = cfg->trait_count;
     P.s_cap = ((cfg->max_alleles + 511) / 512) * 512;
@@ -359,7 +376,10 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     }
     GenState st;
     std::memset(&st, 0, sizeof(st));
-    st.gen = 1;  // Engine::init leaves m_generation at 1 before the first simulate
+    // Engine::init leaves m_generation at 1 before the first simulate; with k
+    // intermediate generations population i of batch g is generation
+    // g (k + 1) + i (gen_offset, QS/engine_parallel_pipeline_batched.hpp:390-396)
+    st.gen = 1 + P.inter_k;
     st.fit_constant = 1;
     ok = ok && hipMemcpy(B.state, &st, sizeof(st), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) return bail(CLOTHO_ERR_HIP);
@@ -438,10 +458,8 @@ int clotho_set_population(clotho_ctx* c, const uint64_t* rows, uint32_t words,
     st.wp = words;
     st.m = 0;
     st.error = 0;
-    rc = write_state(c, st);
+    rc = analyse_upload(c, st);
     if (rc) return rc;
-    HIP_TRY(c, launch_population_analysis(c));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->has_population = true;
     return CLOTHO_OK;
 }
@@ -496,10 +514,8 @@ int clotho_set_synthetic_population(clotho_ctx* c, uint32_t alleles, const doubl
     st.wp = words;
     st.m = 0;
     st.error = 0;
-    rc = write_state(c, st);
+    rc = analyse_upload(c, st);
     if (rc) return rc;
-    HIP_TRY(c, launch_population_analysis(c));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->has_population = true;
     return CLOTHO_OK;
 }
@@ -586,10 +602,9 @@ int clotho_download_free_space(clotho_ctx* c, uint32_t* fixed, uint32_t* lost, s
     if (rc) return rc;
     const uint32_t nf = st.n_fixed, nl = st.free_count - st.n_fixed;
     if (cap < std::max(nf, nl)) return fail(c, CLOTHO_ERR_ARG, "download_free_space: too small");
-    if (fixed && nf)
-        HIP_TRY(c, hipMemcpy(fixed, c->b.free_list, 4 * nf, hipMemcpyDeviceToHost));
-    if (lost && nl)
-        HIP_TRY(c, hipMemcpy(lost, c->b.free_list + nf, 4 * nl, hipMemcpyDeviceToHost));
+    const uint32_t* list = c->b.free_list + st.free_base;  // entries not yet handed out
+    if (fixed && nf) HIP_TRY(c, hipMemcpy(fixed, list, 4 * nf, hipMemcpyDeviceToHost));
+    if (lost && nl) HIP_TRY(c, hipMemcpy(lost, list + nf, 4 * nl, hipMemcpyDeviceToHost));
     return CLOTHO_OK;
 }
 

@@ -259,10 +259,10 @@ struct MutAttrs {
         if (o >= P.h) return;
         const uint32_t c = count[o];
         if (!c) return;
-        const uint32_t gen = st->gen, F = st->free_count, A = st->alleles;
+        const uint32_t gen = st->gen, F = st->free_count, A = st->alleles, fb = st->free_base;
         const uint32_t k0 = local[o] + boff[o / kBlock];
         for (uint32_t k = k0; k < k0 + c; ++k) {
-            const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+            const uint32_t slot = k < F ? free_list[fb + k] : A + (k - F);
             if (slot >= P.s_cap) continue;
             const Draw4 d = philox_draw(P.seed, gen, S_MUTATTR, P.replicate, k, 0);
             keys[slot] = d.x[0];
@@ -644,7 +644,7 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
     const auto boff = gbl(B.mut_bsum);
     const uint32_t wc = uniform_read(&st->wc), wp = uniform_read(&st->wp);
     const uint32_t A = uniform_read(&st->alleles), F = uniform_read(&st->free_count),
-                   S = uniform_read(&st->all_size);
+                   S = uniform_read(&st->all_size), fb = uniform_read(&st->free_base);
     const uint32_t ev_cap = P.ev_cap;  // <= 64 on this path
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -730,7 +730,7 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
         for (uint32_t k = 0; k < max_nm; ++k) {
             if (k < nm) {
                 const uint32_t kk = k0 + k;
-                const uint32_t slot = kk < F ? free_list[kk] : A + (kk - F);
+                const uint32_t slot = kk < F ? free_list[fb + kk] : A + (kk - F);
                 if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
             }
         }
@@ -810,7 +810,7 @@ struct Recombine {
         const bool inpar = valid && w < wp;
         const uint32_t col = wp ? min(w, wp - 1) : 0;  // in-bounds parent column
         const uint32_t A = uniform_read(&st->alleles), F = uniform_read(&st->free_count),
-                       S = uniform_read(&st->all_size);
+                       S = uniform_read(&st->all_size), fb = uniform_read(&st->free_base);
         const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
 
         for (uint32_t k = wave; k < 68; k += kRecWaves)
@@ -959,7 +959,7 @@ struct Recombine {
                 if (nm) {
                     const uint32_t k0 = lane_read(g_k0, q);
                     for (uint32_t k = k0; k < k0 + nm; ++k) {
-                        const uint32_t slot = k < F ? free_list[k] : A + (k - F);
+                        const uint32_t slot = k < F ? free_list[fb + k] : A + (k - F);
                         if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
                     }
                 }
@@ -1075,7 +1075,9 @@ struct FreeReduce {
             const uint64_t fx = va & validm, ls = ~vo & validm;
             cf = (uint32_t)__popcll(fx);
             cl = (uint32_t)__popcll(ls);
-            fixed_bits[w] = fx;
+            // after an intermediate generation nothing is cleared from the
+            // next parents (their fixed columns were handed out as free slots)
+            fixed_bits[w] = st->inter == P.inter_k ? fx : 0;
             lost_bits[w] = ls;
         }
         // wave-level exclusive scans of the fixed and lost counts
@@ -1113,6 +1115,8 @@ struct FreeCompact {
         const uint32_t wc = st->wc;
         const uint32_t nb = (wc + kFreeWords - 1) / kFreeWords;
         if (blk >= nb) return;
+        // intermediate generation: the free list of the last analysis stays
+        if (st->inter != P.inter_k) return;
         // offsets of this block and the fixed total, from the block totals
         uint32_t pf = 0, tfix = 0, pl = 0;
         for (uint32_t q = threadIdx.x; q < nb; q += kBlock) {
@@ -1469,6 +1473,7 @@ __global__ void k_compact_state(Buffers B) {
     st->wp = st->wc = ((V + 511) / 512) * 8;
     st->free_count = 0;
     st->n_fixed = 0;
+    st->free_base = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1630,15 +1635,32 @@ __global__ void __launch_bounds__(kBlock) k_synth_rows(Params P, uint64_t seed,
     rows[id] = x;
 }
 
+// End of a generation.  With intermediate generations
+// (QS/engine_parallel_pipeline_batched.hpp:390-407) the free slots the new
+// mutations took are dropped from the front of the free list until the last
+// generation of the batch, whose analysis wrote a fresh list.
 struct Commit {
     static constexpr int kThreads = 64;
     __host__ __device__ static uint32_t blocks(const Params&) { return 1u; }
-    __device__ static __forceinline__ void run(const Params&, const Buffers& B, uint32_t, uint32_t) {
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t,
+                                               uint32_t) {
         if (threadIdx.x) return;
         const auto st = gbl(B.state);
         st->alleles = st->all_size;
         st->wp = st->wc;
         st->gen = st->gen + 1;
+        if (P.inter_k) {
+            if (st->inter < P.inter_k) {
+                const uint32_t F = st->free_count, used = min(st->m, F);
+                st->free_base = st->free_base + used;
+                st->free_count = F - used;
+                st->n_fixed = st->n_fixed > used ? st->n_fixed - used : 0u;
+                st->inter = st->inter + 1;
+            } else {
+                st->free_base = 0;
+                st->inter = 0;
+            }
+        }
     }
 };
 

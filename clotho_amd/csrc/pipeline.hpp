@@ -34,6 +34,8 @@ struct GenState {
     uint32_t error_gen;    // generation of the first error
     uint64_t sel_total;    // sum of quantised fitness
     uint32_t kept;         // variable columns kept by the last compaction plan
+    uint32_t free_base;    // free-list entries used by earlier intermediate generations
+    uint32_t inter;        // position in the intermediate batch, 0 .. inter_k
 };
 
 // Kernel-side constants that do not change between generations.
@@ -46,6 +48,7 @@ struct Params {
     uint32_t n_par;        // individuals of the parent population of the
                            // generation being produced
     uint32_t n_cap;        // individual capacity of the buffers
+    uint32_t inter_k;      // intermediate generations between free-space analyses
     uint32_t traits;       // T
     uint32_t s_cap;        // allele capacity
     uint32_t w_cap;        // word capacity per row
@@ -72,7 +75,8 @@ struct Buffers {
     uint32_t* keys;           // [s_cap] position * 2^32
     int32_t* effects;         // [s_cap][T] quantised effects
     uint32_t* ages;           // [s_cap]
-    uint32_t* free_list;      // [s_cap] fixed ascending then lost ascending
+    uint32_t* free_list;      // [s_cap] fixed ascending then lost ascending; entries
+                              // from free_base on are unused
     uint64_t* fixed_bits;     // [w_cap] AND of all rows, idx < all_size
     uint32_t* sorted;         // [w_cap][64] position keys of each word, ascending
     uint64_t* pm;             // [w_cap][65] bits of the c smallest keys of each word

@@ -102,7 +102,11 @@ typedef struct clotho_config {
                                  generation (clotho_compact); 0: keep the reference's
                                  slot reuse                                              */
     uint32_t max_individuals; /* individual capacity for population growth (0: N)        */
-    uint32_t reserved[2];
+    uint32_t intermediates;   /* generations.intermediates: k generations between free-
+                                 space analyses (QS/engine_parallel_pipeline_batched.hpp);
+                                 0 = analysis after every generation; not with
+                                 compact_every                                          */
+    uint32_t reserved[1];
 } clotho_config;
 
 /* Shape and bookkeeping of the current (child) population after a generation. */

@@ -271,6 +271,9 @@ inline clotho_config make_config(property_tree& config) {
     c.env_sigma = config.get("environment.sigma", 0.0);
     c.effect_quantum_log2 = (int32_t)config.get("traits.quantum_log2", -24.0);
     c.max_alleles = (uint32_t)config.get("alleles.capacity", 1048576.0);
+    // generations.intermediates (QS/engine_parallel_pipeline_batched.hpp:155-166), written back
+    c.intermediates = (uint32_t)config.get("generations.intermediates", 0.0);
+    if (!config.has("generations.intermediates")) config.put("generations.intermediates", 0.0);
     auto gen = fitness_toolkit::getInstance()->get_tool(config);
     c.fitness_kind = gen ? gen->device_kind() : CLOTHO_FIT_CONSTANT;
     c.fitness_scale = gen ? gen->device_scale() : 1.0;
@@ -562,10 +565,17 @@ public:
         return m_growth ? (*m_growth)(parent, (uint32_t)m_generation) : parent;
     }
 
-    // one generation: draw, recombine+mutate, traits+fitness, free space, commit
+    // one generation: draw, recombine+mutate, traits+fitness, free space,
+    // commit; with k intermediate generations one batch of k + 1 generations
+    // (the batched engine's simulate, QS/engine_parallel_pipeline_batched.hpp:198-288)
     void simulate() {
         clotho_ctx* c = m_pop.ctx();
         check(clotho_set_population_size(c, next_population_size()), c, "set_population_size");
+        if (m_pop.config().intermediates) {
+            check(clotho_run(c, m_pop.config().intermediates + 1, 0), c, "clotho_run");
+            ++m_generation;
+            return;
+        }
         check(clotho_phase_draw(c), c, "phase_draw");
         m_xover(m_pop);
         m_fit(m_pop);
@@ -573,8 +583,9 @@ public:
         check(clotho_phase_commit(c), c, "phase_commit");
         ++m_generation;
     }
-    // many generations without host round trips (hipGraph replay when asked);
-    // a growth model splits them into runs of equal population size
+    // many generations (batches of k + 1 with k intermediates) without host
+    // round trips (hipGraph replay when asked); a growth model splits them
+    // into runs of equal population size
     void simulate(unsigned int generations, bool graph) {
         clotho_ctx* c = m_pop.ctx();
         uint32_t parent = m_pop.getIndividualCount();
@@ -587,7 +598,8 @@ public:
                 ++run;
             if (!m_growth) run = generations;
             check(clotho_set_population_size(c, n), c, "set_population_size");
-            check(clotho_run(c, run, graph ? 1 : 0), c, "clotho_run");
+            check(clotho_run(c, run * (m_pop.config().intermediates + 1), graph ? 1 : 0), c,
+                  "clotho_run");
             m_generation += run;
             generations -= run;
             parent = n;

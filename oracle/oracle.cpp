@@ -521,6 +521,7 @@ struct or_sim {
     clotho_config cfg;
     uint32_t N = 0, H = 0, T = 0;
     uint32_t next_N = 0;  // offspring individuals of the following generations (0: N)
+    uint32_t inter = 0;   // position in the intermediate batch, 0 .. cfg.intermediates
     uint32_t gen = 1;  // Engine::init increments m_generation to 1 before the first simulate
     std::vector<uint64_t> parent, child;
     uint32_t wp = 0, wc = 0;
@@ -542,8 +543,10 @@ struct or_sim {
 
 extern "C" or_sim* or_sim_create(const clotho_config* cfg) {
     if (!cfg || cfg->n_individuals == 0) return nullptr;
+    if (cfg->intermediates && cfg->compact_every) return nullptr;
     or_sim* s = new or_sim();
     s->cfg = *cfg;
+    s->gen = 1 + cfg->intermediates;  // population i of batch g: g (k + 1) + i
     s->N = cfg->n_individuals;
     s->H = 2 * s->N;
     s->T = cfg->trait_count;
@@ -596,6 +599,7 @@ extern "C" int or_sim_set_population(or_sim* s, const uint64_t* rows, uint32_t w
     std::vector<uint32_t> fx(alleles + 1), ls(alleles + 1);
     uint32_t nf = 0, nl = 0;
     or_free_space(s->child.data(), s->H, words, alleles, fx.data(), &nf, ls.data(), &nl);
+    s->inter = 0;  // an upload is analysed like the end of a batch
     s->fixed_list.assign(fx.begin(), fx.begin() + nf);
     s->lost_list.assign(ls.begin(), ls.begin() + nl);
     return 0;
@@ -729,10 +733,15 @@ extern "C" int or_sim_step(or_sim* s) {
     s->wp = s->wc;
 
     // 2: removeFixedAlleles (:409-447; population_space::remove_fixed_allele,
-    //    DS/population_space/population_space_row_block_alignment.hpp:133-153)
-    for (uint32_t j : s->fixed_list) {
-        const uint64_t keep = ~((uint64_t)1 << (j % 64));
-        for (uint32_t r = 0; r < H_par; ++r) s->parent[(size_t)r * s->wp + j / 64] &= keep;
+    //    DS/population_space/population_space_row_block_alignment.hpp:133-153),
+    //    with intermediate generations only on the first parent of a batch
+    //    (buildProjectedPopulations, QS/engine_parallel_pipeline_batched.hpp:370-382)
+    const uint32_t K = cfg.intermediates;
+    if (s->inter == 0) {
+        for (uint32_t j : s->fixed_list) {
+            const uint64_t keep = ~((uint64_t)1 << (j % 64));
+            for (uint32_t r = 0; r < H_par; ++r) s->parent[(size_t)r * s->wp + j / 64] &= keep;
+        }
     }
     const uint32_t F = (uint32_t)(s->fixed_list.size() + s->lost_list.size());
     std::vector<uint32_t> free_slots(s->fixed_list);
@@ -845,7 +854,27 @@ extern "C" int or_sim_step(or_sim* s) {
     s->fitness_constant = all_equal(s->fitness.data(), N);
     auto c3 = std::chrono::steady_clock::now();
 
-    // 13: free_space.finalize(all_size) (DS/free_space/free_space_accumulator.hpp:142-151)
+    // 13: free_space.finalize(all_size) (DS/free_space/free_space_accumulator.hpp:142-151),
+    //     with intermediate generations only after the last of a batch; before
+    //     it the slots the new mutations took leave the front of the free list
+    //     (the shared free-space iterator of grow_child_population,
+    //     QS/engine_parallel_pipeline_batched.hpp:397-425)
+    if (s->inter < K) {
+        uint32_t used = std::min(s->M, F);
+        const uint32_t from_fixed = std::min<uint32_t>(used, (uint32_t)s->fixed_list.size());
+        s->fixed_list.erase(s->fixed_list.begin(), s->fixed_list.begin() + from_fixed);
+        used -= from_fixed;
+        s->lost_list.erase(s->lost_list.begin(), s->lost_list.begin() + used);
+        s->A = all_size;
+        ++s->gen;
+        ++s->inter;
+        auto c4 = std::chrono::steady_clock::now();
+        s->phase[4] = std::chrono::duration<double>(c3 - c2).count();
+        s->phase[5] = std::chrono::duration<double>(c4 - c3).count();
+        s->phase[6] = std::chrono::duration<double>(c1 - c0).count();
+        return 0;
+    }
+    s->inter = 0;
     for (uint32_t j = 1; j < TC; ++j)
         for (uint32_t k = 0; k < s->wc; ++k) {
             var[k] |= var[(size_t)j * s->wc + k];

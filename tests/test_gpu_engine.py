@@ -113,3 +113,24 @@ def test_engine_demo_population_growth(tmp_path, graph):
     assert out["generation"] == s.generation and out["alleles"] == s.alleles
     assert out["free"] == s.free_count and out["fixed"] == s.fixed_count
     assert out["rows_fnv1a"] == fnv1a(orc.rows())
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_engine_demo_intermediates(graph):
+    """generations.intermediates = 2 through the C++ driver: each simulate()
+    is one batch of three generations (the batched engine's simulate), and
+    three of them match nine oracle steps with the same setting."""
+    args = [build.DEMO, "population_growth.params.B=250", "mutation.mu=2",
+            "recombination.rho=2", "random_number.seed=8", "neutral.p=0.4",
+            "fitness.name=gaussian_fixed", "generations.intermediates=2",
+            "--generations", "3"] + (["--graph"] if graph else [])
+    out = json.loads(subprocess.run(args, check=True, capture_output=True, text=True).stdout)
+    cfg = api.make_config(250, 2.0, 2.0, seed=8, traits=1, neutral_p=0.4,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=32768, intermediates=2)
+    orc = oracle.OracleSim(cfg)
+    for _ in range(9):
+        orc.step()
+    s = orc.shape()
+    assert out["generation"] == s.generation and out["alleles"] == s.alleles
+    assert out["free"] == s.free_count and out["fixed"] == s.fixed_count
+    assert out["rows_fnv1a"] == fnv1a(orc.rows())

@@ -491,3 +491,39 @@ def test_oracle_population_size_changes():
         n_par = n
     with pytest.raises(ValueError):
         a.set_population_size(0)
+
+
+def test_oracle_intermediate_generations():
+    """k intermediate generations: generation numbers start at k + 1; between
+    analyses the free list only loses the slots the new mutations took (from
+    the front, fixed before lost) and new alleles beyond it are appended; after
+    the last generation of a batch the lists are the free space of the
+    current rows (or_free_space); an upload starts a batch."""
+    k = 2
+    cfg = api.make_config(120, 3.0, 2.0, seed=12, traits=1, neutral_p=0.5, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 14, intermediates=k)
+    orc = oracle.OracleSim(cfg)
+    assert orc.shape().generation == k + 1
+    for step in range(3 * (k + 1)):
+        before = orc.shape()
+        fx0, ls0 = orc.free_space()
+        free0 = np.concatenate([fx0, ls0])
+        orc.step()
+        s = orc.shape()
+        assert s.generation == before.generation + 1
+        m, F = s.new_mutations, before.free_count
+        assert s.alleles == before.alleles + max(0, m - F)
+        pos, _, ages = orc.alleles()
+        new_slots = np.flatnonzero(ages == before.generation)
+        expect = np.concatenate([free0[:min(m, F)], before.alleles + np.arange(max(0, m - F))])
+        assert sorted(new_slots.tolist()) == sorted(expect.tolist())
+        fx, ls = orc.free_space()
+        if (step + 1) % (k + 1):  # intermediate: the unused tail of the old lists
+            assert np.array_equal(np.concatenate([fx, ls]), free0[min(m, F):])
+        else:  # end of a batch: the free space of the current rows
+            rows = orc.rows()
+            ofx, ols = oracle.free_space(rows, s.alleles)
+            assert np.array_equal(fx, ofx) and np.array_equal(ls, ols)
+    with pytest.raises(ValueError):  # no compaction with intermediate generations
+        oracle.OracleSim(api.make_config(100, 1.0, 1.0, intermediates=1, compact=True,
+                                         max_alleles=1 << 12))
