@@ -123,6 +123,7 @@ EXPORTS = [
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
     "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
     "clotho_sample_rows", "clotho_allele_counts", "clotho_set_population_size",
+    "clotho_pairwise_differences",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -157,6 +158,8 @@ def lib():
     L.clotho_download_gametes.argtypes = [vp, vp, vp, vp, vp, C.c_size_t]
     L.clotho_run.argtypes = [vp, C.c_uint32, C.c_int]
     L.clotho_set_population_size.argtypes = [vp, C.c_uint32]
+    L.clotho_pairwise_differences.argtypes = [vp, u32p, C.c_uint32, u32p, C.c_size_t,
+                                              C.POINTER(C.c_double)]
     for n in ("clotho_generation", "clotho_phase_draw", "clotho_phase_recombine_mutate",
               "clotho_phase_traits",
               "clotho_phase_free_space", "clotho_phase_fitness", "clotho_phase_commit",
@@ -337,6 +340,18 @@ class Simulation:
 
     def compact(self):
         self._check(lib().clotho_compact(self.ctx), "compact")
+
+    def pairwise_differences(self, rows, timed=False):
+        """popcount(row a xor row b) over the current rows for the pairs a < b
+        of `rows` (in the given order), packed row by row."""
+        rows = np.ascontiguousarray(rows, np.uint32)
+        n = rows.size
+        out = np.zeros(max(1, n * (n - 1) // 2), np.uint32)
+        ms = C.c_double()
+        self._check(lib().clotho_pairwise_differences(self.ctx, rows, n, out, out.size,
+                                                      C.byref(ms)), "pairwise_differences")
+        out = out[: n * (n - 1) // 2]
+        return (out, ms.value) if timed else out
 
     def set_population_size(self, n):
         """Offspring individuals of every following generation (the pN of a

@@ -761,6 +761,52 @@ int clotho_allele_counts(clotho_ctx* c, const uint32_t* rows, uint32_t n_rows,
     return CLOTHO_OK;
 }
 
+int clotho_pairwise_differences(clotho_ctx* c, const uint32_t* rows, uint32_t n_rows,
+                                uint32_t* out, size_t cap, double* kernel_ms) {
+    if (!c || (!rows && n_rows) || (!out && n_rows > 1)) return CLOTHO_ERR_ARG;
+    if (n_rows > CLOTHO_MAX_PAIRWISE_ROWS)
+        return fail(c, CLOTHO_ERR_ARG, "pairwise_differences: at most " +
+                                           std::to_string(CLOTHO_MAX_PAIRWISE_ROWS) + " rows");
+    const size_t pairs = (size_t)n_rows * (n_rows ? n_rows - 1 : 0) / 2;
+    if (cap < pairs) return fail(c, CLOTHO_ERR_ARG, "pairwise_differences: output too small");
+    HIP_TRY(c, hipSetDevice(c->device));
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t W = st.wc, H = c->p.h;
+    for (uint32_t i = 0; i < n_rows; ++i)
+        if (rows[i] >= H) return fail(c, CLOTHO_ERR_ARG, "pairwise_differences: row out of range");
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (pairs == 0) return CLOTHO_OK;
+    uint32_t *d_sample = nullptr, *d_out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    step(hipMalloc((void**)&d_sample, 4ull * n_rows));
+    step(hipMalloc((void**)&d_out, 4ull * pairs));
+    step(hipEventCreate(&e0));
+    step(hipEventCreate(&e1));
+    if (e == hipSuccess)
+        step(hipMemcpyAsync(d_sample, rows, 4ull * n_rows, hipMemcpyHostToDevice, c->stream));
+    step(hipEventRecord(e0, c->stream));
+    if (e == hipSuccess) step(launch_pairwise(c, c->b.rows[c->cur], W, d_sample, n_rows, d_out));
+    step(hipEventRecord(e1, c->stream));
+    if (e == hipSuccess)
+        step(hipMemcpyAsync(out, d_out, 4ull * pairs, hipMemcpyDeviceToHost, c->stream));
+    step(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (e == hipSuccess) step(hipEventElapsedTime(&ms, e0, e1));
+    for (uint32_t* p : {d_sample, d_out})
+        if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    if (kernel_ms) *kernel_ms = ms;
+    return CLOTHO_OK;
+}
+
 int clotho_device_bytes(const clotho_ctx* c, uint64_t* bytes) {
     if (!c || !bytes) return CLOTHO_ERR_ARG;
     *bytes = c->device_bytes;

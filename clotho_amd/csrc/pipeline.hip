@@ -2021,6 +2021,56 @@ hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W,
     return hipSuccess;
 }
 
+// Pairwise differences popcount(r_i xor r_j) over the W words of sample
+// rows i < j (pairwise_difference<association_matrix>::eval with the
+// row-grouped evaluator,
+// include/clotho/data_spaces/analysis/pairwise_difference/association_matrix_pairwise_difference.hpp:78-102,
+// row_grouped_pairwise_difference.hpp:25-55).  Block = a 16 x 16 tile of pairs
+// (tiles on or above the diagonal, enumerated row by row), one pair per
+// thread; 64-word chunks of the tile's 16 + 16 rows are staged in LDS with a
+// 65-word row pitch, so the 16 rows a wave reads per word fall in distinct banks.
+constexpr uint32_t kPairTile = 16;
+constexpr uint32_t kPairChunk = 64;
+
+__global__ void __launch_bounds__(256) k_pairwise(const uint64_t* __restrict__ rows_in, uint32_t W,
+                                                  const uint32_t* __restrict__ sample, uint32_t n,
+                                                  uint32_t tiles, uint32_t* __restrict__ out) {
+    const auto rows = gbl(rows_in);
+    __shared__ uint64_t sa[kPairTile][kPairChunk + 1], sb[kPairTile][kPairChunk + 1];
+    uint32_t t = blockIdx.x, bi = 0;
+    while (t >= tiles - bi) {  // uniform: at most `tiles` steps
+        t -= tiles - bi;
+        ++bi;
+    }
+    const uint32_t bj = bi + t;
+    const uint32_t ti = threadIdx.x / kPairTile, tj = threadIdx.x % kPairTile;
+    const uint32_t i = bi * kPairTile + ti, j = bj * kPairTile + tj;
+    uint32_t acc = 0;
+    for (uint32_t w0 = 0; w0 < W; w0 += kPairChunk) {
+        for (uint32_t e = threadIdx.x; e < kPairTile * kPairChunk; e += 256) {
+            const uint32_t r = e / kPairChunk, c = e % kPairChunk, w = w0 + c;
+            const uint32_t ia = bi * kPairTile + r, ib = bj * kPairTile + r;
+            sa[r][c] = (ia < n && w < W) ? rows[(size_t)sample[ia] * W + w] : 0;
+            sb[r][c] = (ib < n && w < W) ? rows[(size_t)sample[ib] * W + w] : 0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (uint32_t c = 0; c < kPairChunk; ++c) acc += (uint32_t)__popcll(sa[ti][c] ^ sb[tj][c]);
+        __syncthreads();
+    }
+    if (i < j && j < n) out[(size_t)i * n - (size_t)i * (i + 1) / 2 + (j - i - 1)] = acc;
+}
+
+hipError_t launch_pairwise(clotho_ctx* c, const uint64_t* rows, uint32_t W,
+                           const uint32_t* d_sample, uint32_t n, uint32_t* d_out) {
+    const uint32_t tiles = (n + kPairTile - 1) / kPairTile;
+    const uint32_t blocks = tiles * (tiles + 1) / 2;
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_pairwise, dim3(blocks), dim3(256), 0, c->stream, rows, W, d_sample, n,
+                       tiles, d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag) {
     const size_t n = (size_t)c->p.s_cap * c->p.traits;
     hipLaunchKernelGGL(k_any_nonzero, dim3(64), dim3(kBlock), 0, c->stream, c->b.effects, n,

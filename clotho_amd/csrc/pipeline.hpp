@@ -193,6 +193,10 @@ hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W,
                                 const uint32_t* d_sample, uint32_t n, uint32_t* d_cols,
                                 uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,
                                 uint32_t chunks);
+// popcount(row_a xor row_b) for the sample pairs a < b, packed row by row
+// (n (n - 1) / 2 values).
+hipError_t launch_pairwise(clotho_ctx* c, const uint64_t* rows, uint32_t W,
+                           const uint32_t* d_sample, uint32_t n, uint32_t* d_out);
 hipError_t launch_all_neutral(clotho_ctx* c, uint32_t* d_flag);
 hipError_t launch_synth_rows(clotho_ctx* c, uint64_t seed, const uint32_t* d_thr, uint32_t alleles,
                              uint32_t words);

@@ -146,9 +146,26 @@ void run_logged(Engine& engine, clotho_amd::property_tree& config, unsigned gene
                 if (k < 0) break;
                 if (k == 0) continue;
                 clotho_amd::allele_frequency af;
-                af.evaluate(*pop, (uint64_t)config.get("random_number.seed", 0.0), sh.generation,
-                            (uint32_t)k);
+                const uint64_t seed = (uint64_t)config.get("random_number.seed", 0.0);
+                af.evaluate(*pop, seed, sh.generation, (uint32_t)k);
                 record(af, "sample." + std::to_string(i) + ".allele_frequency");
+                // sample_log_params::pairwise (examples/qtl/qtl_logging_parameter.hpp:31-62):
+                // the pairwise differences of the same sample
+                if (config.get("log.sampling_size." + std::to_string(i) + ".pairwise", 0.0) != 0.0) {
+                    std::vector<uint32_t> rows(std::min<uint32_t>((uint32_t)k, sh.haplotypes));
+                    clotho_amd::check(clotho_sample_rows(seed, sh.generation, pop->config().replicate,
+                                                         sh.haplotypes, (uint32_t)k, rows.data()),
+                                      pop->ctx(), "sample_rows");
+                    clotho_amd::pairwise_difference pd;
+                    pd.evaluate(*pop, rows.begin(), rows.end());
+                    const auto r = pd.results();
+                    const std::string b = "sample." + std::to_string(i) + ".pairwise_difference.";
+                    log.put(b + "min", r.min);
+                    log.put(b + "max", r.max);
+                    log.put(b + "mean", r.mean);
+                    log.put(b + "median", r.median);
+                    log.put(b + "variance", r.variance);
+                }
             }
             log.at("population_data.free_space")
                 .add_value_array("free", fs->free_begin(), fs->free_end());

@@ -188,6 +188,19 @@ int clotho_allele_counts(clotho_ctx* ctx, const uint32_t* rows, uint32_t n_rows,
                          uint32_t* column_counts, size_t cap_columns, uint32_t* row_counts,
                          double* kernel_ms);
 
+/* Pairwise differences of a row sample (pairwise_difference<association_matrix>
+ * with the row-grouped evaluator,
+ * include/clotho/data_spaces/analysis/pairwise_difference/association_matrix_pairwise_difference.hpp:78-102):
+ * out[p] = popcount(row a xor row b) over the current population's words for
+ * the pairs a < b of rows[0..n_rows) in the given order, packed row by row
+ * (p = a n - a (a + 1) / 2 + b - a - 1; n (n - 1) / 2 values, cap at least
+ * that).  The reference passes its sample through a bitset (unique rows in
+ * increasing order); the C++ adapter does the same before calling.  At most
+ * CLOTHO_MAX_PAIRWISE_ROWS rows.  kernel_ms (may be null) receives the kernel time. */
+#define CLOTHO_MAX_PAIRWISE_ROWS 16384
+int clotho_pairwise_differences(clotho_ctx* ctx, const uint32_t* rows, uint32_t n_rows,
+                                uint32_t* out, size_t cap, double* kernel_ms);
+
 /* Device memory held by the context (both populations, allele table,
  * free-space and selection buffers, scratch), in bytes.  No reference
  * counterpart (the reference's footprint is its std::vector/new[] storage). */

@@ -534,6 +534,55 @@ private:
     double m_ms = 0;
 };
 
+// pairwise_difference<association_matrix> (include/clotho/data_spaces/analysis/
+// pairwise_difference/association_matrix_pairwise_difference.hpp:40-104): the
+// sample goes through a bitset (unique rows, increasing order), every pair
+// a < b contributes popcount(row a xor row b), and recordResults reports min,
+// max, mean, median and variance (exact median and population variance, as
+// allele_frequency::summarize).
+class pairwise_difference {
+public:
+    template <class Iterator>
+    void evaluate(device_population& pop, Iterator first, Iterator last) {
+        const uint32_t H = pop.shape().haplotypes;
+        std::vector<bool> in(H, false);
+        for (; first != last; ++first)
+            if ((size_t)*first < H) in[*first] = true;
+        std::vector<uint32_t> rows;
+        for (uint32_t i = 0; i < H; ++i)
+            if (in[i]) rows.push_back(i);
+        run(pop, rows);
+    }
+    void evaluate(device_population& pop) {
+        std::vector<uint32_t> rows(pop.shape().haplotypes);
+        for (uint32_t i = 0; i < rows.size(); ++i) rows[i] = i;
+        run(pop, rows);
+    }
+    const std::vector<uint32_t>& differences() const { return m_diff; }
+    double kernel_ms() const { return m_ms; }
+    allele_frequency::summary results() const { return allele_frequency::summarize(m_diff); }
+    std::string stats_json() const {
+        const allele_frequency::summary r = results();
+        char buf[256];
+        std::snprintf(buf, sizeof(buf),
+                      "{\"min\": %.17g, \"max\": %.17g, \"mean\": %.17g, \"median\": %.17g, "
+                      "\"variance\": %.17g}",
+                      r.min, r.max, r.mean, r.median, r.variance);
+        return buf;
+    }
+
+private:
+    void run(device_population& pop, const std::vector<uint32_t>& rows) {
+        const size_t n = rows.size();
+        m_diff.assign(n > 1 ? n * (n - 1) / 2 : 0, 0);
+        check(clotho_pairwise_differences(pop.ctx(), rows.data(), (uint32_t)n, m_diff.data(),
+                                          m_diff.size(), &m_ms),
+              pop.ctx(), "pairwise_differences");
+    }
+    std::vector<uint32_t> m_diff;
+    double m_ms = 0;
+};
+
 // ---------------------------------------------------------------------------
 struct mi355x_pipeline {};
 

@@ -78,6 +78,7 @@ def lib():
     L.or_sim_compact.argtypes = [C.c_void_p]
     L.or_allele_counts.argtypes = [u64p, C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, u32p]
     L.or_sim_compact.restype = C.c_int
+    L.or_pairwise_differences.argtypes = [u64p, C.c_uint32, u32p, C.c_uint32, u32p]
     L.or_sim_shape.argtypes = [C.c_void_p, C.c_void_p]
     L.or_sim_rows.argtypes = [C.c_void_p, u64p]
     L.or_sim_alleles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -187,6 +188,17 @@ def select(fitness, seed, generation, replicate=0):
     out = np.zeros(2 * fitness.size, np.uint32)
     lib().or_select(fitness, fitness.size, seed, generation, replicate, out)
     return out
+
+
+def pairwise_differences(rows, sample=None):
+    """popcount(row a xor row b) for the sample pairs a < b, packed row by row."""
+    rows = np.ascontiguousarray(rows, np.uint64)
+    h, w = rows.shape
+    sample = np.arange(h, dtype=np.uint32) if sample is None else np.ascontiguousarray(sample, np.uint32)
+    n = sample.size
+    out = np.zeros(max(1, n * (n - 1) // 2), np.uint32)
+    lib().or_pairwise_differences(rows, w, sample, n, out)
+    return out[: n * (n - 1) // 2]
 
 
 def allele_counts(rows, alleles, sample=None):

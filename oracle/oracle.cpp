@@ -704,6 +704,25 @@ extern "C" void or_allele_counts(const uint64_t* rows, uint32_t W, uint32_t A,
     }
 }
 
+// pairwise_difference<association_matrix>::eval with the row-grouped
+// evaluator (include/clotho/data_spaces/analysis/pairwise_difference/association_matrix_pairwise_difference.hpp:78-102,
+// row_grouped_pairwise_difference.hpp:25-55): for the sample pairs a < b in
+// order, the set bits of row a xor row b over W words, counted one bit at a
+// time.
+extern "C" void or_pairwise_differences(const uint64_t* rows, uint32_t W, const uint32_t* sample,
+                                        uint32_t n, uint32_t* out) {
+    size_t p = 0;
+    for (uint32_t a = 0; a < n; ++a)
+        for (uint32_t b = a + 1; b < n; ++b) {
+            const uint64_t* ra = rows + (size_t)sample[a] * W;
+            const uint64_t* rb = rows + (size_t)sample[b] * W;
+            uint32_t d = 0;
+            for (uint32_t w = 0; w < W; ++w)
+                for (uint64_t x = ra[w] ^ rb[w]; x; x &= x - 1) ++d;
+            out[p++] = d;
+        }
+}
+
 extern "C" int or_sim_compact(or_sim* s);
 
 // Offspring population size of the following generations: the pN the

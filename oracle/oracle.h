@@ -73,6 +73,9 @@ int or_sim_compact(or_sim* s);
 /* Column and row margins of rows[sample[i]] (i < n), W words per row, slots < A. */
 void or_allele_counts(const uint64_t* rows, uint32_t W, uint32_t A, const uint32_t* sample,
                       uint32_t n, uint32_t* column_counts, uint32_t* row_counts);
+/* popcount(row a xor row b) for the sample pairs a < b, packed row by row. */
+void or_pairwise_differences(const uint64_t* rows, uint32_t W, const uint32_t* sample, uint32_t n,
+                             uint32_t* out);
 void or_sim_shape(const or_sim* s, clotho_shape* out);
 void or_sim_rows(const or_sim* s, uint64_t* out);
 void or_sim_alleles(const or_sim* s, double* positions, double* effects, uint32_t* ages);

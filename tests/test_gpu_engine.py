@@ -134,3 +134,29 @@ def test_engine_demo_intermediates(graph):
     assert out["generation"] == s.generation and out["alleles"] == s.alleles
     assert out["free"] == s.free_count and out["fixed"] == s.fixed_count
     assert out["rows_fnv1a"] == fnv1a(orc.rows())
+
+
+def test_engine_demo_pairwise_sample(tmp_path):
+    """log.sampling_size entries with pairwise = true add the sample's
+    pairwise-difference statistics to each .status record (the reference's
+    sample_log_params); they match the oracle on the same sample rows."""
+    prefix = str(tmp_path / "pw")
+    args = [build.DEMO, "population_growth.params.B=150", "mutation.mu=3",
+            "recombination.rho=2", "random_number.seed=5", "log.period=2",
+            'log.sampling_size=[{"size": "60", "pairwise": "true"}, {"size": "20"}]',
+            "--generations", "2", "--log-prefix", prefix]
+    subprocess.check_call(args, stdout=subprocess.DEVNULL)
+    cfg = api.make_config(150, 3.0, 2.0, seed=5, traits=1, neutral_p=1.0, max_alleles=32768)
+    orc = oracle.OracleSim(cfg)
+    orc.step()
+    orc.step()
+    st = json.load(open(f"{prefix}.0.status.json"))
+    s = orc.shape()
+    sample = np.unique(api.sample_rows(5, s.generation, 0, s.haplotypes, 60))
+    d = np.sort(oracle.pairwise_differences(orc.rows(), sample)).astype(np.float64)
+    pd = st["sample"]["0"]["pairwise_difference"]
+    assert float(pd["min"]) == d[0] and float(pd["max"]) == d[-1]
+    assert float(pd["mean"]) == pytest.approx(d.mean())
+    assert float(pd["median"]) == float(np.median(d))
+    assert float(pd["variance"]) == pytest.approx(d.var())
+    assert "pairwise_difference" not in st["sample"]["1"]

@@ -85,3 +85,24 @@ def test_engine_demo_statistics():
             assert d["mean"] == pytest.approx(v.mean(), rel=1e-12)
             assert d["median"] == pytest.approx(np.median(v), rel=1e-12)
             assert d["variance"] == pytest.approx(v.var(), rel=1e-9)
+
+
+def test_pairwise_differences_match_oracle():
+    """Device pairwise differences against the oracle on a simulated
+    population: all rows, an unordered sample, rows spanning several 64-word
+    chunks, the empty cases, and an out-of-range row."""
+    cfg = api.make_config(200, 6.0, 2.0, seed=31, traits=1, neutral_p=0.5, sigma=0.2,
+                          fitness=api.FIT_GAUSSIAN_FIXED, max_alleles=1 << 16)
+    sim = api.Simulation(cfg)
+    sim.run(12)
+    rows = sim.rows()
+    assert rows.shape[1] > 64
+    d = sim.pairwise_differences(np.arange(rows.shape[0]))
+    assert np.array_equal(d, oracle.pairwise_differences(rows))
+    sample = np.random.default_rng(2).permutation(rows.shape[0])[:77].astype(np.uint32)
+    d, ms = sim.pairwise_differences(sample, timed=True)
+    assert np.array_equal(d, oracle.pairwise_differences(rows, sample)) and ms > 0
+    assert sim.pairwise_differences(np.array([3], np.uint32)).size == 0
+    assert sim.pairwise_differences(np.array([], np.uint32)).size == 0
+    with pytest.raises(api.ClothoError, match="out of range"):
+        sim.pairwise_differences(np.array([0, rows.shape[0]], np.uint32))

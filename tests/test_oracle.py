@@ -527,3 +527,17 @@ def test_oracle_intermediate_generations():
     with pytest.raises(ValueError):  # no compaction with intermediate generations
         oracle.OracleSim(api.make_config(100, 1.0, 1.0, intermediates=1, compact=True,
                                          max_alleles=1 << 12))
+
+
+def test_oracle_pairwise_differences_against_numpy():
+    """popcount(row a xor row b) for the pairs a < b of a sample, packed row by
+    row, against numpy's bit unpacking; empty and single-row samples give no pairs."""
+    rng = np.random.default_rng(21)
+    rows = rng.integers(0, 2**63, (40, 24), dtype=np.uint64) ^ rng.integers(0, 2**63, (40, 24),
+                                                                            dtype=np.uint64)
+    bits = np.unpackbits(rows.view(np.uint8), axis=1)
+    for sample in (np.arange(40), rng.permutation(40)[:17], np.array([5, 3]), np.array([7]),
+                   np.array([], dtype=np.int64)):
+        d = oracle.pairwise_differences(rows, sample)
+        ref = [int((bits[a] != bits[b]).sum()) for i, a in enumerate(sample) for b in sample[i + 1:]]
+        assert d.tolist() == ref
