@@ -44,6 +44,8 @@ def test_invalid_configs_rejected_before_device():
         api.make_config(10, 0.0, 1.0, fitness=api.FIT_GAUSSIAN),  # sigma = 0
         api.make_config(10, 1.0, 1.0, quantum_log2=3),
         api.make_config(10, 1.0, 1.0, max_alleles=0),
+        api.make_config(10, 1.0, 1.0, intermediates=2, compact=True),  # batch consumes the list
+        api.make_config(10, 1.0, 1.0, max_individuals=(1 << 28) + 1),
     ]
     for cfg in bad:
         assert L.clotho_create(C.addressof(cfg), 0, C.byref(ctx)) == -1
