@@ -265,6 +265,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     if (hipGetDeviceProperties(&c->prop, device) != hipSuccess) return bail(CLOTHO_ERR_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(CLOTHO_ERR_HIP);
+    if (query_trait_residency(c) != hipSuccess) return bail(CLOTHO_ERR_HIP);
 
     Params& P = c->p;
     std::memset(&P, 0, sizeof(P));

@@ -1697,18 +1697,40 @@ struct PackI8 {
     }
 };
 
-template <int RT, int NT>
+// Up to RT row tiles of 16 rows per wave.  kEven: P.trait_blocks blocks
+// share the row tiles evenly (i8::wave_tiles_even), the grid a multiple of
+// the compute-unit count, and a wave one tile short runs the RT - 1 body;
+// otherwise RT tiles per wave in block order, the grid following from Params
+// alone (as the batched launches need).
+template <int RT, int NT, bool kEven>
 struct TraitsI8 {
     static constexpr int kThreads = 256;
+    static constexpr uint32_t kRowTiles = RT;
     __host__ __device__ static uint32_t blocks(const Params& P) {
-        return P.traits ? (P.h + 64 * RT - 1) / (64 * RT) : 0u;
+        if (!P.traits) return 0u;
+        return kEven ? P.trait_blocks : (P.h + 64 * RT - 1) / (64 * RT);
     }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
-        i8::traits_i8_block<RT, NT>(B.rows[sel], P.h, &B.state->wc, P.traits, B.digits,
-                                    B.traits[sel], blk);
+        const uint32_t wave = threadIdx.x >> 6;
+        const i8::WaveTiles wt = kEven ? i8::wave_tiles_even(blk, wave, P.h, P.trait_blocks)
+                                       : i8::wave_tiles_fixed(blk, wave, RT);
+        i8::traits_i8_block<RT, NT, kEven>(B.rows[sel], P.h, &B.state->wc, P.traits, B.digits,
+                                           B.traits[sel], wt);
     }
 };
+
+// Grid of an even traits launch over h rows: the fewest blocks that keep
+// every wave at RT tiles or fewer, rounded up to a multiple of the compute
+// units once there are more blocks than units; 0 when that grid exceeds one
+// round of `resident` blocks per unit (the fixed-tile kernel is used then).
+uint32_t even_trait_blocks(uint32_t h, uint32_t rt, uint32_t cus, uint32_t resident) {
+    const uint32_t tiles = (h + 15u) / 16u;
+    const uint32_t bmin = (tiles + 4u * rt - 1u) / (4u * rt);
+    if (bmin <= cus) return bmin;
+    const uint32_t b = (bmin + cus - 1u) / cus * cus;
+    return b <= resident * cus ? b : 0u;
+}
 
 // ---------------------------------------------------------------------------
 // Kernels.  Single population: one block per unit of Op::blocks, arguments by
@@ -1751,11 +1773,13 @@ using BatchRecombineWide = Recombine<true, 1>;
 CLOTHO_SINGLE(k_recombine, RecombineNarrow)
 CLOTHO_SINGLE(k_recombine_wide, RecombineWide)
 CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
-using Traits4 = TraitsI8<4, 1>;   // T <= 4
-using Traits8 = TraitsI8<2, 2>;   // T <= 8
-using Traits16 = TraitsI8<2, 4>;  // T <= 16
-using Traits32 = TraitsI8<1, 8>;  // T <= 32
-CLOTHO_SINGLE(k_traits_i8_pop, Traits4)
+using Traits4Even = TraitsI8<5, 1, true>;  // T <= 4, grid within one round of resident blocks
+using Traits4 = TraitsI8<4, 1, false>;      // T <= 4
+using Traits8 = TraitsI8<2, 2, false>;      // T <= 8
+using Traits16 = TraitsI8<2, 4, false>;     // T <= 16
+using Traits32 = TraitsI8<1, 8, false>;     // T <= 32
+CLOTHO_SINGLE(k_traits_i8_pop, Traits4Even)
+CLOTHO_SINGLE(k_traits_i8_pop_fixed, Traits4)
 CLOTHO_SINGLE(k_traits_i8_pop_t8, Traits8)
 CLOTHO_SINGLE(k_traits_i8_pop_t16, Traits16)
 CLOTHO_SINGLE(k_traits_i8_pop_t32, Traits32)
@@ -1852,10 +1876,26 @@ static hipError_t launch_traits_of(clotho_ctx* c, int which) {
     const uint32_t sel = (uint32_t)which;
     TRY_LAUNCH(launch_single<PackI8>(k_pack_i8_pop, c, sel));
     const uint32_t nt = (c->p.traits + 3) / 4;
-    if (nt <= 1) return launch_single<Traits4>(k_traits_i8_pop, c, sel);
+    if (nt <= 1) {
+        // one round of the even kernel where it fits, else the fixed tiles
+        if (!c->trait_resident) TRY_LAUNCH(query_trait_residency(c));
+        const uint32_t cus = (uint32_t)std::max(1, c->prop.multiProcessorCount);
+        c->p.trait_blocks = even_trait_blocks(c->p.h, Traits4Even::kRowTiles, cus, c->trait_resident);
+        if (c->p.trait_blocks) return launch_single<Traits4Even>(k_traits_i8_pop, c, sel);
+        return launch_single<Traits4>(k_traits_i8_pop_fixed, c, sel);
+    }
     if (nt <= 2) return launch_single<Traits8>(k_traits_i8_pop_t8, c, sel);
     if (nt <= 4) return launch_single<Traits16>(k_traits_i8_pop_t16, c, sel);
     return launch_single<Traits32>(k_traits_i8_pop_t32, c, sel);
+}
+
+hipError_t query_trait_residency(clotho_ctx* c) {
+    int per_cu = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_traits_i8_pop), Traits4Even::kThreads, 0);
+    if (e != hipSuccess) return e;
+    c->trait_resident = (uint32_t)std::max(1, per_cu);
+    return hipSuccess;
 }
 
 hipError_t launch_recombine(clotho_ctx* c) {

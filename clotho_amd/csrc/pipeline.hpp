@@ -64,6 +64,7 @@ struct Params {
     uint32_t mut_blocks;   // scan blocks over gametes
     uint32_t word_blocks;  // 64-word free-space blocks at capacity
     uint32_t max_chunks;   // row-chunk capacity of the partial buffers
+    uint32_t trait_blocks; // grid of the single-population traits launch
 };
 
 // Device buffers.  Index [c] is the population that is current after the
@@ -173,6 +174,7 @@ struct clotho_ctx {
     hipDeviceProp_t prop;
     size_t device_bytes = 0;    // sum of the context's device allocations
     uint32_t next_n = 0;        // offspring individuals of following generations (0: keep)
+    uint32_t trait_resident = 0;  // resident blocks per unit of the even traits kernel (0: unknown)
 };
 
 namespace clotho {
@@ -184,6 +186,7 @@ hipError_t launch_free_space(clotho_ctx* c);
 hipError_t launch_fitness(clotho_ctx* c);
 hipError_t launch_commit(clotho_ctx* c);
 hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
+hipError_t query_trait_residency(clotho_ctx* c);  // sets c->trait_resident
 // Column compaction of population [src] into [1 - src] (fixed and lost
 // columns removed, slots renumbered in order); the caller flips cur.
 hipError_t launch_compact(clotho_ctx* c, int src);

@@ -81,30 +81,52 @@ __device__ __forceinline__ void pack_i8_block(const int32_t* __restrict__ eff_in
         u32x4{d[0], d[1], d[2], d[3]};
 }
 
-// Exact traits: one wave = RT row tiles of 16 rows over all K; NT N tiles.
+// Row tiles of one wave: 16-row tiles t0 .. t0 + cnt - 1 (cnt <= RT).
+struct WaveTiles {
+    uint32_t t0, cnt;
+};
+
+// RT consecutive tiles per wave, waves in block order.
+__device__ inline WaveTiles wave_tiles_fixed(uint32_t block, uint32_t wave, uint32_t RT) {
+    return WaveTiles{(block * 4 + wave) * RT, RT};
+}
+
+// The ceil(h / 16) tiles split over the 4 * blocks waves as evenly as
+// possible (counts differ by at most one), so that a grid that is a multiple
+// of the compute-unit count gives every unit the same number of rows.  The
+// caller launches blocks >= ceil(h / (64 RT)), so that no wave gets more than
+// RT tiles.
+__device__ inline WaveTiles wave_tiles_even(uint32_t block, uint32_t wave, uint32_t h,
+                                            uint32_t blocks) {
+    const uint64_t tiles = (h + 15u) / 16u, waves = 4ull * blocks, i = 4ull * block + wave;
+    const uint32_t t0 = (uint32_t)(i * tiles / waves);
+    return WaveTiles{t0, (uint32_t)((i + 1) * tiles / waves) - t0};
+}
+
+// Exact traits: one wave = up to RT row tiles of 16 rows over all K; NT N tiles.
 // The block stages the B fragments in LDS, shared by its four waves, in
 // stages of 16 KB: stage s of group G holds q-blocks s QS .. s QS + QS - 1
 // (QS = 16 / NT) of all NT tiles, double buffered and moved by direct
 // global-to-LDS loads while the previous stage is multiplied.  A group's rows
 // are loaded at its first stage for the next group.  Bits expand to int8 0/1
 // through a 256-entry LDS table (byte -> 8 bytes).
+//
+// traits_i8_wave is one wave's share: rows base .. base + 16 cnt - 1 (cnt <=
+// RT, base and cnt wave-uniform); s_b and s_lut are the block's tables.
 template <int RT, int NT>
-__device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows_in, uint32_t h,
-                                                const uint32_t* p_words, uint32_t T,
-                                                const uint4* __restrict__ packed_in,
-                                                int64_t* __restrict__ traits_in, uint32_t block) {
+__device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows_in, uint32_t h,
+                                               uint32_t W, uint32_t T,
+                                               const uint4* __restrict__ packed_in,
+                                               int64_t* __restrict__ traits_in, uint32_t base,
+                                               uint32_t cnt, u32x4* s_b, uint2* s_lut) {
     static_assert(NT == 1 || NT == 2 || NT == 4 || NT == 8, "N tiles: 1, 2, 4 or 8");
     constexpr int QS = 16 / NT;  // q-blocks per stage
     const auto rows = as_global(rows_in);
     const auto packed = as_global(packed_in);
     const auto traits = as_global(traits_in);
-    __shared__ u32x4 s_b[2 * 16 * 64];
-    __shared__ uint2 s_lut[256];
-    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
     const uint32_t NG = (W + kGroupWords - 1) / kGroupWords;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t m = lane & 15, g = lane >> 4;
-    const uint32_t base = (block * 4 + wave) * (16 * RT);
     {
         const uint32_t v = threadIdx.x;
         s_lut[v] = make_uint2(spread4(v & 15), spread4(v >> 4));
@@ -119,7 +141,7 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const uint32_t row = base + 16 * r + m;
-        rv[r] = row < h;
+        rv[r] = (uint32_t)r < cnt && row < h;
         rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
     }
     auto load_rows = [&](uint32_t G, u32x4 (&x)[RT][2]) {
@@ -214,11 +236,33 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
                 const int64_t c3 = __shfl_down((int)acc[r][nt][i], 3, 64);
                 const uint32_t row = base + 16 * r + 4 * g + i;
                 const uint32_t t = 4 * nt + m / 4;
-                if ((m & 3) == 0 && row < h && t < T)
+                if ((m & 3) == 0 && (uint32_t)r < cnt && row < h && t < T)
                     traits[(size_t)row * T + t] = c0 + (c1 << 8) + (c2 << 16) + (c3 << 24);
             }
         }
     }
+}
+
+// Exact traits of the rows of WaveTiles wt, one wave each.  kSplit: a wave
+// of fewer than RT tiles runs the RT - 1 body, so that it issues no lookups
+// or products for an empty tile (both bodies pass the same barriers).
+template <int RT, int NT, bool kSplit = false>
+__device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ rows, uint32_t h,
+                                                const uint32_t* p_words, uint32_t T,
+                                                const uint4* __restrict__ packed,
+                                                int64_t* __restrict__ traits, WaveTiles wt) {
+    __shared__ u32x4 s_b[2 * 16 * 64];
+    __shared__ uint2 s_lut[256];
+    const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
+    const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(wt.cnt, (uint32_t)RT));
+    const uint32_t base = 16u * (uint32_t)__builtin_amdgcn_readfirstlane((int)wt.t0);
+    if constexpr (kSplit && RT > 1) {
+        if (cnt < (uint32_t)RT) {
+            traits_i8_wave<RT - 1, NT>(rows, h, W, T, packed, traits, base, cnt, s_b, s_lut);
+            return;
+        }
+    }
+    traits_i8_wave<RT, NT>(rows, h, W, T, packed, traits, base, cnt, s_b, s_lut);
 }
 
 // Launch shape of traits_i8_block: rows per block and the (RT, NT) choice by

@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ 
                                                    const uint32_t* p_words, uint32_t T,
                                                    const uint4* __restrict__ packed,
                                                    int64_t* __restrict__ traits) {
-    i8::traits_i8_block<RT, NT>(rows, h, p_words, T, packed, traits, blockIdx.x);
+    i8::traits_i8_block<RT, NT>(rows, h, p_words, T, packed, traits,
+                                i8::wave_tiles_fixed(blockIdx.x, threadIdx.x >> 6, RT));
 }
 
 

@@ -136,3 +136,32 @@ def test_many_traits(traits):
                           sigma=0.3, fitness=api.FIT_MV_STABILIZING, fitness_scale=3.0,
                           max_alleles=1 << 13)
     run_pair(cfg, 5, check_every=1, population=_uploaded_population(160, 2500, traits, traits))
+
+
+def _dense_population(n, alleles, traits, seed):
+    """Rows of density 1/4 (AND of two random words) over `alleles` columns,
+    three of them fixed; generated word-wise so that large n stays cheap."""
+    rng = np.random.default_rng(seed)
+    words = ((alleles + 511) // 512) * 8
+    draw = lambda: rng.integers(0, np.iinfo(np.int64).max, size=(2 * n, words), dtype=np.int64).view(np.uint64)
+    rows = draw() & draw()
+    cols = np.arange(words * 64)
+    live = np.packbits((cols < alleles).astype(np.uint8), bitorder="little").view(np.uint64)
+    rows &= live[None, :]
+    for c in rng.integers(0, alleles, 3):
+        rows[:, c // 64] |= np.uint64(1) << np.uint64(c % 64)
+    pos = np.floor(rng.uniform(0, 1, alleles) * 2**32) / 2**32
+    eff = np.round(rng.normal(0, 0.2, (alleles, traits)) * 2**24) / 2**24
+    fit = rng.uniform(0.1, 1.0, n)
+    return rows, pos, eff, fit
+
+
+@pytest.mark.parametrize("n", [20001, 125000])
+def test_trait_row_grids(n):
+    # up to four traits: 40,002 rows give waves of four and of five 16-row
+    # tiles on the even grid; 250,000 rows exceed one round of resident
+    # blocks and take the fixed four-tile grid
+    cfg = api.make_config(n, 0.01, 1.0, seed=71, traits=2, neutral_p=0.2, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=1.0, max_alleles=1 << 13)
+    print(f"trait row grids: n={n}", flush=True)
+    run_pair(cfg, 2, population=_dense_population(n, 700, 2, 71))
