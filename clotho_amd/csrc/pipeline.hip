@@ -1033,7 +1033,8 @@ struct Recombine {
 constexpr uint32_t kFreeWords = 64;
 
 struct FreeReduce {
-    static constexpr int kThreads = kBlock;
+    static constexpr int kThreads = 1024;  // 16 waves share the chunk partials of 64 words
+    static constexpr int kRWaves = kThreads / 64;
     __host__ __device__ static uint32_t blocks(const Params& P) { return P.word_blocks; }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
@@ -1047,14 +1048,14 @@ struct FreeReduce {
         const auto word_cnt = gbl(B.word_cnt);
         const auto word_bsum = gbl(B.word_bsum);
         const uint32_t word_blocks = P.word_blocks;
-        __shared__ uint64_t s_or[kWaves][64];
-        __shared__ uint64_t s_and[kWaves][64];
+        __shared__ uint64_t s_or[kRWaves][64];
+        __shared__ uint64_t s_and[kRWaves][64];
         const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const uint32_t w = blk * kFreeWords + lane;
         const uint32_t wc = st->wc, S = st->all_size;
         uint64_t vo = 0, va = ~(uint64_t)0;
         if (w < wc) {
-            for (uint32_t c = wave; c < chunks; c += kWaves) {
+            for (uint32_t c = wave; c < chunks; c += kRWaves) {
                 vo |= var_part[(size_t)c * P.w_cap + w];
                 va &= fix_part[(size_t)c * P.w_cap + w];
             }
@@ -1063,7 +1064,7 @@ struct FreeReduce {
         s_and[wave][lane] = va;
         __syncthreads();
         if (wave) return;  // no barrier below
-        for (int q = 1; q < kWaves; ++q) {
+        for (int q = 1; q < kRWaves; ++q) {
             vo |= s_or[q][lane];
             va &= s_and[q][lane];
         }
