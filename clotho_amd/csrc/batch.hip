@@ -217,9 +217,10 @@ int clotho_batch_run(clotho_batch* b, uint32_t n, int mode, double* elapsed_ms) 
     tm.generations = 0;
     const bool phase_timing = tm.enabled && mode == CLOTHO_BATCH_EAGER;
     std::vector<std::vector<hipEvent_t>> marks;
+    EventSet events;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    BATCH_TRY(b, hipEventCreate(&e0));
-    BATCH_TRY(b, hipEventCreate(&e1));
+    BATCH_TRY(b, events.add(&e0));
+    BATCH_TRY(b, events.add(&e1));
     BATCH_TRY(b, hipEventRecord(e0, b->stream));
     for (uint32_t g = 0; g < n; ++g) {
         const uint32_t par = (uint32_t)b->cur;
@@ -229,7 +230,7 @@ int clotho_batch_run(clotho_batch* b, uint32_t n, int mode, double* elapsed_ms) 
             hipEvent_t* mk = nullptr;
             if (phase_timing) {
                 marks.emplace_back(6);
-                for (hipEvent_t& e : marks.back()) BATCH_TRY(b, hipEventCreate(&e));
+                for (hipEvent_t& e : marks.back()) BATCH_TRY(b, events.add(&e));
                 mk = marks.back().data();
             }
             BATCH_TRY(b, launch_batch_generation(b->L, par, b->stream, mk));
@@ -243,8 +244,6 @@ int clotho_batch_run(clotho_batch* b, uint32_t n, int mode, double* elapsed_ms) 
     BATCH_TRY(b, hipEventSynchronize(e1));
     float ms = 0.f;
     BATCH_TRY(b, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (elapsed_ms) *elapsed_ms = ms;
     if (phase_timing) {
         for (auto& mk : marks) {
@@ -255,7 +254,6 @@ int clotho_batch_run(clotho_batch* b, uint32_t n, int mode, double* elapsed_ms) 
                 BATCH_TRY(b, hipEventElapsedTime(&t, mk[q], mk[q + 1]));
                 tm.ms[1 + q] += t;
             }
-            for (hipEvent_t& e : mk) (void)hipEventDestroy(e);
         }
         tm.generations = (int)n;
     }

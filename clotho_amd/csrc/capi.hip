@@ -791,9 +791,9 @@ int clotho_pairwise_differences(clotho_ctx* c, const uint32_t* rows, uint32_t n_
     step(hipEventCreate(&e1));
     if (e == hipSuccess)
         step(hipMemcpyAsync(d_sample, rows, 4ull * n_rows, hipMemcpyHostToDevice, c->stream));
-    step(hipEventRecord(e0, c->stream));
+    if (e == hipSuccess) step(hipEventRecord(e0, c->stream));
     if (e == hipSuccess) step(launch_pairwise(c, c->b.rows[c->cur], W, d_sample, n_rows, d_out));
-    step(hipEventRecord(e1, c->stream));
+    if (e == hipSuccess) step(hipEventRecord(e1, c->stream));
     if (e == hipSuccess)
         step(hipMemcpyAsync(out, d_out, 4ull * pairs, hipMemcpyDeviceToHost, c->stream));
     step(hipStreamSynchronize(c->stream));
@@ -851,10 +851,11 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
         end_generation(c);  // steady sizes: parent and offspring of equal size
         int rc = capture_graphs(c);
         if (rc) return rc;
+        EventSet events;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (tm.enabled) {
-            HIP_TRY(c, hipEventCreate(&e0));
-            HIP_TRY(c, hipEventCreate(&e1));
+            HIP_TRY(c, events.add(&e0));
+            HIP_TRY(c, events.add(&e1));
             HIP_TRY(c, hipEventRecord(e0, c->stream));
         }
         for (uint32_t g = 0; g < n; ++g) {
@@ -868,16 +869,15 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
             HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
             tm.ms[0] = ms;
             tm.generations = (int)n;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
         }
     } else {
+        EventSet events;
         std::vector<std::vector<hipEvent_t>> all;
         for (uint32_t g = 0; g < n; ++g) {
             std::vector<hipEvent_t>* mk = nullptr;
             if (tm.enabled) {
                 all.emplace_back(6);
-                for (hipEvent_t& e : all.back()) HIP_TRY(c, hipEventCreate(&e));
+                for (hipEvent_t& e : all.back()) HIP_TRY(c, events.add(&e));
                 mk = &all.back();
             }
             begin_generation(c);
@@ -896,7 +896,6 @@ int clotho_run(clotho_ctx* c, uint32_t n, int use_graph) {
                     HIP_TRY(c, hipEventElapsedTime(&t, mk[q], mk[q + 1]));
                     tm.ms[1 + q] += t;
                 }
-                for (hipEvent_t& e : mk) (void)hipEventDestroy(e);
             }
             tm.generations = (int)n;
         }
@@ -949,8 +948,9 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
     // two passes over the candidates, each candidate keeping its faster pass.
     // Launches longer than 2 ms are timed once, in one pass.
     constexpr int kReps = 3;
+    EventSet events;
     hipEvent_t ev[2 * kReps] = {};
-    for (hipEvent_t& e : ev) HIP_TRY(c, hipEventCreate(&e));
+    for (hipEvent_t& e : ev) HIP_TRY(c, events.add(&e));
     int reps = kReps, passes = 2;
     {
         HIP_TRY(c, launch_recombine(c));
@@ -984,7 +984,6 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
             best_ms[i] = std::min(best_ms[i], tot);
         }
     }
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     float best = 1e30f;
     uint32_t best_chunks = base;
     for (size_t i = 0; i < cand.size(); ++i) {

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/clotho_amd.h"
 
@@ -151,6 +152,23 @@ struct Timing {
     bool enabled = false;
     double ms[6] = {0, 0, 0, 0, 0, 0};
     int generations = 0;
+};
+
+// Events of a timed section, destroyed on every return path.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    EventSet() = default;
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+    ~EventSet() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t add(hipEvent_t* out) {
+        *out = nullptr;
+        const hipError_t r = hipEventCreate(out);
+        if (r == hipSuccess) ev.push_back(*out);
+        return r;
+    }
 };
 
 struct GraphPair {
