@@ -1777,7 +1777,8 @@ CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
 using Traits4Even = TraitsI8<5, 1, true>;  // T <= 4, grid within one round of resident blocks
 using Traits4 = TraitsI8<4, 1, false>;      // T <= 4
 using Traits8 = TraitsI8<2, 2, false>;      // T <= 8
-using Traits16 = TraitsI8<2, 4, false>;     // T <= 16
+using Traits16 = TraitsI8<4, 4, false>;     // T <= 16 (four row tiles: 5% faster than two at H = 1e6)
+using BatchTraits16 = TraitsI8<2, 4, false>;  // batched T <= 16: the rows per block of Traits8
 using Traits32 = TraitsI8<1, 8, false>;     // T <= 32
 CLOTHO_SINGLE(k_traits_i8_pop, Traits4Even)
 CLOTHO_SINGLE(k_traits_i8_pop_fixed, Traits4)
@@ -1948,7 +1949,7 @@ void batch_op_blocks(const Params& P, uint32_t* out) {
     out[kOpPackI8] = PackI8::blocks(P);
     const uint32_t nt = (P.traits + 3) / 4;
     out[kOpTraits] = nt <= 1 ? Traits4::blocks(P)
-                   : nt <= 4 ? Traits8::blocks(P)  // same rows per block as Traits16
+                   : nt <= 4 ? Traits8::blocks(P)  // same rows per block as BatchTraits16
                              : Traits32::blocks(P);
     out[kOpFreeReduce] = FreeReduce::blocks(P);
     out[kOpFreeCompact] = FreeCompact::blocks(P);
@@ -2003,7 +2004,7 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
         else if (nt <= 2)
             TRY_LAUNCH(launch_batched<Traits8>(L, kOpTraits, chi, s));
         else if (nt <= 4)
-            TRY_LAUNCH(launch_batched<Traits16>(L, kOpTraits, chi, s));
+            TRY_LAUNCH(launch_batched<BatchTraits16>(L, kOpTraits, chi, s));
         else
             TRY_LAUNCH(launch_batched<Traits32>(L, kOpTraits, chi, s));
     }
