@@ -265,11 +265,5 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
     traits_i8_wave<RT, NT>(rows, h, W, T, packed, traits, base, cnt, s_b, s_lut);
 }
 
-// Launch shape of traits_i8_block: rows per block and the (RT, NT) choice by
-// trait count.
-__host__ __device__ inline uint32_t traits_i8_rows_per_block(uint32_t ntiles) {
-    return ntiles <= 1 ? 256u : ntiles <= 4 ? 128u : 64u;
-}
-
 }  // namespace i8
 }  // namespace clotho
