@@ -702,17 +702,16 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
                 for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
-                uint32_t sk[kEventGroup][4];
+                uint32_t sk[kEventGroup][3];
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) sk[i][q] = s_sorted[(p[i] + q) * 64 + lane];
+                    for (int q = 0; q < 3; ++q) sk[i][q] = s_sorted[(p[i] + q) * 64 + lane];
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i) {
                     const bool real = j * FW + x + i < ne;
-                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u) +
-                                       (sk[i][2] < key[i] ? 1u : 0u);
-                    more[i] = real && sk[i][3] < key[i];
+                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u);
+                    more[i] = real && sk[i][2] < key[i];
                     p[i] = real ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
                 }
 #pragma unroll
@@ -926,23 +925,22 @@ struct Recombine {
                     for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
 #pragma unroll
                     for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
-                    uint32_t sk[kEventGroup][4];
+                    uint32_t sk[kEventGroup][3];
 #pragma unroll
                     for (int i = 0; i < kEventGroup; ++i)
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
+                        for (int j = 0; j < 3; ++j) sk[i][j] = s_sorted[(p[i] + j) * 64 + lane];
 #pragma unroll
                     for (int i = 0; i < kEventGroup; ++i) {
                         const bool live = e + i < ne;
-                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u) +
-                                           (sk[i][2] < key[i] ? 1u : 0u);
-                        more[i] = live && sk[i][3] < key[i];
+                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u);
+                        more[i] = live && sk[i][2] < key[i];
                         p[i] = live ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
                     }
                     bool any_more = false;
 #pragma unroll
                     for (int i = 0; i < kEventGroup; ++i) any_more = any_more || more[i];
-                    if (any_more) {  // four or more keys of a bucket below an event: rare
+                    if (any_more) {  // three or more keys of a bucket below an event: rare
 #pragma unroll
                         for (int i = 0; i < kEventGroup; ++i)
                             if (more[i])

@@ -165,3 +165,14 @@ def test_trait_row_grids(n):
                           fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=1.0, max_alleles=1 << 13)
     print(f"trait row grids: n={n}", flush=True)
     run_pair(cfg, 2, population=_dense_population(n, 700, 2, 71))
+
+
+def test_clustered_positions():
+    # every standing position inside two 2^-8 buckets: most keys of a word
+    # share an event's bucket, so the lookup's walk past the compared keys runs
+    rows, pos, eff, fit = _uploaded_population(200, 1500, 1, 83)
+    rng = np.random.default_rng(83)
+    pos = np.floor((0.5 + rng.uniform(-2.0**-8, 2.0**-8, pos.size)) * 2**32) / 2**32
+    cfg = api.make_config(200, 1.0, 25.0, seed=83, traits=1, neutral_p=0.3, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=1.0, max_alleles=1 << 13)
+    run_pair(cfg, 6, population=(rows, pos, eff, fit))
