@@ -697,11 +697,10 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i)
                     key[i] = (uint32_t)__shfl((int)kv, (int)(f * FW + ((x + i) & (FW - 1))), 64);
-                uint32_t tb[kEventGroup];
+                const uint8_t* sb = reinterpret_cast<const uint8_t*>(s_bucket);
 #pragma unroll
-                for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                for (int i = 0; i < kEventGroup; ++i)
+                    p[i] = sb[lane * 4u + (((key[i] >> 26) << 8) | ((key[i] >> 24) & 3u))];
                 uint32_t sk[kEventGroup][3];
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i)
@@ -920,11 +919,11 @@ struct Recombine {
                         const uint32_t a = lane_read(lo, x & 63);
                         key[i] = (kWideEvents && x >= 64) ? lane_read(hi, x & 63) : a;
                     }
-                    uint32_t tb[kEventGroup];
+                    // byte (key >> 24) & 3 of dword [key >> 26][lane]: one byte read
+                    const uint8_t* sb = reinterpret_cast<const uint8_t*>(s_bucket);
 #pragma unroll
-                    for (int i = 0; i < kEventGroup; ++i) tb[i] = s_bucket[(key[i] >> 26) * 64 + lane];
-#pragma unroll
-                    for (int i = 0; i < kEventGroup; ++i) p[i] = (tb[i] >> ((key[i] >> 21) & 24u)) & 0xFFu;
+                    for (int i = 0; i < kEventGroup; ++i)
+                        p[i] = sb[lane * 4u + (((key[i] >> 26) << 8) | ((key[i] >> 24) & 3u))];
                     uint32_t sk[kEventGroup][3];
 #pragma unroll
                     for (int i = 0; i < kEventGroup; ++i)
