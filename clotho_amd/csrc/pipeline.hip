@@ -682,8 +682,8 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
         const uint32_t ne = live ? nev_in[r] : 0u;
         const uint32_t nm = live ? count[r] : 0u;
         const uint32_t k0 = local[r] + boff[r / kBlock];
-        const uint64_t t = parent[(size_t)top * wp + col] & keep;
-        const uint64_t b = parent[(size_t)(top ^ 1u) * wp + col] & keep;
+        const uint64_t t = parent[(size_t)top * wp + col];
+        const uint64_t b = parent[(size_t)(top ^ 1u) * wp + col];
         const auto ev = events + (size_t)r * ev_cap;
         uint32_t kv = ev[min(w, ev_cap - 1)];
 
@@ -723,7 +723,7 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
                 m ^= xm;
             }
         }
-        uint64_t c = (t & ~m) | (b & m);
+        uint64_t c = ((t & ~m) | (b & m)) & keep;
         const uint32_t max_nm = wave_max(nm);
         for (uint32_t k = 0; k < max_nm; ++k) {
             if (k < nm) {
@@ -950,7 +950,8 @@ struct Recombine {
                     for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
                     m ^= xm;
                 }
-                uint64_t c = (t & ~m) | (b & m);  // t, b already cleared by keep
+                // keep clears fixed columns and columns past the parent width in both
+                uint64_t c = ((t & ~m) | (b & m)) & keep;
                 // new mutations of this row (free columns: zero in every parent row)
                 const uint32_t nm = nx >> 8;
                 if (nm) {
@@ -978,13 +979,12 @@ struct Recombine {
             // each child takes the loaded pair in its own strand order
             auto process_item = [&](const Item& it) {
                 const uint32_t top0 = lane_read(g_top, it.q);
-                const uint64_t t0 = it.t & keep, b0 = it.b & keep;
 #pragma unroll
                 for (int k = 0; k < kSibs; ++k) {
                     if ((uint32_t)k < it.len) {
                         const uint32_t q = it.q + k;
                         const bool flip = lane_read(g_top, q) != top0;
-                        child_row(q, t0, b0, it.lo[k], kWideEvents ? it.hi[k] : 0u, flip);
+                        child_row(q, it.t, it.b, it.lo[k], kWideEvents ? it.hi[k] : 0u, flip);
                     }
                 }
             };
