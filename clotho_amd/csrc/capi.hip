@@ -952,6 +952,7 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
     hipEvent_t ev[2 * kReps] = {};
     for (hipEvent_t& e : ev) HIP_TRY(c, events.add(&e));
     int reps = kReps, passes = 2;
+    HIP_TRY(c, launch_word_sort(c));  // order statistics of the resident columns
     {
         HIP_TRY(c, launch_recombine(c));
         HIP_TRY(c, hipEventRecord(ev[0], c->stream));

@@ -1730,6 +1730,26 @@ uint32_t even_trait_blocks(uint32_t h, uint32_t rt, uint32_t cus, uint32_t resid
     return b <= resident * cus ? b : 0u;
 }
 
+// Two independent ops in one launch: blocks [0, A::blocks) run A, the rest
+// run Bo (neither reads what the other writes).  One launch instead of two
+// saves the fixed cost of a dependent kernel in the sequence.
+template <class A, class Bo>
+struct Fused {
+    static_assert(A::kThreads == Bo::kThreads, "fused ops share the block size");
+    static constexpr int kThreads = A::kThreads;
+    __host__ __device__ static uint32_t blocks(const Params& P) {
+        return A::blocks(P) + Bo::blocks(P);
+    }
+    __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
+                                               uint32_t blk) {
+        const uint32_t na = A::blocks(P);
+        if (blk < na)
+            A::run(P, B, sel, blk);
+        else
+            Bo::run(P, B, sel, blk - na);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Kernels.  Single population: one block per unit of Op::blocks, arguments by
 // value.  Batched: a persistent grid walks the concatenated block ranges of
@@ -1753,17 +1773,21 @@ struct MinWaves<Recombine<W, K>> {
         Op::run(P, B, sel, blockIdx.x);                                                         \
     }
 
-CLOTHO_SINGLE(k_mut_counts, MutCounts)
-CLOTHO_SINGLE(k_fitness_stats, FitnessStats)
 CLOTHO_SINGLE(k_prepare, Prepare)
-CLOTHO_SINGLE(k_mut_attrs, MutAttrs)
-CLOTHO_SINGLE(k_sel_scan, SelScan)
 CLOTHO_SINGLE(k_sel_blockscan, SelBlockscan)
-CLOTHO_SINGLE(k_gametes, Gametes)
 CLOTHO_SINGLE(k_parent_scan, ParentScan)
 CLOTHO_SINGLE(k_parent_blockscan, ParentBlockscan)
 CLOTHO_SINGLE(k_parent_scatter, ParentScatter)
 CLOTHO_SINGLE(k_word_sort, WordSort)
+// the draw phase of a single population: counts and fitness statistics, then
+// attributes and selection scan, then gametes and the per-word order
+// statistics, each pair in one launch
+using DrawCounts = Fused<MutCounts, FitnessStats>;
+using DrawAttrs = Fused<MutAttrs, SelScan>;
+using DrawGametes = Fused<Gametes, WordSort>;
+CLOTHO_SINGLE(k_draw_counts, DrawCounts)
+CLOTHO_SINGLE(k_draw_attrs, DrawAttrs)
+CLOTHO_SINGLE(k_draw_gametes, DrawGametes)
 using RecombineNarrow = Recombine<false, kSibsOrdered>;  // at most 64 events per gamete
 using RecombineWide = Recombine<true, 2>;  // two sets of 128 event keys stay in registers
 using BatchRecombineNarrow = Recombine<false, 1>;
@@ -1852,15 +1876,15 @@ hipError_t launch_single(void (*kernel)(Params, Buffers, uint32_t), clotho_ctx* 
 
 }  // namespace
 
+// Draws of the generation and the per-word order statistics of the child
+// columns (k_word_sort, which needs only the new mutations' keys).
 hipError_t launch_draw(clotho_ctx* c) {
     const uint32_t par = (uint32_t)c->cur;  // parent = current population
-    TRY_LAUNCH(launch_single<MutCounts>(k_mut_counts, c, par));
-    TRY_LAUNCH(launch_single<FitnessStats>(k_fitness_stats, c, par));
+    TRY_LAUNCH(launch_single<DrawCounts>(k_draw_counts, c, par));
     TRY_LAUNCH(launch_single<Prepare>(k_prepare, c, par));
-    TRY_LAUNCH(launch_single<MutAttrs>(k_mut_attrs, c, par));
-    TRY_LAUNCH(launch_single<SelScan>(k_sel_scan, c, par));
+    TRY_LAUNCH(launch_single<DrawAttrs>(k_draw_attrs, c, par));
     TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
-    TRY_LAUNCH(launch_single<Gametes>(k_gametes, c, par));
+    TRY_LAUNCH(launch_single<DrawGametes>(k_draw_gametes, c, par));
     if (c->b.by_parent) {
         TRY_LAUNCH(launch_single<ParentScan>(k_parent_scan, c, par));
         TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
@@ -1897,9 +1921,12 @@ hipError_t query_trait_residency(clotho_ctx* c) {
     return hipSuccess;
 }
 
+hipError_t launch_word_sort(clotho_ctx* c) {
+    return launch_single<WordSort>(k_word_sort, c, (uint32_t)c->cur);
+}
+
 hipError_t launch_recombine(clotho_ctx* c) {
     const uint32_t par = (uint32_t)c->cur;
-    TRY_LAUNCH(launch_single<WordSort>(k_word_sort, c, par));
     if (c->p.ev_cap > 64) return launch_single<RecombineWide>(k_recombine_wide, c, par);
     return launch_single<RecombineNarrow>(k_recombine, c, par);
 }

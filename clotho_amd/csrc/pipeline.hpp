@@ -197,7 +197,8 @@ struct clotho_ctx {
 
 namespace clotho {
 // Launch sequences (pipeline.hip).
-hipError_t launch_draw(clotho_ctx* c);
+hipError_t launch_draw(clotho_ctx* c);       // draws + per-word order statistics
+hipError_t launch_word_sort(clotho_ctx* c);  // order statistics alone (autotuner replays)
 hipError_t launch_recombine(clotho_ctx* c);
 hipError_t launch_traits(clotho_ctx* c);
 hipError_t launch_free_space(clotho_ctx* c);
