@@ -168,9 +168,9 @@ int enqueue_generation(clotho_ctx* c, std::vector<hipEvent_t>* marks) {
     HIP_TRY(c, mark(2));
     HIP_TRY(c, launch_traits(c));
     HIP_TRY(c, mark(3));
-    HIP_TRY(c, launch_free_space(c));
+    HIP_TRY(c, launch_free_reduce(c));
     HIP_TRY(c, mark(4));
-    HIP_TRY(c, launch_fitness(c));
+    HIP_TRY(c, launch_compact_fitness(c));
     HIP_TRY(c, launch_commit(c));
     // the child (population 1 - cur) is compacted back into population cur
     if (c->cfg.compact_every) HIP_TRY(c, launch_compact(c, 1 - c->cur));

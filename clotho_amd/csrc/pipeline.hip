@@ -1775,7 +1775,6 @@ struct MinWaves<Recombine<W, K>> {
 
 CLOTHO_SINGLE(k_prepare, Prepare)
 CLOTHO_SINGLE(k_sel_blockscan, SelBlockscan)
-CLOTHO_SINGLE(k_parent_scan, ParentScan)
 CLOTHO_SINGLE(k_parent_blockscan, ParentBlockscan)
 CLOTHO_SINGLE(k_parent_scatter, ParentScatter)
 CLOTHO_SINGLE(k_word_sort, WordSort)
@@ -1810,6 +1809,13 @@ CLOTHO_SINGLE(k_free_reduce, FreeReduce)
 CLOTHO_SINGLE(k_free_compact, FreeCompact)
 CLOTHO_SINGLE(k_fitness, Fitness)
 CLOTHO_SINGLE(k_commit, Commit)
+// more shared launches of a single-population generation: the effect digit
+// planes (new effects are final after the draws) with the parent-order counts,
+// and the free-list compaction with the offspring fitness
+using DrawParents = Fused<ParentScan, PackI8>;
+using CompactFitness = Fused<FreeCompact, Fitness>;
+CLOTHO_SINGLE(k_draw_parents, DrawParents)
+CLOTHO_SINGLE(k_compact_fitness, CompactFitness)
 #undef CLOTHO_SINGLE
 
 // Occupancy floor of a batched kernel: the recombination kernel needs two
@@ -1876,8 +1882,9 @@ hipError_t launch_single(void (*kernel)(Params, Buffers, uint32_t), clotho_ctx* 
 
 }  // namespace
 
-// Draws of the generation and the per-word order statistics of the child
-// columns (k_word_sort, which needs only the new mutations' keys).
+// Draws of the generation, the per-word order statistics of the child columns
+// (which need only the new mutations' keys) and the effect digit planes of
+// the traits pass.
 hipError_t launch_draw(clotho_ctx* c) {
     const uint32_t par = (uint32_t)c->cur;  // parent = current population
     TRY_LAUNCH(launch_single<DrawCounts>(k_draw_counts, c, par));
@@ -1885,19 +1892,17 @@ hipError_t launch_draw(clotho_ctx* c) {
     TRY_LAUNCH(launch_single<DrawAttrs>(k_draw_attrs, c, par));
     TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
     TRY_LAUNCH(launch_single<DrawGametes>(k_draw_gametes, c, par));
-    if (c->b.by_parent) {
-        TRY_LAUNCH(launch_single<ParentScan>(k_parent_scan, c, par));
-        TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
-        TRY_LAUNCH(launch_single<ParentScatter>(k_parent_scatter, c, par));
-    }
-    return hipSuccess;
+    if (!c->b.by_parent) return launch_single<PackI8>(k_pack_i8_pop, c, par);
+    TRY_LAUNCH(launch_single<DrawParents>(k_draw_parents, c, par));
+    TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
+    return launch_single<ParentScatter>(k_parent_scatter, c, par);
 }
 
 // Exact genome traits of population [which] (width st->wc, alleles st->all_size).
-static hipError_t launch_traits_of(clotho_ctx* c, int which) {
+static hipError_t launch_traits_of(clotho_ctx* c, int which, bool pack) {
     if (!c->p.traits) return hipSuccess;
     const uint32_t sel = (uint32_t)which;
-    TRY_LAUNCH(launch_single<PackI8>(k_pack_i8_pop, c, sel));
+    if (pack) TRY_LAUNCH(launch_single<PackI8>(k_pack_i8_pop, c, sel));
     const uint32_t nt = (c->p.traits + 3) / 4;
     if (nt <= 1) {
         // one round of the even kernel where it fits, else the fixed tiles
@@ -1931,13 +1936,20 @@ hipError_t launch_recombine(clotho_ctx* c) {
     return launch_single<RecombineNarrow>(k_recombine, c, par);
 }
 
-hipError_t launch_traits(clotho_ctx* c) { return launch_traits_of(c, 1 - c->cur); }
+// the digit planes were packed by the draw phase
+hipError_t launch_traits(clotho_ctx* c) { return launch_traits_of(c, 1 - c->cur, false); }
 
 hipError_t launch_free_space(clotho_ctx* c) {
     TRY_LAUNCH(launch_single<FreeReduce>(k_free_reduce, c, 0));
     return launch_single<FreeCompact>(k_free_compact, c, 0);
 }
 
+hipError_t launch_free_reduce(clotho_ctx* c) {
+    return launch_single<FreeReduce>(k_free_reduce, c, 0);
+}
+hipError_t launch_compact_fitness(clotho_ctx* c) {
+    return launch_single<CompactFitness>(k_compact_fitness, c, (uint32_t)c->cur);
+}
 hipError_t launch_fitness(clotho_ctx* c) {
     return launch_single<Fitness>(k_fitness, c, (uint32_t)c->cur);
 }
@@ -1949,7 +1961,7 @@ hipError_t launch_population_analysis(clotho_ctx* c) {
     const Params& P = c->p;
     Buffers& B = c->b;
     const int cur = c->cur;
-    TRY_LAUNCH(launch_traits_of(c, cur));
+    TRY_LAUNCH(launch_traits_of(c, cur, true));
     hipLaunchKernelGGL(k_column_reduce, dim3((P.w_cap + kBlock - 1) / kBlock, P.chunks),
                        dim3(kBlock), 0, c->stream, P, B.state, B.rows[cur], B.var_part,
                        B.fix_part);

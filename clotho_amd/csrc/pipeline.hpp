@@ -203,6 +203,10 @@ hipError_t launch_recombine(clotho_ctx* c);
 hipError_t launch_traits(clotho_ctx* c);
 hipError_t launch_free_space(clotho_ctx* c);
 hipError_t launch_fitness(clotho_ctx* c);
+// the generation's late launches: the reduction alone, then the free-list
+// compaction and the offspring fitness in one launch
+hipError_t launch_free_reduce(clotho_ctx* c);
+hipError_t launch_compact_fitness(clotho_ctx* c);
 hipError_t launch_commit(clotho_ctx* c);
 hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
 hipError_t query_trait_residency(clotho_ctx* c);  // sets c->trait_resident
