@@ -223,7 +223,10 @@ int clotho_generation(clotho_ctx* ctx);
  * whose first generation changes the size uses eager launches. */
 int clotho_set_population_size(clotho_ctx* ctx, uint32_t n);
 
-/* Phase-level entry points, in generation order (clotho_generation = these). */
+/* Phase-level entry points, in generation order (clotho_generation = these).
+ * clotho_phase_draw also builds the per-word order statistics that
+ * clotho_phase_recombine_mutate reads and the effect digit planes that
+ * clotho_phase_traits reads, so the phases run in this order. */
 int clotho_phase_draw(clotho_ctx* ctx);
 int clotho_phase_recombine_mutate(clotho_ctx* ctx);
 int clotho_phase_traits(clotho_ctx* ctx);
