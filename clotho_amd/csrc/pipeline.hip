@@ -80,6 +80,32 @@ __device__ inline T block_inclusive_scan(T v, T* lds, T& total) {
     return off + x;
 }
 
+// Scan of nb <= kTotalsLds block totals into LDS by one 256-thread block
+// (eight totals per thread): dst[i] = src[0] + ... + src[i] (kInclusive) or
+// src[0] + ... + src[i - 1]; returns the sum of all.  Every value is an
+// integer below 2^53, so the order of the additions does not matter.
+constexpr uint32_t kTotalsLds = 2048;
+template <typename T, bool kInclusive, class Src>
+__device__ inline T block_scan_totals(Src src, uint32_t nb, T* dst, T* lds) {
+    const uint32_t t0 = threadIdx.x * 8u;
+    T v[8], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        v[j] = t0 + j < nb ? src[t0 + j] : T(0);
+        sum = sum + v[j];
+    }
+    T total;
+    T run = block_inclusive_scan<T>(sum, lds, total) - sum;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        if (kInclusive) run = run + v[j];
+        if (t0 + j < nb) dst[t0 + j] = run;
+        if (!kInclusive) run = run + v[j];
+    }
+    __syncthreads();
+    return total;
+}
+
 // ---------------------------------------------------------------------------
 // Draw phase
 // ---------------------------------------------------------------------------
@@ -352,11 +378,11 @@ struct SelBlockscan {
 
 // Parent pick for one uniform: uniform over [0, N) when fitness is constant,
 // else min{i : target < C[i]} with target = floor(k * C[N-1] / 2^53).
-template <class StPtr, class FPtr>
-__device__ inline uint32_t pick_parent(const Params& P, StPtr st, uint64_t k, FPtr cum_local,
-                                       FPtr cum_block) {
-    if (st->fit_constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n_par);
-    const double target = (double)__umul64hi(k << 11, st->sel_total);
+template <class LPtr, class BPtr>
+__device__ inline uint32_t pick_parent(const Params& P, bool constant, uint64_t total, uint64_t k,
+                                       LPtr cum_local, BPtr cum_block) {
+    if (constant) return (uint32_t)__umul64hi(k << 11, (uint64_t)P.n_par);
+    const double target = (double)__umul64hi(k << 11, total);
     // first block whose inclusive total exceeds target
     uint32_t lo = 0, hi = parent_blocks(P);
     while (lo < hi) {
@@ -383,7 +409,10 @@ __device__ inline uint32_t pick_parent(const Params& P, StPtr st, uint64_t k, FP
 // Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
 // count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
 // (common_crossover::operator(), fill_events), and the top/bottom parent rows.
-struct Gametes {
+// kScanTotals: each block scans the selection block totals into LDS itself
+// (up to kTotalsLds of them), in place of a separate SelBlockscan launch.
+template <bool kScanTotals>
+struct GametesOp {
     static constexpr int kThreads = kBlock;
     __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
@@ -397,13 +426,25 @@ struct Gametes {
         const auto bot_out = gbl(B.bottom);
         const auto nev_out = gbl(B.n_events);
         const auto events = gbl(B.events);
+        __shared__ double s_cb[kScanTotals ? kTotalsLds : 1];
+        __shared__ double s_lds[kWaves];
+        const bool constant = uniform_read(&st->fit_constant) != 0;
+        bool staged = false;
+        double scanned_total = 0.0;
+        if constexpr (kScanTotals) {
+            const uint32_t nb = parent_blocks(P);
+            staged = nb <= kTotalsLds && !constant;
+            if (staged) scanned_total = block_scan_totals<double, true>(cum_block, nb, s_cb, s_lds);
+        }
         const uint32_t o = blk * kBlock + threadIdx.x;
         if (o >= P.h) return;
         const uint32_t gen = st->gen;
         const uint32_t i = o >> 1;
         const Draw4 pd = philox_draw(P.seed, gen, S_PARENTS, P.replicate, i, 0);
         const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
-        const uint32_t p = pick_parent(P, st, k, cum_local, cum_block);
+        const uint32_t p =
+            staged ? pick_parent(P, false, (uint64_t)scanned_total, k, cum_local, (const double*)s_cb)
+                   : pick_parent(P, constant, st->sel_total, k, cum_local, cum_block);
         const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
         const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
         const bool swap = u53(g.x[2], g.x[3]) < P.bias;
@@ -420,6 +461,8 @@ struct Gametes {
         }
     }
 };
+
+using Gametes = GametesOp<false>;
 
 // Parent order of the child rows: a counting sort of the gametes by parent
 // individual (counts from the gamete draws, block-local exclusive scan, scan
@@ -480,17 +523,28 @@ struct ParentBlockscan {
     }
 };
 
-struct ParentScatter {
+// kScanTotals: each block scans the per-block parent counts into LDS itself
+// (up to kTotalsLds of them), in place of a separate ParentBlockscan launch.
+template <bool kScanTotals>
+struct ParentScatterOp {
     static constexpr int kThreads = kBlock;
     __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
         (void)sel;
+        __shared__ uint32_t s_off[kScanTotals ? kTotalsLds : 1];
+        __shared__ uint32_t s_lds[kWaves];
+        bool staged = false;
+        if constexpr (kScanTotals) {
+            const uint32_t nb = parent_blocks(P);
+            staged = nb <= kTotalsLds;
+            if (staged) (void)block_scan_totals<uint32_t, false>(gbl(B.parent_bsum), nb, s_off, s_lds);
+        }
         const uint32_t o = blk * kBlock + threadIdx.x;
         if (o >= P.h) return;
         const uint32_t top = gbl(B.top)[o], p = top >> 1;
         if (p >= P.n_par) return;
-        const uint32_t pos = gbl(B.parent_bsum)[p / kBlock] +
+        const uint32_t pos = (staged ? s_off[p / kBlock] : gbl(B.parent_bsum)[p / kBlock]) +
                              __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t k0 = gbl(B.mut_local)[o] + gbl(B.mut_bsum)[o / kBlock];
@@ -1776,6 +1830,7 @@ struct MinWaves<Recombine<W, K>> {
 CLOTHO_SINGLE(k_prepare, Prepare)
 CLOTHO_SINGLE(k_sel_blockscan, SelBlockscan)
 CLOTHO_SINGLE(k_parent_blockscan, ParentBlockscan)
+using ParentScatter = ParentScatterOp<true>;
 CLOTHO_SINGLE(k_parent_scatter, ParentScatter)
 CLOTHO_SINGLE(k_word_sort, WordSort)
 // the draw phase of a single population: counts and fitness statistics, then
@@ -1783,7 +1838,7 @@ CLOTHO_SINGLE(k_word_sort, WordSort)
 // statistics, each pair in one launch
 using DrawCounts = Fused<MutCounts, FitnessStats>;
 using DrawAttrs = Fused<MutAttrs, SelScan>;
-using DrawGametes = Fused<Gametes, WordSort>;
+using DrawGametes = Fused<GametesOp<true>, WordSort>;
 CLOTHO_SINGLE(k_draw_counts, DrawCounts)
 CLOTHO_SINGLE(k_draw_attrs, DrawAttrs)
 CLOTHO_SINGLE(k_draw_gametes, DrawGametes)
@@ -1890,11 +1945,14 @@ hipError_t launch_draw(clotho_ctx* c) {
     TRY_LAUNCH(launch_single<DrawCounts>(k_draw_counts, c, par));
     TRY_LAUNCH(launch_single<Prepare>(k_prepare, c, par));
     TRY_LAUNCH(launch_single<DrawAttrs>(k_draw_attrs, c, par));
-    TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
+    // up to kTotalsLds block totals, the gamete and scatter blocks scan them
+    // themselves; beyond, a one-block scan runs first
+    const bool scan_apart = parent_blocks(c->p) > kTotalsLds;
+    if (scan_apart) TRY_LAUNCH(launch_single<SelBlockscan>(k_sel_blockscan, c, par));
     TRY_LAUNCH(launch_single<DrawGametes>(k_draw_gametes, c, par));
     if (!c->b.by_parent) return launch_single<PackI8>(k_pack_i8_pop, c, par);
     TRY_LAUNCH(launch_single<DrawParents>(k_draw_parents, c, par));
-    TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
+    if (scan_apart) TRY_LAUNCH(launch_single<ParentBlockscan>(k_parent_blockscan, c, par));
     return launch_single<ParentScatter>(k_parent_scatter, c, par);
 }
 

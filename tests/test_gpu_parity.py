@@ -176,3 +176,14 @@ def test_clustered_positions():
     cfg = api.make_config(200, 1.0, 25.0, seed=83, traits=1, neutral_p=0.3, sigma=0.3,
                           fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=1.0, max_alleles=1 << 13)
     run_pair(cfg, 6, population=(rows, pos, eff, fit))
+
+
+@pytest.mark.parametrize("n", [300000, 530000])
+def test_selection_block_totals(n):
+    # up to 2048 blocks of 256 individuals the gamete and scatter blocks scan
+    # the selection and parent-count block totals themselves (n = 300,000:
+    # 1,172 blocks); beyond, one-block scans run first (n = 530,000)
+    cfg = api.make_config(n, 0.001, 1.0, seed=89, traits=1, neutral_p=0.2, sigma=0.3,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=1.0, max_alleles=1 << 13)
+    print(f"selection block totals: n={n}", flush=True)
+    run_pair(cfg, 2, population=_dense_population(n, 300, 1, 89))
