@@ -68,7 +68,7 @@ void free_buffers(clotho_ctx* c) {
                     B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
                     B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
                     B.cdf_rho,    B.cdf_mu,    B.state,     B.by_parent,  B.parent_cnt,
-                    B.parent_bsum};
+                    B.parent_bsum, B.prank};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     std::memset(&B, 0, sizeof(B));
@@ -335,6 +335,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.by_parent, h_cap);
     rc |= dalloc(c, &B.parent_cnt, P.n_cap);
     rc |= dalloc(c, &B.parent_bsum, sel_blocks);
+    rc |= dalloc(c, &B.prank, h_cap);
     rc |= dalloc(c, &B.var_part, (size_t)(P.max_chunks + 1) * P.w_cap);
     rc |= dalloc(c, &B.fix_part, (size_t)P.max_chunks * P.w_cap);
     rc |= dalloc(c, &B.word_cnt, P.w_cap);

@@ -451,14 +451,19 @@ struct GametesOp {
         top_out[o] = swap ? 2 * p + 1 : 2 * p;
         bot_out[o] = swap ? 2 * p : 2 * p + 1;
         nev_out[o] = ne;
-        if (B.by_parent && p < P.n_par)
-            __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        // the gamete's rank among its parent's picks (atomic order) places its
+        // record in the parent order without a second atomic
+        uint32_t rank = 0;
+        const bool ranked = B.by_parent && p < P.n_par;
+        if (ranked)
+            rank = __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
         const auto ev = events + (size_t)o * P.ev_cap;
         for (uint32_t e = 0; e < ne; e += 4) {
             const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
             for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
         }
+        if (ranked) gbl(B.prank)[o] = rank;
     }
 };
 
@@ -545,8 +550,7 @@ struct ParentScatterOp {
         const uint32_t top = gbl(B.top)[o], p = top >> 1;
         if (p >= P.n_par) return;
         const uint32_t pos = (staged ? s_off[p / kBlock] : gbl(B.parent_bsum)[p / kBlock]) +
-                             __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+                             gbl(B.parent_cnt)[p] + gbl(B.prank)[o];
         const uint32_t k0 = gbl(B.mut_local)[o] + gbl(B.mut_bsum)[o / kBlock];
         const uint32_t nx = gbl(B.n_events)[o] | (gbl(B.mut_count)[o] << 8);
         if (pos < P.h)

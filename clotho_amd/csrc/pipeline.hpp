@@ -97,6 +97,7 @@ struct Buffers {
     // kSibs of its children; nullptr = child order (the batched launches).
     uint4* by_parent;         // [h]
     uint32_t* parent_cnt;     // [n] children per parent -> exclusive in-block offsets
+    uint32_t* prank;          // [h] rank of the gamete among its parent's picks
     uint32_t* parent_bsum;    // [ceil(n/256)] block totals -> exclusive offsets
     uint64_t* var_part;       // [chunks][w_cap] OR partials
     uint64_t* fix_part;       // [chunks][w_cap] AND partials
