@@ -24,6 +24,7 @@ TRAITS_PLAIN_FP32 = 0
 TRAITS_MFMA_BF16 = 1
 TRAITS_EXACT_FP64 = 2
 TRAITS_WALK_FP32 = 3
+TRAITS_EXACT_I8 = 4
 
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded", -4: "bad state"}
 

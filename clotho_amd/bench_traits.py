@@ -3,9 +3,10 @@ x 65,536 sites x 16 traits, genotype bits Bernoulli(f_j) with f_j ~ U(0,1)
 (seed 42, drawn on the device), effects from a multivariate normal with unit
 variances and pairwise correlation 0.3 (seed 43).
 
-Reports kernel time of the plain FP32 set-bit walk, the FP64 walk and the
-split-BF16 MFMA contraction (1, 2, 3 BF16 terms, FP32 accumulation per K
-chunk, FP64 chunk reduction), the MFMA/plain speedup, and per-trait error
+Reports kernel time of the plain FP32 set-bit walk, the FP64 walk, the
+pipeline's exact int8 path (effects quantised to k 2^-q, int8 digit planes
+on int8 MFMA, exact integer sums) and the split-BF16 MFMA contraction (1, 2,
+3 BF16 terms, FP32 accumulation per K chunk, FP64 chunk reduction), the MFMA/plain speedup, and per-trait error
 against an FP64 reference: normwise max_i |z_i - z*_i| / max_i |z*_i| and the
 median elementwise relative error.  "plain" is the dense bit x effect FP32
 kernel (lane = haplotype); "walk" is the sparse set-bit walk.  The FP64 reference is the GPU FP64 walk
@@ -66,6 +67,16 @@ def main():
                                            repeats=1)
     out["walk_fp32_ms"] = msw
     out["walk_fp32_err_normwise_max"] = float(errors(zw, z64)[0].max())
+    zi, msi = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_I8, seed=42,
+                                           repeats=args.repeats)
+    ni, mi = errors(zi, z64)
+    out["exact_i8_ms"] = msi
+    out["exact_i8_speedup_vs_plain"] = ms32 / msi
+    out["exact_i8_err_normwise_max"] = float(ni.max())
+    out["exact_i8_err_median_rel_max"] = float(mi.max())
+    # four int8 digit planes per trait: 2 H S (4 T) integer operations executed
+    out["exact_i8_exec_tops"] = 2.0 * h * s * 4 * t / (msi * 1e-3) / 1e12
+    out["exact_i8_row_gbps"] = h * (((s + 511) // 512) * 64) / (msi * 1e-3) / 1e9
     for terms in (1, 2, 3):
         zm, msm = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_MFMA_BF16, seed=42,
                                                split_terms=terms, repeats=args.repeats)

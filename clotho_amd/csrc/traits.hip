@@ -6,7 +6,8 @@
 // Modes: PLAIN_FP32 walks the set bits and sums FP32 effects (the plain
 // popcount-weighted baseline); EXACT_FP64 does the same walk in FP64;
 // MFMA_BF16 expands bits to BF16 and runs the contraction on matrix cores
-// with effects split into BF16 terms (traits_mfma.hip).
+// with effects split into BF16 terms (traits_mfma.hip); EXACT_I8 is the
+// pipeline's exact path (int8 digit planes of quantised effects).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -172,8 +173,10 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
         (size_t)w * 64 < sites || repeats < 1)
         return CLOTHO_ERR_ARG;
     if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
-        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32)
+        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32 &&
+        mode != CLOTHO_TRAITS_EXACT_I8)
         return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_EXACT_I8 && sites >= (1u << 23)) return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
@@ -219,8 +222,10 @@ extern "C" int clotho_trait_contract_synthetic(uint32_t h, uint32_t sites, const
         repeats < 1)
         return CLOTHO_ERR_ARG;
     if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&
-        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32)
+        mode != CLOTHO_TRAITS_MFMA_BF16 && mode != CLOTHO_TRAITS_WALK_FP32 &&
+        mode != CLOTHO_TRAITS_EXACT_I8)
         return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_EXACT_I8 && sites >= (1u << 23)) return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;

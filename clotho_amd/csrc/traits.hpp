@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -49,6 +50,11 @@ struct TraitWorkspace {
     float* d_z32 = nullptr;
     double* d_z64 = nullptr;
     double* d_partial = nullptr;
+    int32_t* d_qeff = nullptr;   // EXACT_I8: effects as integers k (effect ~ k 2^-q)
+    uint32_t* d_dims = nullptr;  // EXACT_I8: {words, sites} as the pack kernel reads them
+    uint4* d_packed8 = nullptr;  // EXACT_I8: int8 digit planes
+    int64_t* d_sum = nullptr;    // EXACT_I8: exact sums of k
+    int q = 0;
 
     int init(const uint64_t* rows, uint32_t h_, uint32_t w_, const double* eff, uint32_t s_,
              uint32_t t_, int mode_, int terms_, hipStream_t st) {
@@ -70,6 +76,31 @@ struct TraitWorkspace {
                 hipMalloc((void**)&d_z32, 4 * (size_t)h * traits) != hipSuccess)
                 return CLOTHO_ERR_HIP;
             if (hipMemcpy(d_eff32, e.data(), 4 * e.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+        } else if (mode == CLOTHO_TRAITS_EXACT_I8) {
+            // largest q with every |effect| 2^q <= 2^30 - 1, then round to nearest
+            double mx = 0.0;
+            for (size_t i = 0; i < (size_t)sites * traits; ++i) mx = std::max(mx, std::fabs(eff[i]));
+            q = 0;
+            if (mx > 0.0) {
+                q = 29 - std::ilogb(mx);
+                while (std::ldexp(mx, q) > 1073741823.0) --q;
+                q = std::max(-60, std::min(60, q));
+            }
+            std::vector<int32_t> k((size_t)sites * traits);
+            for (size_t i = 0; i < k.size(); ++i) k[i] = (int32_t)std::llround(std::ldexp(eff[i], q));
+            const uint32_t dims[2] = {w, sites};
+            if (hipMalloc((void**)&d_qeff, 4 * k.size()) != hipSuccess ||
+                hipMalloc((void**)&d_dims, sizeof(dims)) != hipSuccess ||
+                hipMalloc((void**)&d_packed8, sizeof(uint4) * i8_packed_elems(traits, w)) !=
+                    hipSuccess ||
+                hipMalloc((void**)&d_sum, 8 * (size_t)h * traits) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            if (hipMemcpy(d_qeff, k.data(), 4 * k.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_dims, dims, sizeof(dims), hipMemcpyHostToDevice) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            // the digit planes are built once per effect set, as in a generation
+            if (launch_pack_i8(d_qeff, traits, d_dims, d_dims + 1, w, d_packed8, st) != hipSuccess)
                 return CLOTHO_ERR_HIP;
         } else if (mode == CLOTHO_TRAITS_EXACT_FP64) {
             if (hipMalloc((void**)&d_eff64, 8 * (size_t)sites * traits) != hipSuccess ||
@@ -100,6 +131,8 @@ struct TraitWorkspace {
             return launch_traits_walk_fp32(d_rows, h, w, d_eff32, sites, traits, d_z32, st);
         if (mode == CLOTHO_TRAITS_EXACT_FP64)
             return launch_traits_walk_fp64(d_rows, h, w, d_eff64, sites, traits, d_z64, st);
+        if (mode == CLOTHO_TRAITS_EXACT_I8)
+            return launch_traits_i8(d_rows, h, d_dims, traits, d_packed8, d_sum, st);
         return launch_traits_mfma(d_rows, h, w, d_packed, sites, traits, terms, d_partial, d_z64,
                                   st);
     }
@@ -112,6 +145,12 @@ struct TraitWorkspace {
             if (hipMemcpy(tmp.data(), d_z32, 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
                 return CLOTHO_ERR_HIP;
             for (size_t i = 0; i < n; ++i) z[i] = tmp[i];
+        } else if (mode == CLOTHO_TRAITS_EXACT_I8) {
+            std::vector<int64_t> tmp(n);
+            if (hipMemcpy(tmp.data(), d_sum, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
+                return CLOTHO_ERR_HIP;
+            // |sum| < 2^53 (sites < 2^23), so the conversion and the scaling are exact
+            for (size_t i = 0; i < n; ++i) z[i] = std::ldexp((double)tmp[i], -q);
         } else {
             if (hipMemcpy(z, d_z64, 8 * n, hipMemcpyDeviceToHost) != hipSuccess)
                 return CLOTHO_ERR_HIP;
@@ -120,7 +159,8 @@ struct TraitWorkspace {
     }
 
     void release() {
-        void* ptrs[] = {d_rows, d_eff32, d_eff64, d_packed, d_z32, d_z64, d_partial};
+        void* ptrs[] = {d_rows, d_eff32, d_eff64, d_packed, d_z32, d_z64, d_partial,
+                        d_qeff, d_dims, d_packed8, d_sum};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }

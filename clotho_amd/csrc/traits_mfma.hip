@@ -220,9 +220,9 @@ hipError_t launch_traits_i8(const uint64_t* rows, uint32_t h, const uint32_t* p_
         const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
         hipLaunchKernelGGL((k_traits_i8<2, 2>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
                            traits, packed, out);
-    } else if (nt <= 4) {
-        const uint32_t blocks = (h + 4 * 32 - 1) / (4 * 32);
-        hipLaunchKernelGGL((k_traits_i8<2, 4>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
+    } else if (nt <= 4) {  // four row tiles per wave, as the pipeline's 16-trait kernel
+        const uint32_t blocks = (h + 4 * 64 - 1) / (4 * 64);
+        hipLaunchKernelGGL((k_traits_i8<4, 4>), dim3(blocks), dim3(256), 0, s, rows, h, p_words,
                            traits, packed, out);
     } else {
         const uint32_t blocks = (h + 4 * 16 - 1) / (4 * 16);

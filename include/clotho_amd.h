@@ -80,6 +80,9 @@ extern "C" {
 #define CLOTHO_TRAITS_MFMA_BF16 1  /* bits -> BF16, effects split into BF16 terms, MFMA */
 #define CLOTHO_TRAITS_EXACT_FP64 2 /* set-bit walk, FP64 sums                          */
 #define CLOTHO_TRAITS_WALK_FP32 3  /* set-bit walk, FP32 sums                          */
+#define CLOTHO_TRAITS_EXACT_I8 4   /* the pipeline's exact path: effects quantised to   */
+                                   /* k 2^-q (largest q with |k| < 2^30), int8 digit    */
+                                   /* planes on int8 MFMA, exact int64 sums x 2^-q      */
 
 typedef struct clotho_config {
     uint64_t seed;            /* random_number.seed (Philox key)                         */

@@ -68,3 +68,36 @@ def test_synthetic_contraction_matches_numpy_fp64():
     zm, _ = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_MFMA_BF16, seed=42,
                                          split_terms=3)
     assert np.max(normwise(zm, ref)) < 1e-6
+
+
+@pytest.mark.parametrize("h,sites,t", [(1000, 1024, 16), (333, 700, 5), (48, 64, 1), (700, 3000, 9)])
+def test_contraction_exact_i8(h, sites, t):
+    """The pipeline's exact path: effects quantised to k 2^-q with the largest
+    q keeping |k| below 2^30; the result equals the float64 sum of the
+    quantised effects bit for bit (integer sums below 2^53)."""
+    rows, eff, ref = random_case(h, sites, t, 0.4, seed=7 * h + t)
+    bits = np.unpackbits(rows.view(np.uint8), axis=1, bitorder="little")[:, :sites]
+    q = 29 - int(np.floor(np.log2(np.max(np.abs(eff)))))
+    while np.max(np.abs(eff)) * 2.0**q > 2**30 - 1:
+        q -= 1
+    k = np.round(eff * 2.0**q)  # ties: numpy rounds half to even, llround away from zero
+    ties = np.abs(eff * 2.0**q - np.trunc(eff * 2.0**q)) == 0.5
+    k[ties] = np.trunc(eff * 2.0**q)[ties] + np.sign(eff[ties])
+    exact = (bits.astype(np.int64) @ k.astype(np.int64)).astype(np.float64) * 2.0**-q
+    zi, _ = api.trait_contract(rows, eff, api.TRAITS_EXACT_I8)
+    assert np.array_equal(zi, exact)
+    # against the unquantised float64 product: at most half a quantum per set bit
+    bound = 0.5 * 2.0**-q * bits.sum(axis=1, keepdims=True) + 1e-12 * np.abs(ref)
+    assert np.all(np.abs(zi - ref) <= bound)
+
+
+def test_synthetic_contraction_exact_i8():
+    import oracle
+
+    h, s, t = 400, 3000, 16
+    freqs = np.random.default_rng(42).uniform(0, 1, s)
+    eff = np.random.default_rng(43).standard_normal((s, t))
+    rows = oracle.synth_rows(42, freqs, 0, h)
+    zr, _ = api.trait_contract(rows, eff, api.TRAITS_EXACT_I8)
+    zs, _ = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_I8, seed=42)
+    assert np.array_equal(zr, zs)
