@@ -653,7 +653,7 @@ constexpr int kRecBlock = 64 * kRecWaves;
 constexpr int kEventGroup = 2;
 // children of one parent individual produced from one load of its row pair
 // (parent-ordered gametes; the batched launches keep child order and take 1)
-constexpr int kSibsOrdered = 4;
+constexpr int kSibsOrdered = 3;
 // dword 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
 // data format, offsets range-checked against num_records in bytes
 constexpr int kBufferWord3 = 0x00020000;
