@@ -16,8 +16,21 @@ free space, traits, fitness, selection) over resident HBM state, replayed as
 a captured hipGraph.  `value` = offspring genomes (haplotype rows) per second.
 The dominant kernel (recombination) is timed with HIP events on the
 library's stream in a second pass; the CPU baseline is the oracle (a C++
-restatement of the reference engine) timed on one host core for one
-generation of the same population, and its output is compared bit for bit.
+restatement of the reference engine) timed for one generation of the same
+population with the reference's thread model (contiguous offspring batches,
+one per thread) on the host's CPU share, and on one core; its output is
+compared bit for bit.
+
+The rest of the metric rides on the same line:
+  mfma_vs_plain          the precision workload (200,000 haplotypes x 65,536
+                         sites x 16 traits, MVN effects): exact int8 MFMA and
+                         split-BF16 MFMA contraction against the plain FP32
+                         kernel, with the error of each against FP64;
+  batched_vs_sequential  the replicate workload (256 replicates, N
+                         log-uniform on [1e3, 5e4]): one persistent launch per
+                         phase and a graph per generation, against each
+                         replicate's own graph in turn;
+  large_population       the footprint of the N = 5e5, 16-trait population.
 """
 from __future__ import annotations
 
@@ -76,6 +89,105 @@ def traffic_from_profiles():
         return None
 
 
+def host_cpu():
+    """CPU model, logical CPUs and the CPU share this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    return model, nproc, share
+
+
+def mfma_vs_plain(api):
+    """The precision workload: 200,000 x 65,536 x 16, f ~ U(0,1), MVN effects."""
+    from clotho_amd.bench_traits import effects_mvn, errors
+
+    h, s, t = 200_000, 65_536, 16
+    freqs = np.random.default_rng(42).uniform(0, 1, s)
+    eff = effects_mvn(s, t)
+    z64, _ = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_FP64, seed=42)
+    z32, ms32 = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_PLAIN_FP32, seed=42,
+                                             repeats=3)
+    zi, msi = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_I8, seed=42,
+                                           repeats=10)
+    zb, msb = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_MFMA_BF16, seed=42,
+                                           split_terms=3, repeats=10)
+    n32, ni, nb = errors(z32, z64)[0], errors(zi, z64)[0], errors(zb, z64)[0]
+    flops = 2.0 * h * s * t
+    return {
+        "workload": "H=200000 x S=65536 x T=16, genotypes Bernoulli(f), f~U(0,1) seed 42, "
+                    "effects MVN(unit var, corr 0.3) seed 43; error = per trait "
+                    "max|z-z*|/max|z*| against an FP64 sum",
+        "plain_fp32_ms": ms32, "plain_fp32_err_max": float(n32.max()),
+        "exact_i8_ms": msi, "exact_i8_err_per_trait": [float(x) for x in ni],
+        "exact_i8_speedup": ms32 / msi,
+        "exact_i8_exec_tops": 4 * flops / (msi * 1e-3) / 1e12,
+        "bf16x3_ms": msb, "bf16x3_err_per_trait": [float(x) for x in nb],
+        "bf16x3_speedup": ms32 / msb,
+        "bf16x3_alg_tflops": flops / (msb * 1e-3) / 1e12,
+        "bf16x3_exec_tflops": 3 * flops / (msb * 1e-3) / 1e12,
+        "bf16x3_exec_frac_of_2500": 3 * flops / (msb * 1e-3) / 1e12 / 2500.0,
+    }
+
+
+def batched_vs_sequential(api, generations):
+    """The replicate workload, both modes from the same founders."""
+    from clotho_amd.bench_batch import MU_SCALE, CAPACITY, replicate_configs
+
+    cfgs, N, _, _ = replicate_configs(api, 256)
+    ms = {}
+    shapes = {}
+    for mode in ("graph", "sequential"):
+        b = api.Batch(cfgs)
+        ms[mode] = b.run(generations, mode=mode)
+        shapes[mode] = [b.replicate(r).shape().as_dict() for r in range(len(cfgs))]
+        b.close()
+    genomes = 2.0 * float(N.sum()) * generations
+    return {
+        "workload": f"256 replicates, N log-uniform [1e3, 5e4], rho U[0.5, 5], "
+                    f"mu U[0.5, 5] x {MU_SCALE}, {CAPACITY} slots each, 1 trait, "
+                    f"{generations} generations from empty founders",
+        "batched_ms_per_generation": ms["graph"] / generations,
+        "sequential_ms_per_generation": ms["sequential"] / generations,
+        "batched_genomes_per_s": genomes / (ms["graph"] * 1e-3),
+        "sequential_genomes_per_s": genomes / (ms["sequential"] * 1e-3),
+        "speedup": ms["sequential"] / ms["graph"],
+        "shapes_equal": shapes["graph"] == shapes["sequential"],
+    }
+
+
+def large_population_footprint(api):
+    """Device bytes of the N = 5e5, 16-trait population: two probe contexts
+    give bytes per allele slot, the bench capacity is allocated and read
+    back, and the free HBM gives the largest capacity that fits."""
+    from clotho_amd.bench_large import config as large_config
+
+    probes = []
+    for slots in (1 << 16, 1 << 17):
+        p = api.Simulation(large_config(api, slots))
+        probes.append((slots, p.device_bytes()))
+        p.close()
+    (sa, ba), (sb, bb) = probes
+    per_slot = (bb - ba) / (sb - sa)
+    fixed = ba - per_slot * sa
+    slots = 262_144 + 8 * 4096
+    p = api.Simulation(large_config(api, slots))
+    dev = p.device_bytes()
+    p.close()
+    free, total = api.device_memory(0)
+    return {"haplotypes": 1_000_000, "traits": 16, "slots": slots, "device_bytes": dev,
+            "device_gib": dev / 2**30, "bytes_per_slot": per_slot, "fixed_bytes": fixed,
+            "hbm_total_gib": total / 2**30,
+            "max_slots_in_free_hbm": int((free * 0.97 - fixed) / per_slot) // 512 * 512}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +195,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg")
     ap.add_argument("--eager", action="store_true", help="time eager launches instead of a graph")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the precision, replicate and footprint legs")
+    ap.add_argument("--batch-generations", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus above 1 is not supported "
@@ -114,10 +229,23 @@ def main():
         orc = oracle.OracleSim(cfg)
         orc.set_population(rows, pos, eff, None)
         del rows
-        log("[bench] oracle loaded; timing one generation on one core")
+        # the reference's thread model: T contiguous offspring batches, one
+        # per thread (examples/QTLSim/engine_parallel_pipeline.hpp:203-226),
+        # on the CPU share of this process; then the same generation on one core
+        model, nproc, share = host_cpu()
+        threads = max(1, min(share, 16))
+        orc_mt = oracle.OracleSim(cfg)
+        orc_mt.set_population(rows, pos, eff, None)
+        orc_mt.set_threads(threads)
+        log(f"[bench] oracle loaded; timing one generation on {threads} threads, then one core")
+        t0 = time.perf_counter()
+        orc_mt.step()
+        t_mt = time.perf_counter() - t0
         t0 = time.perf_counter()
         orc.step()
         t_cpu = time.perf_counter() - t0
+        mt_equal = bool(np.array_equal(orc_mt.rows(), orc.rows()))
+        orc_mt.close()
         sim.run(1)
         g_rows = sim.rows()
         parity = bool(np.array_equal(g_rows, orc.rows()))
@@ -130,14 +258,19 @@ def main():
         top, bot, _, _ = sim.gametes()
         otop, obot, _, _ = orc.gametes()
         parity = parity and np.array_equal(top, otop) and np.array_equal(bot, obot)
-        cpu = {"value": h / t_cpu, "unit": "offspring genomes/s", "cores": 1, "kind": "port",
-               "sample": f"one full generation (H={h}, W={s0.words}) of the oracle, single thread, "
-                         f"{t_cpu:.1f} s; phases crossover/mutate/phenotype/fixed/fitness/"
-                         f"finalize/draws = "
-                         + "/".join(f"{x:.2f}" for x in orc.phase_seconds()) + " s",
-               "seconds": t_cpu}
+        cpu = {"value": h / t_mt, "unit": "offspring genomes/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one full generation (H={h}, W={s0.words}) of the oracle in {threads} "
+                         f"contiguous offspring batches on {threads} threads, {t_mt:.2f} s; the "
+                         f"same generation on one thread {t_cpu:.1f} s (phases crossover/mutate/"
+                         f"phenotype/fixed/fitness/finalize/draws = "
+                         + "/".join(f"{x:.2f}" for x in orc.phase_seconds()) + " s)",
+               "seconds": t_mt, "single_thread_value": h / t_cpu,
+               "single_thread_seconds": t_cpu, "threads_equal_single": mt_equal,
+               "cpu_model": model, "nproc": nproc, "cpu_share": share}
         orc.close()
-        log(f"[bench] cpu generation {t_cpu:.2f}s; parity={parity}")
+        log(f"[bench] cpu generation {t_mt:.2f}s on {threads} threads, {t_cpu:.2f}s on one; "
+            f"parity={parity}")
 
     graph = not args.eager
     sim.run(args.warmup, graph=graph)
@@ -173,6 +306,22 @@ def main():
     _, _, stats_ms = sim.allele_counts(timed=True)
     log(f"[bench] population statistics (all rows): {stats_ms:.3f} ms")
 
+    sim.close()
+    extras = {}
+    if not args.no_extras:
+        t0 = time.perf_counter()
+        extras["mfma_vs_plain"] = mfma_vs_plain(api)
+        log(f"[bench] precision workload: {extras['mfma_vs_plain']} "
+            f"({time.perf_counter() - t0:.1f}s)")
+        t0 = time.perf_counter()
+        extras["batched_vs_sequential"] = batched_vs_sequential(api, args.batch_generations)
+        log(f"[bench] replicate workload: {extras['batched_vs_sequential']} "
+            f"({time.perf_counter() - t0:.1f}s)")
+        t0 = time.perf_counter()
+        extras["large_population"] = large_population_footprint(api)
+        log(f"[bench] large population footprint: {extras['large_population']} "
+            f"({time.perf_counter() - t0:.1f}s)")
+
     ms_per_step = elapsed / args.steps * 1e3
     value = h * args.steps / elapsed
     out = {
@@ -196,7 +345,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
                      "kernel": "k_recombine", "kernel_ms": rec_ms,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     # bytes the kernel actually moves (hardware counters) per
+                     # second: siblings share one load of their parents' rows
+                     "traffic_gbps": traffic / (rec_ms * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": (traffic / (rec_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS
+                                      if traffic else None)},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
         "hbm_copy_gbps_measured": copy_gbps,
@@ -204,6 +358,7 @@ def main():
         "diploid_per_s": value / 2,
         "population_stats": {"ms": stats_ms,
                              "gbps": 8.0 * words * h / (stats_ms * 1e-3) / 1e9 if stats_ms else None},
+        **extras,
     }
     print(json.dumps(out), flush=True)
 

@@ -124,7 +124,7 @@ EXPORTS = [
     "clotho_batch_size", "clotho_batch_replicate", "clotho_batch_run", "clotho_batch_timings",
     "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
     "clotho_sample_rows", "clotho_allele_counts", "clotho_set_population_size",
-    "clotho_pairwise_differences",
+    "clotho_pairwise_differences", "clotho_device_memory",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -177,6 +177,7 @@ def lib():
                                                   C.c_uint32, C.c_int, C.c_int, C.c_int, vp,
                                                   C.POINTER(C.c_double), C.c_int]
     L.clotho_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.clotho_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.clotho_sample_rows.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                      u32p]
     L.clotho_allele_counts.argtypes = [vp, vp, C.c_uint32, u32p, C.c_size_t, u32p,
@@ -397,6 +398,15 @@ def stream_copy_gbps(nbytes=1 << 30, repeats=20, device=0):
     if rc != 0:
         raise ClothoError(f"stream_copy failed ({ERRORS.get(rc, rc)})")
     return out.value
+
+
+def device_memory(device=0):
+    """(free, total) device memory in bytes."""
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    rc = lib().clotho_device_memory(device, C.byref(f), C.byref(t))
+    if rc != 0:
+        raise ClothoError(f"device_memory failed ({ERRORS.get(rc, rc)})")
+    return f.value, t.value
 
 
 def trait_contract(rows, effects, mode, split_terms=3, repeats=1, device=0):

@@ -71,9 +71,8 @@ def main():
     args = ap.parse_args()
 
     from clotho_amd import api
-    import torch
 
-    free0, total = torch.cuda.mem_get_info(0)
+    free0, total = api.device_memory(0)
     h = 2 * N_IND
     cap = args.slots + 8 * 4096
     sim = api.Simulation(config(api, cap))
@@ -117,7 +116,7 @@ def main():
     (s_a, b_a), (s_b, b_b) = probes
     per_slot = (b_b - b_a) / (s_b - s_a)
     fixed = b_a - per_slot * s_a
-    free1, _ = torch.cuda.mem_get_info(0)
+    free1, _ = api.device_memory(0)
     s_max = int((free1 * 0.97 - fixed) / per_slot) // 512 * 512
     max_ok = None
     max_bytes = None

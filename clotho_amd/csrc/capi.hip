@@ -234,7 +234,9 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     if (!(cfg->neutral_p >= 0.0 && cfg->neutral_p <= 1.0)) return CLOTHO_ERR_ARG;
     if (cfg->fitness_kind > CLOTHO_FIT_MV_STABILIZING) return CLOTHO_ERR_ARG;
     if (cfg->effect_quantum_log2 > 0 || cfg->effect_quantum_log2 < -60) return CLOTHO_ERR_ARG;
-    if (cfg->max_alleles == 0 || cfg->max_alleles > (1u << 30)) return CLOTHO_ERR_ARG;
+    // the exact traits accumulate balanced int8 digits (|d| <= 128) over a whole
+    // row in int32: at most 2^24 slots keep every partial sum in range
+    if (cfg->max_alleles == 0 || cfg->max_alleles > kMaxAlleleSlots) return CLOTHO_ERR_ARG;
     if ((cfg->fitness_kind == CLOTHO_FIT_GAUSSIAN || cfg->fitness_kind == CLOTHO_FIT_QUADRATIC) &&
         !(cfg->mu > 0.0))
         return CLOTHO_ERR_ARG;

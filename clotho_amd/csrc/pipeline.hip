@@ -1,9 +1,9 @@
 // pipeline.hip — gfx950 kernels of one generation of the offspring pipeline.
 //
-// Generation = draw -> recombine+mutate (+fused free-space partials and the
-// incremental single-trait update) -> free-space reduce/compact -> fitness
-// -> commit, all on the context stream, all shapes read from GenState so a
-// captured hipGraph replays generation after generation.
+// Generation = draw -> recombine+mutate (+fused free-space partials) ->
+// exact traits -> free-space reduce/compact -> fitness -> commit, all on the
+// context stream, all shapes read from GenState so a captured hipGraph
+// replays generation after generation.
 //
 // Reference path restated (DS/ = include/clotho/data_spaces/,
 // QS/ = examples/QTLSim/):

@@ -17,6 +17,8 @@ constexpr int kMaxTraits = 32;    // trait capacity of the fused paths
 constexpr int kScanBlock = 256;   // items per block of every two-level scan
 // |quantised effect| bound: four balanced base-256 digits represent it exactly
 constexpr double kEffectLimit = 1073741823.0;  // 2^30 - 1
+// allele-slot capacity limit: int8 digit sums over a row stay within int32
+constexpr uint32_t kMaxAlleleSlots = 1u << 24;
 
 // Scalars of the generation, kept in device memory so that a captured
 // hipGraph replays successive generations without host involvement.

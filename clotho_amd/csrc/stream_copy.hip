@@ -61,3 +61,13 @@ extern "C" int clotho_stream_copy(size_t bytes, int repeats, double* gbps, int d
     if (b) (void)hipFree(b);
     return rc;
 }
+
+extern "C" int clotho_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return CLOTHO_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return CLOTHO_ERR_HIP;
+    *free_bytes = f;
+    *total_bytes = t;
+    return CLOTHO_OK;
+}

@@ -60,9 +60,11 @@ __global__ void __launch_bounds__(kBlock) k_traits_walk(const uint64_t* __restri
 }
 
 // Dense popcount-weighted FP32 contraction: lane = haplotype row, effects of
-// a 64-site block staged in LDS and broadcast, z_t += bit * w_t for every
-// site (the reference's per-sequence bit x effect accumulation,
+// a 64-site block staged in LDS and broadcast, z_t = fma(bit, w_t, z_t) for
+// every site (the reference's per-sequence bit x effect accumulation,
 // CU/phenotype/translation_kernels.hpp:123-181, generalised to T traits).
+// The library builds with -ffp-contract=off for bit-exact parity elsewhere,
+// so the fused multiply-add is written out: one v_fma_f32 per site and trait.
 template <int TMAX>
 __global__ void __launch_bounds__(kBlock) k_traits_dense(const uint64_t* __restrict__ rows,
                                                          uint32_t h, uint32_t w,
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_traits_dense(const uint64_t* __restr
         for (uint32_t j = 0; j < 64; ++j) {
             const float bit = (float)((uint32_t)(x >> j) & 1u);
 #pragma unroll
-            for (int t = 0; t < TMAX; ++t) acc[t] += bit * s_eff[j * TMAX + t];
+            for (int t = 0; t < TMAX; ++t) acc[t] = __builtin_fmaf(bit, s_eff[j * TMAX + t], acc[t]);
         }
     }
     if (valid)
@@ -179,6 +181,11 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
     if (mode == CLOTHO_TRAITS_EXACT_I8 && sites >= (1u << 23)) return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_EXACT_I8) {
+        int q = 0;
+        if (i8_quantum_exponent(effects, (size_t)sites * traits, &q) != CLOTHO_OK)
+            return CLOTHO_ERR_ARG;
+    }
     if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CLOTHO_ERR_HIP;
@@ -228,6 +235,11 @@ extern "C" int clotho_trait_contract_synthetic(uint32_t h, uint32_t sites, const
     if (mode == CLOTHO_TRAITS_EXACT_I8 && sites >= (1u << 23)) return CLOTHO_ERR_ARG;
     if (mode == CLOTHO_TRAITS_MFMA_BF16 && (split_terms < 1 || split_terms > 3))
         return CLOTHO_ERR_ARG;
+    if (mode == CLOTHO_TRAITS_EXACT_I8) {
+        int q = 0;
+        if (i8_quantum_exponent(effects, (size_t)sites * traits, &q) != CLOTHO_OK)
+            return CLOTHO_ERR_ARG;
+    }
     if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
     const uint32_t words = ((sites + 511) / 512) * 8;
     std::vector<uint32_t> thr(sites);

@@ -40,6 +40,26 @@ hipError_t launch_traits_mfma(const uint64_t* rows, uint32_t h, uint32_t w,
                               const uint16_t* packed, uint32_t sites, uint32_t traits, int terms,
                               double* partial, double* z, hipStream_t s);
 
+// Quantum exponent of the exact int8 contraction: the largest q with every
+// |effect| 2^q <= 2^30 - 1.  Effects must be finite, and their largest
+// magnitude must give q in [-60, 60] (roughly 2^-31 <= max |effect| <= 2^89);
+// otherwise the call is rejected with CLOTHO_ERR_ARG before anything runs.
+inline int i8_quantum_exponent(const double* eff, size_t n, int* q_out) {
+    double mx = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(eff[i])) return CLOTHO_ERR_ARG;
+        mx = std::max(mx, std::fabs(eff[i]));
+    }
+    int q = 0;
+    if (mx > 0.0) {
+        q = 29 - std::ilogb(mx);
+        while (std::ldexp(mx, q) > 1073741823.0) --q;
+        if (q < -60 || q > 60) return CLOTHO_ERR_ARG;
+    }
+    *q_out = q;
+    return CLOTHO_OK;
+}
+
 struct TraitWorkspace {
     int mode = 0, terms = 0;
     uint32_t h = 0, w = 0, sites = 0, traits = 0;
@@ -79,14 +99,8 @@ struct TraitWorkspace {
                 return CLOTHO_ERR_HIP;
         } else if (mode == CLOTHO_TRAITS_EXACT_I8) {
             // largest q with every |effect| 2^q <= 2^30 - 1, then round to nearest
-            double mx = 0.0;
-            for (size_t i = 0; i < (size_t)sites * traits; ++i) mx = std::max(mx, std::fabs(eff[i]));
-            q = 0;
-            if (mx > 0.0) {
-                q = 29 - std::ilogb(mx);
-                while (std::ldexp(mx, q) > 1073741823.0) --q;
-                q = std::max(-60, std::min(60, q));
-            }
+            if (i8_quantum_exponent(eff, (size_t)sites * traits, &q) != CLOTHO_OK)
+                return CLOTHO_ERR_ARG;
             std::vector<int32_t> k((size_t)sites * traits);
             for (size_t i = 0; i < k.size(); ++i) k[i] = (int32_t)std::llround(std::ldexp(eff[i], q));
             const uint32_t dims[2] = {w, sites};

@@ -100,7 +100,7 @@ typedef struct clotho_config {
     double fitness_scale;     /* quadratic scale, or sigma of the fixed Gaussians        */
     double env_sigma;         /* s.d. of environmental noise added per individual (0=off)*/
     int32_t effect_quantum_log2; /* effects are integer multiples of 2^q (e.g. -24)      */
-    uint32_t max_alleles;     /* allele-slot capacity S_cap                              */
+    uint32_t max_alleles;     /* allele-slot capacity S_cap, at most 2^24                */
     uint32_t compact_every;   /* 1: compact fixed and lost columns away after every
                                  generation (clotho_compact); 0: keep the reference's
                                  slot reuse                                              */
@@ -305,6 +305,9 @@ int clotho_trait_contract(const uint64_t* rows, uint32_t haplotypes, uint32_t wo
  * no reference counterpart: it measures the HBM bandwidth the roofline is
  * quoted against and calibrates hardware byte counters. */
 int clotho_stream_copy(size_t bytes, int repeats, double* gbps, int device);
+/* Free and total device memory of `device` in bytes (hipMemGetInfo); no
+ * reference counterpart (sizes the largest population that fits). */
+int clotho_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Same contraction over a genotype matrix drawn on the device: bit j of row
  * r is set when Philox word (seed, generation 0, stream 7, replicate 0,

@@ -64,6 +64,8 @@ def lib():
     L.or_select.argtypes = [f64p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, u32p]
     L.or_fitness_value.argtypes = [C.c_void_p, C.c_uint32, f64p]
     L.or_fitness_value.restype = C.c_double
+    L.or_env_noise.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.or_env_noise.restype = C.c_double
     L.or_sim_create.argtypes = [C.c_void_p]
     L.or_sim_create.restype = C.c_void_p
     L.or_sim_destroy.argtypes = [C.c_void_p]
@@ -215,6 +217,13 @@ def allele_counts(rows, alleles, sample=None):
 def fitness_value(cfg, n, z):
     z = np.ascontiguousarray(z, np.float64)
     return lib().or_fitness_value(C.addressof(cfg), n, z)
+
+
+def env_noise(cfg, generation, individuals, trait=0):
+    """Environmental noise draws of individuals 0..individuals-1 (one trait)."""
+    f = lib().or_env_noise
+    a = C.addressof(cfg)
+    return np.array([f(a, generation, i, trait) for i in range(individuals)], np.float64)
 
 
 # ---------------------------------------------------------------- simulator

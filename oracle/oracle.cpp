@@ -508,6 +508,19 @@ extern "C" void or_select(const double* fitness, uint32_t n, uint64_t seed, uint
     }
 }
 
+// Environmental noise of individual i, trait t in generation gen: env_sigma *
+// Phi^-1(u), u the open uniform of Philox stream 4 (no reference counterpart:
+// the reference ignores environmental factors, examples/qtl/README).
+inline double env_noise(const clotho_config& cfg, uint32_t gen, uint32_t i, uint32_t t) {
+    const Draw d = draw(cfg.seed, gen, S_ENV, cfg.replicate, i, t);
+    return cfg.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
+}
+
+extern "C" double or_env_noise(const clotho_config* cfg, uint32_t generation, uint32_t individual,
+                               uint32_t trait) {
+    return env_noise(*cfg, generation, individual, trait);
+}
+
 extern "C" double or_fitness_value(const clotho_config* cfg, uint32_t n, const double* z) {
     FitnessOp op;
     op.generate(*cfg, n);
@@ -862,12 +875,8 @@ extern "C" int or_sim_step(or_sim* s) {
     for (uint32_t i = 0; i < N; ++i) {
         for (uint32_t t = 0; t < T; ++t)
             z[t] = s->traits[(size_t)(2 * i) * T + t] + s->traits[(size_t)(2 * i + 1) * T + t];
-        if (cfg.env_sigma > 0.0) {
-            for (uint32_t t = 0; t < T; ++t) {
-                Draw d = draw(cfg.seed, s->gen, S_ENV, cfg.replicate, i, t);
-                z[t] = z[t] + cfg.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
-            }
-        }
+        if (cfg.env_sigma > 0.0)
+            for (uint32_t t = 0; t < T; ++t) z[t] = z[t] + env_noise(cfg, s->gen, i, t);
         s->fitness[i] = op(z.data());
     }
     s->fitness_constant = all_equal(s->fitness.data(), N);

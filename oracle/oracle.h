@@ -55,6 +55,10 @@ void or_phenotype(const uint64_t* rows, uint32_t h, uint32_t w, const double* ef
 void or_select(const double* fitness, uint32_t n, uint64_t seed, uint32_t generation,
                uint32_t replicate, uint32_t* parents /* 2n */);
 double or_fitness_value(const clotho_config* cfg, uint32_t n, const double* z);
+/* Environmental noise env_sigma * Phi^-1(u) of (generation, individual, trait),
+   the draw the simulation adds to the diploid phenotype. */
+double or_env_noise(const clotho_config* cfg, uint32_t generation, uint32_t individual,
+                    uint32_t trait);
 
 /* Whole-simulation oracle. */
 typedef struct or_sim or_sim;

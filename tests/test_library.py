@@ -46,6 +46,8 @@ def test_invalid_configs_rejected_before_device():
         api.make_config(10, 1.0, 1.0, max_alleles=0),
         api.make_config(10, 1.0, 1.0, intermediates=2, compact=True),  # batch consumes the list
         api.make_config(10, 1.0, 1.0, max_individuals=(1 << 28) + 1),
+        # int8 digit sums of a row stay within int32 only up to 2^24 slots
+        api.make_config(10, 1.0, 1.0, max_alleles=(1 << 24) + 1),
     ]
     for cfg in bad:
         assert L.clotho_create(C.addressof(cfg), 0, C.byref(ctx)) == -1
@@ -71,3 +73,25 @@ def test_engine_adapter_registry_and_config():
     r = subprocess.run([demo, "--registry", "population_growth.params.B=0"],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "founder size" in r.stderr
+
+
+def test_exact_contraction_rejects_unquantisable_effects():
+    """CLOTHO_TRAITS_EXACT_I8 quantises effects to k 2^-q with the largest q
+    keeping |k| <= 2^30 - 1; non-finite effects, and magnitudes that would
+    need q outside [-60, 60], are rejected before any device work."""
+    rows = np.zeros((4, 8), np.uint64)
+    rows[:, 0] = 0b1011
+    good = np.full((512, 2), 0.25)
+    bad_sets = [
+        np.where(np.arange(1024).reshape(512, 2) == 7, np.inf, good),
+        np.where(np.arange(1024).reshape(512, 2) == 9, -np.inf, good),
+        np.where(np.arange(1024).reshape(512, 2) == 3, np.nan, good),
+        np.full((512, 2), 2.0**95),     # q = -66
+        np.full((512, 2), 2.0**-40),    # q = 69
+    ]
+    for eff in bad_sets:
+        with pytest.raises(api.ClothoError, match="invalid argument"):
+            api.trait_contract(rows, eff, api.TRAITS_EXACT_I8)
+        freqs = np.full(512, 0.5)
+        with pytest.raises(api.ClothoError, match="invalid argument"):
+            api.trait_contract_synthetic(4, freqs, eff, api.TRAITS_EXACT_I8)

@@ -541,3 +541,61 @@ def test_oracle_pairwise_differences_against_numpy():
         d = oracle.pairwise_differences(rows, sample)
         ref = [int((bits[a] != bits[b]).sum()) for i, a in enumerate(sample) for b in sample[i + 1:]]
         assert d.tolist() == ref
+
+
+# ------------------------------------------- new-allele and noise laws
+def test_new_allele_effect_law():
+    """TraitSpaceGenerator2 (include/clotho/data_spaces/phenotype_evaluator/
+    trait_space_generator.hpp:118-129) with weight_distribution_helper
+    (include/clotho/data_spaces/generators/weight_distribution_helper.hpp:24-33):
+    per allele and trait, independently, Bernoulli(neutral.p) gives 0, else
+    Normal(traits.mean, traits.sigma).  About 1e5 new alleles of one oracle
+    generation are checked against that law: the neutral fraction per trait
+    and jointly (independence), a KS test of the non-neutral values against
+    N(0.3, 0.7^2) (a sigma^2-for-sigma slip, a dropped mean or a shared
+    draw fails), and the 2^q quantisation grid."""
+    mean, sigma, p, q = 0.3, 0.7, 0.4, -24
+    cfg = api.make_config(2000, 25.0, 0.0, seed=7, traits=2, neutral_p=p, mean=mean,
+                          sigma=sigma, quantum_log2=q, max_alleles=1 << 17)
+    orc = oracle.OracleSim(cfg)
+    orc.step()
+    pos, eff, age = orc.alleles()
+    m = orc.shape().new_mutations
+    assert m == eff.shape[0] and m > 95_000
+    neutral = eff == 0.0
+    sd = math.sqrt(m * p * (1 - p))
+    for t in range(2):
+        assert abs(neutral[:, t].sum() - p * m) <= 4 * sd
+    both = np.logical_and(neutral[:, 0], neutral[:, 1]).sum()
+    assert abs(both - p * p * m) <= 4 * math.sqrt(m * p * p * (1 - p * p))
+    vals = eff[~neutral]
+    assert st.kstest(vals, "norm", args=(mean, sigma)).pvalue > 1e-3
+    assert abs(vals.mean() - mean) <= 4 * sigma / math.sqrt(vals.size)
+    assert abs(vals.std() - sigma) <= 4 * sigma / math.sqrt(2 * vals.size)
+    k = eff * 2.0**-q
+    assert np.array_equal(k, np.round(k)) and np.abs(k).max() <= 2**30 - 1
+    assert st.kstest(pos, "uniform").pvalue > 1e-3
+    assert np.all(age == 1)
+
+
+def test_environmental_noise_law():
+    """Environmental noise N(0, env_sigma^2) per individual and trait, added to
+    the diploid phenotype before fitness (no reference counterpart).  The
+    draws are KS-tested against N(0, 1.7^2), and an all-neutral population
+    with fixed-sigma Gaussian fitness exp(-z^2/(2 s^2)) must score exactly
+    those draws (traits are zero, so z is the noise alone)."""
+    env, s = 1.7, 2.0
+    cfg = api.make_config(20_000, 0.5, 1.0, seed=11, traits=1, neutral_p=1.0,
+                          fitness=api.FIT_GAUSSIAN_FIXED, fitness_scale=s, env_sigma=env,
+                          max_alleles=1 << 16)
+    noise = oracle.env_noise(cfg, 1, 20_000)
+    assert st.kstest(noise, "norm", args=(0.0, env)).pvalue > 1e-3
+    assert abs(noise.std() - env) <= 4 * env / math.sqrt(2 * noise.size)
+    orc = oracle.OracleSim(cfg)
+    orc.step()  # generation 1
+    assert np.all(orc.traits() == 0.0)
+    want = np.exp(-(noise * noise) / (2 * s * s))
+    np.testing.assert_allclose(orc.fitness(), want, rtol=1e-14, atol=0)
+    # a different trait index or generation is a different draw
+    other = oracle.env_noise(cfg, 2, 100)
+    assert not np.array_equal(other, noise[:100])
