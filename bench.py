@@ -228,7 +228,6 @@ def main():
         pos, eff, _ = sim.alleles()
         orc = oracle.OracleSim(cfg)
         orc.set_population(rows, pos, eff, None)
-        del rows
         # the reference's thread model: T contiguous offspring batches, one
         # per thread (examples/QTLSim/engine_parallel_pipeline.hpp:203-226),
         # on the CPU share of this process; then the same generation on one core
@@ -236,6 +235,7 @@ def main():
         threads = max(1, min(share, 16))
         orc_mt = oracle.OracleSim(cfg)
         orc_mt.set_population(rows, pos, eff, None)
+        del rows
         orc_mt.set_threads(threads)
         log(f"[bench] oracle loaded; timing one generation on {threads} threads, then one core")
         t0 = time.perf_counter()

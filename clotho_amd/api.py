@@ -26,6 +26,8 @@ TRAITS_EXACT_FP64 = 2
 TRAITS_WALK_FP32 = 3
 TRAITS_EXACT_I8 = 4
 
+FLAG_HOST_MUTATIONS = 1
+
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded", -4: "bad state"}
 
 
@@ -49,7 +51,7 @@ class Config(C.Structure):
         ("compact_every", C.c_uint32),
         ("max_individuals", C.c_uint32),
         ("intermediates", C.c_uint32),
-        ("reserved", C.c_uint32 * 1),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -78,7 +80,7 @@ class Shape(C.Structure):
 def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, neutral_p=0.5,
                 mean=0.0, sigma=1.0, fitness_scale=1.0, env_sigma=0.0, quantum_log2=-24,
                 max_alleles=1 << 20, replicate=0, compact=False, max_individuals=0,
-                intermediates=0):
+                intermediates=0, host_mutations=False):
     """Build a Config; defaults follow the reference's parameter defaults
     (sequence_bias.p 0.5, neutral.p 0.5, traits.mean 0, traits.sigma 1)."""
     c = Config()
@@ -100,6 +102,7 @@ def make_config(n, mu, rho, seed=42, traits=1, fitness=FIT_CONSTANT, bias=0.5, n
     c.compact_every = 1 if compact else 0
     c.max_individuals = max_individuals
     c.intermediates = intermediates
+    c.flags = FLAG_HOST_MUTATIONS if host_mutations else 0
     return c
 
 
@@ -125,6 +128,8 @@ EXPORTS = [
     "clotho_batch_set_timing", "clotho_compact", "clotho_device_bytes",
     "clotho_sample_rows", "clotho_allele_counts", "clotho_set_population_size",
     "clotho_pairwise_differences", "clotho_device_memory",
+    "clotho_set_parents", "clotho_add_mutations", "clotho_evaluate_fitness",
+    "clotho_crossover_rows", "clotho_classify_free_space",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -178,6 +183,13 @@ def lib():
                                                   C.POINTER(C.c_double), C.c_int]
     L.clotho_device_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.clotho_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.clotho_classify_free_space.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp,
+                                             C.c_int]
+    L.clotho_set_parents.argtypes = [vp, vp, C.c_uint32]
+    L.clotho_add_mutations.argtypes = [vp, vp, vp, vp, C.c_uint32, vp]
+    L.clotho_evaluate_fitness.argtypes = [vp]
+    L.clotho_crossover_rows.argtypes = [vp, vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32,
+                                        vp, C.c_uint32]
     L.clotho_sample_rows.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                      u32p]
     L.clotho_allele_counts.argtypes = [vp, vp, C.c_uint32, u32p, C.c_size_t, u32p,
@@ -366,6 +378,42 @@ class Simulation:
         self._check(lib().clotho_autotune(self.ctx, C.byref(ch)), "autotune")
         return ch.value
 
+    def set_parents(self, parents):
+        """Parent individual of every gamete of the next generation."""
+        parents = np.ascontiguousarray(parents, np.uint32)
+        self._check(lib().clotho_set_parents(self.ctx, parents.ctypes.data, parents.size),
+                    "set_parents")
+
+    def add_mutations(self, rows, positions, effects):
+        """New alleles on the current population; returns their slots."""
+        rows = np.ascontiguousarray(rows, np.uint32)
+        positions = np.ascontiguousarray(positions, np.float64)
+        effects = np.ascontiguousarray(effects, np.float64)
+        slots = np.zeros(max(rows.size, 1), np.uint32)
+        self._check(lib().clotho_add_mutations(self.ctx, rows.ctypes.data, positions.ctypes.data,
+                                               effects.ctypes.data, rows.size, slots.ctypes.data),
+                    "add_mutations")
+        return slots[: rows.size]
+
+    def evaluate_fitness(self):
+        self._check(lib().clotho_evaluate_fitness(self.ctx), "evaluate_fitness")
+
+    def crossover_rows(self, top, bottom, n_events, event_keys, out_words=None):
+        """Per-gamete recombination of rows top[g] / bottom[g] with event keys
+        event_keys[g, :n_events[g]] against the current allele positions."""
+        top = np.ascontiguousarray(top, np.uint64)
+        bottom = np.ascontiguousarray(bottom, np.uint64)
+        n_events = np.ascontiguousarray(n_events, np.uint32)
+        event_keys = np.ascontiguousarray(event_keys, np.uint32)
+        g, w = top.shape
+        ow = w if out_words is None else out_words
+        out = np.zeros((g, ow), np.uint64)
+        stride = event_keys.shape[1] if event_keys.ndim == 2 else 0
+        self._check(lib().clotho_crossover_rows(self.ctx, top.ctypes.data, bottom.ctypes.data, w,
+                                                n_events.ctypes.data, event_keys.ctypes.data,
+                                                stride, g, out.ctypes.data, ow), "crossover_rows")
+        return out
+
     def set_timing(self, on=True):
         lib().clotho_set_timing(self.ctx, 1 if on else 0)
 
@@ -398,6 +446,22 @@ def stream_copy_gbps(nbytes=1 << 30, repeats=20, device=0):
     if rc != 0:
         raise ClothoError(f"stream_copy failed ({ERRORS.get(rc, rc)})")
     return out.value
+
+
+def classify_free_space(fixed_and, variable_or, max_alleles, device=0):
+    """(fixed, lost) slots of reduced AND / OR column masks, below max_alleles."""
+    fa = np.ascontiguousarray(fixed_and, np.uint64)
+    vo = np.ascontiguousarray(variable_or, np.uint64)
+    cap = max(1, min(fa.size * 64, max_alleles))
+    fixed = np.zeros(cap, np.uint32)
+    lost = np.zeros(cap, np.uint32)
+    counts = np.zeros(2, np.uint32)
+    rc = lib().clotho_classify_free_space(fa.ctypes.data, vo.ctypes.data, fa.size, max_alleles,
+                                          fixed.ctypes.data, lost.ctypes.data,
+                                          counts.ctypes.data, device)
+    if rc != 0:
+        raise ClothoError(f"classify_free_space failed ({ERRORS.get(rc, rc)})")
+    return fixed[: counts[0]], lost[: counts[1]]
 
 
 def device_memory(device=0):

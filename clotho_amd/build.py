@@ -44,18 +44,20 @@ def build(verbose: bool = False) -> str:
 
 
 DEMO = os.path.join(HERE, "lib", "engine_demo")
+GENERIC_DEMO = os.path.join(HERE, "lib", "generic_demo")
 
 
 def build_host_demo(verbose: bool = False) -> str:
-    """The C++ adapter driver: plain g++ against the C ABI library."""
+    """The C++ adapter drivers: plain g++ against the C ABI library."""
     root = os.path.dirname(HERE)
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
-           os.path.join(HERE, "host", "engine_demo.cpp"), "-o", DEMO + ".tmp",
-           "-L", os.path.dirname(OUT), "-lclotho_amd", "-Wl,-rpath,$ORIGIN"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
-    os.replace(DEMO + ".tmp", DEMO)
+    for src, out in (("engine_demo.cpp", DEMO), ("generic_demo.cpp", GENERIC_DEMO)):
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
+               os.path.join(HERE, "host", src), "-o", out + ".tmp",
+               "-L", os.path.dirname(OUT), "-lclotho_amd", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
     return DEMO
 
 

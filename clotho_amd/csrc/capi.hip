@@ -68,7 +68,7 @@ void free_buffers(clotho_ctx* c) {
                     B.bottom,     B.n_events,  B.events,    B.var_part,   B.fix_part,
                     B.word_cnt,   B.word_bsum, B.cum_local, B.cum_block,  B.fit_stats,
                     B.cdf_rho,    B.cdf_mu,    B.state,     B.by_parent,  B.parent_cnt,
-                    B.parent_bsum, B.prank};
+                    B.parent_bsum, B.prank,    B.given_parents};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     std::memset(&B, 0, sizeof(B));
@@ -250,9 +250,12 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     if (!c) return CLOTHO_ERR_ARG;
     c->cfg = *cfg;
     c->device = device;
+    if (cfg->flags & ~CLOTHO_FLAG_HOST_MUTATIONS) return CLOTHO_ERR_ARG;
     double cdf_rho[kMaxCdf], cdf_mu[kMaxCdf];
     const int nr = poisson_table(cfg->rho, cdf_rho, kMaxCdf);
-    const int nm = poisson_table(cfg->mu, cdf_mu, kMaxCdf);
+    // host mutations: the per-gamete counts are Poisson(0), always zero
+    const double mu_drawn = (cfg->flags & CLOTHO_FLAG_HOST_MUTATIONS) ? 0.0 : cfg->mu;
+    const int nm = poisson_table(mu_drawn, cdf_mu, kMaxCdf);
     if (nr < 0 || nm < 0) {
         delete c;
         return CLOTHO_ERR_ARG;
@@ -338,6 +341,7 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.parent_cnt, P.n_cap);
     rc |= dalloc(c, &B.parent_bsum, sel_blocks);
     rc |= dalloc(c, &B.prank, h_cap);
+    rc |= dalloc(c, &B.given_parents, h_cap);
     rc |= dalloc(c, &B.var_part, (size_t)(P.max_chunks + 1) * P.w_cap);
     rc |= dalloc(c, &B.fix_part, (size_t)P.max_chunks * P.w_cap);
     rc |= dalloc(c, &B.word_cnt, P.w_cap);
@@ -808,6 +812,231 @@ int clotho_pairwise_differences(clotho_ctx* c, const uint32_t* rows, uint32_t n_
     if (e1) (void)hipEventDestroy(e1);
     if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
     if (kernel_ms) *kernel_ms = ms;
+    return CLOTHO_OK;
+}
+
+int clotho_set_parents(clotho_ctx* c, const uint32_t* parents, uint32_t gametes) {
+    if (!c || (!parents && gametes)) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t next_n = c->next_n ? c->next_n : c->p.n;
+    if (gametes != 2 * next_n)
+        return fail(c, CLOTHO_ERR_ARG, "set_parents: expected " + std::to_string(2 * next_n) +
+                                           " gamete parents (two per offspring individual)");
+    for (uint32_t o = 0; o < gametes; ++o)
+        if (parents[o] >= c->p.n)
+            return fail(c, CLOTHO_ERR_ARG, "set_parents: parent " + std::to_string(parents[o]) +
+                                               " outside the current population");
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(c->b.given_parents, parents, 4ull * gametes, hipMemcpyHostToDevice));
+    st.given = 1;
+    return write_state(c, st);
+}
+
+int clotho_add_mutations(clotho_ctx* c, const uint32_t* rows, const double* positions,
+                         const double* effects, uint32_t n, uint32_t* slots_out) {
+    if (!c) return CLOTHO_ERR_ARG;
+    if (n == 0) return CLOTHO_OK;
+    const Params& P = c->p;
+    if (!rows || !positions || (P.traits && !effects))
+        return fail(c, CLOTHO_ERR_ARG, "add_mutations: null buffer");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t T = P.traits;
+    std::vector<uint32_t> keys(n);
+    std::vector<int32_t> eq((size_t)n * std::max<uint32_t>(1, T), 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (rows[k] >= P.h) return fail(c, CLOTHO_ERR_ARG, "add_mutations: row out of range");
+        const double x = positions[k], kk = x * 4294967296.0;
+        if (!(x >= 0.0 && x < 1.0) || kk != std::floor(kk))
+            return fail(c, CLOTHO_ERR_ARG, "add_mutations: positions must be k*2^-32 in [0,1)");
+        keys[k] = (uint32_t)kk;
+        for (uint32_t t = 0; t < T; ++t) {
+            const double w = effects[(size_t)k * T + t] * P.eff_scale;
+            if (w != std::floor(w) || std::fabs(w) > kEffectLimit)
+                return fail(c, CLOTHO_ERR_ARG,
+                            "add_mutations: effects must be integer multiples of 2^q");
+            eq[(size_t)k * T + t] = (int32_t)w;
+        }
+    }
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    // slots: the free list (fixed ascending, lost ascending), then appended
+    const uint32_t F = st.free_count, A = st.all_size;
+    const uint64_t A1 = (uint64_t)A + (n > F ? n - F : 0);
+    if (A1 > P.s_cap)
+        return fail(c, CLOTHO_ERR_CAPACITY, "add_mutations: allele capacity exceeded");
+    std::vector<uint32_t> slots(n);
+    const uint32_t reuse = std::min(n, F);
+    if (reuse)
+        HIP_TRY(c, hipMemcpy(slots.data(), c->b.free_list + st.free_base, 4ull * reuse,
+                             hipMemcpyDeviceToHost));
+    for (uint32_t k = reuse; k < n; ++k) slots[k] = A + (k - F);
+    const uint32_t W0 = st.wc, W1 = (uint32_t)(((A1 + 511) / 512) * 8);
+    if (W1 > W0) {  // wider rows: re-stride into the other buffer, which becomes current
+        HIP_TRY(c, launch_restride(c, c->cur, W0, W1));
+        const size_t nt = (size_t)P.h * std::max<uint32_t>(1, T);
+        HIP_TRY(c, hipMemcpyAsync(c->b.traits[1 - c->cur], c->b.traits[c->cur], 8 * nt,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->b.fitness[1 - c->cur], c->b.fitness[c->cur], 8ull * P.n,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        c->cur = 1 - c->cur;
+    }
+    uint32_t *d_rows = nullptr, *d_slots = nullptr, *d_keys = nullptr;
+    int32_t* d_eff = nullptr;
+    uint64_t* d_mask = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    // reused fixed slots: the allele leaves every row first (removeFixedAlleles)
+    const uint32_t n_fixed_reused = std::min(reuse, st.n_fixed);
+    if (n_fixed_reused) {
+        std::vector<uint64_t> mask(W1, 0);
+        for (uint32_t k = 0; k < n_fixed_reused; ++k) mask[slots[k] / 64] |= 1ull << (slots[k] % 64);
+        step(hipMalloc((void**)&d_mask, 8ull * W1));
+        if (e == hipSuccess)
+            step(hipMemcpy(d_mask, mask.data(), 8ull * W1, hipMemcpyHostToDevice));
+        if (e == hipSuccess) step(launch_clear_columns(c, c->cur, W1, d_mask));
+    }
+    step(hipMalloc((void**)&d_rows, 4ull * n));
+    step(hipMalloc((void**)&d_slots, 4ull * n));
+    step(hipMalloc((void**)&d_keys, 4ull * n));
+    step(hipMalloc((void**)&d_eff, 4ull * eq.size()));
+    if (e == hipSuccess) step(hipMemcpy(d_rows, rows, 4ull * n, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_slots, slots.data(), 4ull * n, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_keys, keys.data(), 4ull * n, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_eff, eq.data(), 4ull * eq.size(), hipMemcpyHostToDevice));
+    // the age of the generation that produced the population
+    const uint32_t age = st.gen > 0 ? st.gen - 1 : 0;
+    if (e == hipSuccess)
+        step(launch_add_mutations(c, c->cur, W1, d_rows, d_slots, d_keys, d_eff, n, age));
+    step(hipStreamSynchronize(c->stream));
+    for (void* p : {(void*)d_rows, (void*)d_slots, (void*)d_keys, (void*)d_eff, (void*)d_mask})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    // traits and free space of the new population
+    st.alleles = st.all_size = (uint32_t)A1;
+    st.wc = st.wp = W1;
+    st.m = n;
+    rc = analyse_upload(c, st);
+    if (rc) return rc;
+    if (slots_out) std::copy(slots.begin(), slots.end(), slots_out);
+    return CLOTHO_OK;
+}
+
+int clotho_evaluate_fitness(clotho_ctx* c) {
+    if (!c) return CLOTHO_ERR_ARG;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, launch_fitness_current(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return CLOTHO_OK;
+}
+
+int clotho_crossover_rows(clotho_ctx* c, const uint64_t* top, const uint64_t* bottom,
+                          uint32_t words, const uint32_t* n_events, const uint32_t* event_keys,
+                          uint32_t event_stride, uint32_t gametes, uint64_t* out,
+                          uint32_t out_words) {
+    if (!c) return CLOTHO_ERR_ARG;
+    if (gametes == 0) return CLOTHO_OK;
+    if (!top || !bottom || !n_events || !out || out_words < words)
+        return fail(c, CLOTHO_ERR_ARG, "crossover_rows: null buffer or out_words < words");
+    HIP_TRY(c, hipSetDevice(c->device));
+    GenState st;
+    int rc = read_state(c, &st);
+    if (rc) return rc;
+    const uint32_t A = st.all_size;
+    if ((size_t)words * 64 > (size_t)c->p.s_cap)
+        return fail(c, CLOTHO_ERR_ARG, "crossover_rows: rows wider than the allele capacity");
+    size_t n_ev = 0;
+    for (uint32_t g = 0; g < gametes; ++g) {
+        if (n_events[g] > event_stride)
+            return fail(c, CLOTHO_ERR_ARG, "crossover_rows: more events than event_stride");
+        n_ev = std::max(n_ev, (size_t)n_events[g]);
+    }
+    if (n_ev && !event_keys) return fail(c, CLOTHO_ERR_ARG, "crossover_rows: null event keys");
+    // bits of slots >= A have no position
+    for (uint32_t g = 0; g < gametes; ++g)
+        for (uint32_t w = A / 64; w < words; ++w) {
+            const uint32_t lo = w * 64;
+            const uint64_t allowed =
+                A > lo ? (A - lo >= 64 ? ~0ull : ((1ull << (A - lo)) - 1)) : 0;
+            if ((top[(size_t)g * words + w] | bottom[(size_t)g * words + w]) & ~allowed)
+                return fail(c, CLOTHO_ERR_ARG, "crossover_rows: bit set beyond the allele count");
+        }
+    // the classifier sorts its events (region_classifier's constructor)
+    const uint32_t stride = std::max<uint32_t>(1, event_stride);
+    std::vector<uint32_t> ev((size_t)gametes * stride, 0);
+    for (uint32_t g = 0; g < gametes; ++g) {
+        const uint32_t ne = n_events[g];
+        if (ne) std::copy(event_keys + (size_t)g * event_stride,
+                          event_keys + (size_t)g * event_stride + ne, ev.begin() + (size_t)g * stride);
+        std::sort(ev.begin() + (size_t)g * stride, ev.begin() + (size_t)g * stride + ne);
+    }
+    const size_t rw = (size_t)gametes * words, ow = (size_t)gametes * out_words;
+    uint64_t *d_top = nullptr, *d_bot = nullptr, *d_out = nullptr;
+    uint32_t *d_nev = nullptr, *d_ev = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    step(hipMalloc((void**)&d_top, 8 * std::max<size_t>(rw, 1)));
+    step(hipMalloc((void**)&d_bot, 8 * std::max<size_t>(rw, 1)));
+    step(hipMalloc((void**)&d_out, 8 * ow));
+    step(hipMalloc((void**)&d_nev, 4ull * gametes));
+    step(hipMalloc((void**)&d_ev, 4 * ev.size()));
+    if (e == hipSuccess && rw) step(hipMemcpy(d_top, top, 8 * rw, hipMemcpyHostToDevice));
+    if (e == hipSuccess && rw) step(hipMemcpy(d_bot, bottom, 8 * rw, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_nev, n_events, 4ull * gametes, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_ev, ev.data(), 4 * ev.size(), hipMemcpyHostToDevice));
+    if (e == hipSuccess)
+        step(launch_crossover_rows(c, d_top, d_bot, words, d_nev, d_ev, stride, gametes, d_out,
+                                   out_words));
+    step(hipStreamSynchronize(c->stream));
+    if (e == hipSuccess) step(hipMemcpy(out, d_out, 8 * ow, hipMemcpyDeviceToHost));
+    for (void* p : {(void*)d_top, (void*)d_bot, (void*)d_out, (void*)d_nev, (void*)d_ev})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(c, CLOTHO_ERR_HIP, hipGetErrorString(e));
+    return CLOTHO_OK;
+}
+
+int clotho_classify_free_space(const uint64_t* fixed_and, const uint64_t* variable_or,
+                               uint32_t words, uint32_t max_alleles, uint32_t* fixed,
+                               uint32_t* lost, uint32_t* counts, int device) {
+    if (!fixed_and || !variable_or || !fixed || !lost || !counts || words == 0)
+        return CLOTHO_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CLOTHO_ERR_HIP;
+    const size_t cap = std::min<size_t>((size_t)words * 64, max_alleles);
+    uint64_t *d_fix = nullptr, *d_var = nullptr;
+    uint32_t *d_fixed = nullptr, *d_lost = nullptr, *d_counts = nullptr;
+    hipStream_t st = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    step(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    step(hipMalloc((void**)&d_fix, 8ull * words));
+    step(hipMalloc((void**)&d_var, 8ull * words));
+    step(hipMalloc((void**)&d_fixed, 4 * std::max<size_t>(cap, 1)));
+    step(hipMalloc((void**)&d_lost, 4 * std::max<size_t>(cap, 1)));
+    step(hipMalloc((void**)&d_counts, 8));
+    if (e == hipSuccess) step(hipMemcpy(d_fix, fixed_and, 8ull * words, hipMemcpyHostToDevice));
+    if (e == hipSuccess) step(hipMemcpy(d_var, variable_or, 8ull * words, hipMemcpyHostToDevice));
+    if (e == hipSuccess)
+        step(launch_classify_columns(d_fix, d_var, words, max_alleles, d_fixed, d_lost, d_counts,
+                                     st));
+    if (e == hipSuccess) step(hipStreamSynchronize(st));
+    uint32_t n[2] = {0, 0};
+    if (e == hipSuccess) step(hipMemcpy(n, d_counts, 8, hipMemcpyDeviceToHost));
+    if (e == hipSuccess && n[0]) step(hipMemcpy(fixed, d_fixed, 4ull * n[0], hipMemcpyDeviceToHost));
+    if (e == hipSuccess && n[1]) step(hipMemcpy(lost, d_lost, 4ull * n[1], hipMemcpyDeviceToHost));
+    for (void* p : {(void*)d_fix, (void*)d_var, (void*)d_fixed, (void*)d_lost, (void*)d_counts})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return CLOTHO_ERR_HIP;
+    counts[0] = n[0];
+    counts[1] = n[1];
     return CLOTHO_OK;
 }
 

@@ -442,9 +442,12 @@ struct GametesOp {
         const uint32_t i = o >> 1;
         const Draw4 pd = philox_draw(P.seed, gen, S_PARENTS, P.replicate, i, 0);
         const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
+        // given parents (one generation): Crossover::update's mate pairs
+        const bool given = uniform_read(&st->given) != 0;
         const uint32_t p =
-            staged ? pick_parent(P, false, (uint64_t)scanned_total, k, cum_local, (const double*)s_cb)
-                   : pick_parent(P, constant, st->sel_total, k, cum_local, cum_block);
+            given ? min(gbl(B.given_parents)[o], P.n_par - 1)
+            : staged ? pick_parent(P, false, (uint64_t)scanned_total, k, cum_local, (const double*)s_cb)
+                     : pick_parent(P, constant, st->sel_total, k, cum_local, cum_block);
         const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
         const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
         const bool swap = u53(g.x[2], g.x[3]) < P.bias;
@@ -1645,18 +1648,22 @@ __device__ inline double fitness_of(const Params& P, const double* z) {
     }
 }
 
-struct Fitness {
+// kCurrent: the current population [sel] after its generation was committed
+// (noise of the generation that produced it, gen - 1); else the offspring
+// population [1 - sel] of the generation in progress.
+template <bool kCurrent>
+struct FitnessOp {
     static constexpr int kThreads = kBlock;
     __host__ __device__ static uint32_t blocks(const Params& P) { return (P.n + kBlock - 1) / kBlock; }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t sel,
                                                uint32_t blk) {
-        (void)sel;
         const auto st = gbl(B.state);
-        const auto traits = gbl(B.traits[1 - sel]);
-        const auto fit = gbl(B.fitness[1 - sel]);
+        const uint32_t which = kCurrent ? sel : 1 - sel;
+        const auto traits = gbl(B.traits[which]);
+        const auto fit = gbl(B.fitness[which]);
         const uint32_t i = blk * kBlock + threadIdx.x;
         if (i >= P.n) return;
-        const uint32_t gen = st->gen;
+        const uint32_t gen = kCurrent ? st->gen - 1 : st->gen;
         double z[kMaxTraits];
         for (uint32_t t = 0; t < P.traits; ++t) {
             const int64_t g = traits[(size_t)(2 * i) * P.traits + t] +
@@ -1670,6 +1677,8 @@ struct Fitness {
         fit[i] = fitness_of(P, z);
     }
 };
+using Fitness = FitnessOp<false>;
+using FitnessCurrent = FitnessOp<true>;
 
 
 // Synthetic standing variation: bit j of row r set when Philox word
@@ -1705,6 +1714,7 @@ struct Commit {
         st->alleles = st->all_size;
         st->wp = st->wc;
         st->gen = st->gen + 1;
+        st->given = 0;
         if (P.inter_k) {
             if (st->inter < P.inter_k) {
                 const uint32_t F = st->free_count, used = min(st->m, F);
@@ -1867,6 +1877,7 @@ CLOTHO_SINGLE(k_traits_i8_pop_t32, Traits32)
 CLOTHO_SINGLE(k_free_reduce, FreeReduce)
 CLOTHO_SINGLE(k_free_compact, FreeCompact)
 CLOTHO_SINGLE(k_fitness, Fitness)
+CLOTHO_SINGLE(k_fitness_current, FitnessCurrent)
 CLOTHO_SINGLE(k_commit, Commit)
 // more shared launches of a single-population generation: the effect digit
 // planes (new effects are final after the draws) with the parent-order counts,
@@ -2014,6 +2025,10 @@ hipError_t launch_compact_fitness(clotho_ctx* c) {
 }
 hipError_t launch_fitness(clotho_ctx* c) {
     return launch_single<Fitness>(k_fitness, c, (uint32_t)c->cur);
+}
+
+hipError_t launch_fitness_current(clotho_ctx* c) {
+    return launch_single<FitnessCurrent>(k_fitness_current, c, (uint32_t)c->cur);
 }
 
 hipError_t launch_commit(clotho_ctx* c) { return launch_single<Commit>(k_commit, c, 0); }
@@ -2208,6 +2223,176 @@ hipError_t launch_pairwise(clotho_ctx* c, const uint64_t* rows, uint32_t W,
     if (!blocks) return hipSuccess;
     hipLaunchKernelGGL(k_pairwise, dim3(blocks), dim3(256), 0, c->stream, rows, W, d_sample, n,
                        tiles, d_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Host-directed operations of the generic operator families: row re-stride,
+// column clears, mutations at given rows and slots, per-gamete recombination
+// of given rows.  None is on the generation's path.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void __launch_bounds__(kBlock) k_restride(const uint64_t* __restrict__ src, uint32_t w0,
+                                                     uint64_t* __restrict__ dst, uint32_t w1,
+                                                     uint32_t h) {
+    const size_t id = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (size_t)h * w1) return;
+    const uint32_t r = (uint32_t)(id / w1), w = (uint32_t)(id % w1);
+    dst[id] = w < w0 ? src[(size_t)r * w0 + w] : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_columns(uint64_t* __restrict__ rows, uint32_t w,
+                                                          uint32_t h,
+                                                          const uint64_t* __restrict__ mask) {
+    const size_t id = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (size_t)h * w) return;
+    rows[id] &= ~mask[id % w];
+}
+
+// Mutation k: allele attributes into slot slots[k], bit slots[k] of row
+// rows[k] set.  Slots are distinct, so the bits are; two mutations of one
+// row word meet in the atomic OR.
+__global__ void __launch_bounds__(kBlock) k_add_mutations(
+    uint64_t* __restrict__ pop, uint32_t w, const uint32_t* __restrict__ rows,
+    const uint32_t* __restrict__ slots, const uint32_t* __restrict__ keys_in,
+    const int32_t* __restrict__ eff_in, uint32_t n, uint32_t traits, uint32_t age,
+    uint32_t* __restrict__ keys, int32_t* __restrict__ effects, uint32_t* __restrict__ ages) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t slot = slots[k];
+    keys[slot] = keys_in[k];
+    ages[slot] = age;
+    for (uint32_t t = 0; t < traits; ++t)
+        effects[(size_t)slot * traits + t] = eff_in[(size_t)k * traits + t];
+    atomicOr((unsigned long long*)(pop + (size_t)rows[k] * w + slot / 64),
+             (unsigned long long)1 << (slot % 64));
+}
+
+// One (gamete, word) per thread: mask bit j = odd #{events e <= key_j} over
+// the heterozygous bits j (block_crossover with region_classifier's
+// upper_bound), out = (top & ~mask) | (bottom & mask); words past the input
+// width are zero.
+__global__ void __launch_bounds__(kBlock) k_crossover_rows(
+    const uint32_t* __restrict__ keys, const uint64_t* __restrict__ top,
+    const uint64_t* __restrict__ bottom, uint32_t w, const uint32_t* __restrict__ nev,
+    const uint32_t* __restrict__ ev, uint32_t stride, uint32_t gametes,
+    uint64_t* __restrict__ out, uint32_t ow) {
+    const size_t id = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (id >= (size_t)gametes * ow) return;
+    const uint32_t g = (uint32_t)(id / ow), x = (uint32_t)(id % ow);
+    if (x >= w) {
+        out[id] = 0;
+        return;
+    }
+    const uint64_t t = top[(size_t)g * w + x], b = bottom[(size_t)g * w + x];
+    const uint32_t ne = nev[g];
+    const uint32_t* e = ev + (size_t)g * stride;
+    uint64_t het = t ^ b, mask = 0;
+    while (het) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(het);
+        het &= het - 1;
+        const uint32_t key = keys[x * 64 + j];
+        uint32_t lo = 0, hi = ne;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (e[mid] <= key)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo & 1) mask |= (uint64_t)1 << j;
+    }
+    out[id] = (t & ~mask) | (b & mask);
+}
+
+uint32_t grid_of(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// analyze_free_indices over reduced column masks
+// (include/clotho/data_spaces/free_space/free_space_details.hpp:359-408):
+// fixed = set bits of the AND mask, lost = set bits of the complemented OR
+// mask, both below M, fixed ascending then lost ascending.  One block walks
+// the words 256 at a time, a word's counts (fixed << 32 | lost) scanned
+// across the block and carried over the chunks.
+__global__ void __launch_bounds__(kBlock) k_classify_columns(
+    const uint64_t* __restrict__ fix_and, const uint64_t* __restrict__ var_or, uint32_t words,
+    uint32_t M, uint32_t* __restrict__ fixed, uint32_t* __restrict__ lost,
+    uint32_t* __restrict__ counts) {
+    __shared__ uint64_t lds[kWaves];
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < words; base += kBlock) {
+        const uint32_t w = base + threadIdx.x;
+        uint64_t fx = 0, ls = 0;
+        if (w < words) {
+            const uint64_t valid = (uint64_t)w * 64 >= M       ? 0
+                                 : (uint64_t)w * 64 + 64 <= M ? ~(uint64_t)0
+                                                              : (((uint64_t)1 << (M - w * 64)) - 1);
+            fx = fix_and[w] & valid;
+            ls = ~var_or[w] & valid;
+        }
+        const uint64_t c = ((uint64_t)__popcll(fx) << 32) | (uint64_t)__popcll(ls);
+        uint64_t total;
+        const uint64_t ex = carry + block_inclusive_scan<uint64_t>(c, lds, total) - c;
+        uint32_t f = (uint32_t)(ex >> 32), l = (uint32_t)ex;
+        while (fx) {
+            fixed[f++] = w * 64 + (uint32_t)__builtin_ctzll(fx);
+            fx &= fx - 1;
+        }
+        while (ls) {
+            lost[l++] = w * 64 + (uint32_t)__builtin_ctzll(ls);
+            ls &= ls - 1;
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        counts[0] = (uint32_t)(carry >> 32);
+        counts[1] = (uint32_t)carry;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_classify_columns(const uint64_t* d_fix, const uint64_t* d_var, uint32_t words,
+                                   uint32_t M, uint32_t* d_fixed, uint32_t* d_lost,
+                                   uint32_t* d_counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_classify_columns, dim3(1), dim3(kBlock), 0, s, d_fix, d_var, words, M,
+                       d_fixed, d_lost, d_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_restride(clotho_ctx* c, int src, uint32_t w0, uint32_t w1) {
+    const size_t n = (size_t)c->p.h * w1;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_restride, dim3(grid_of(n)), dim3(kBlock), 0, c->stream, c->b.rows[src],
+                       w0, c->b.rows[1 - src], w1, c->p.h);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_columns(clotho_ctx* c, int which, uint32_t w, const uint64_t* d_mask) {
+    const size_t n = (size_t)c->p.h * w;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_columns, dim3(grid_of(n)), dim3(kBlock), 0, c->stream,
+                       c->b.rows[which], w, c->p.h, d_mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_mutations(clotho_ctx* c, int which, uint32_t w, const uint32_t* d_rows,
+                                const uint32_t* d_slots, const uint32_t* d_keys,
+                                const int32_t* d_eff, uint32_t n, uint32_t age) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_add_mutations, dim3(grid_of(n)), dim3(kBlock), 0, c->stream,
+                       c->b.rows[which], w, d_rows, d_slots, d_keys, d_eff, n, c->p.traits, age,
+                       c->b.keys, c->b.effects, c->b.ages);
+    return hipGetLastError();
+}
+
+hipError_t launch_crossover_rows(clotho_ctx* c, const uint64_t* d_top, const uint64_t* d_bottom,
+                                 uint32_t w, const uint32_t* d_nev, const uint32_t* d_ev,
+                                 uint32_t stride, uint32_t gametes, uint64_t* d_out, uint32_t ow) {
+    const size_t n = (size_t)gametes * ow;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_crossover_rows, dim3(grid_of(n)), dim3(kBlock), 0, c->stream, c->b.keys,
+                       d_top, d_bottom, w, d_nev, d_ev, stride, gametes, d_out, ow);
     return hipGetLastError();
 }
 

@@ -39,6 +39,7 @@ struct GenState {
     uint32_t kept;         // variable columns kept by the last compaction plan
     uint32_t free_base;    // free-list entries used by earlier intermediate generations
     uint32_t inter;        // position in the intermediate batch, 0 .. inter_k
+    uint32_t given;        // 1: the next draw takes Buffers::given_parents (one generation)
 };
 
 // Kernel-side constants that do not change between generations.
@@ -101,6 +102,7 @@ struct Buffers {
     uint32_t* parent_cnt;     // [n] children per parent -> exclusive in-block offsets
     uint32_t* prank;          // [h] rank of the gamete among its parent's picks
     uint32_t* parent_bsum;    // [ceil(n/256)] block totals -> exclusive offsets
+    uint32_t* given_parents;  // [h] parent individual of each gamete when GenState::given
     uint64_t* var_part;       // [chunks][w_cap] OR partials
     uint64_t* fix_part;       // [chunks][w_cap] AND partials
     uint32_t* word_cnt;       // [w_cap] (fixed count << 16 | lost count) per word
@@ -206,6 +208,26 @@ hipError_t launch_recombine(clotho_ctx* c);
 hipError_t launch_traits(clotho_ctx* c);
 hipError_t launch_free_space(clotho_ctx* c);
 hipError_t launch_fitness(clotho_ctx* c);
+// fitness of the current population from its traits (noise of generation gen - 1)
+hipError_t launch_fitness_current(clotho_ctx* c);
+// rows of population [src] re-strided from w0 to w1 >= w0 words into [1 - src]
+hipError_t launch_restride(clotho_ctx* c, int src, uint32_t w0, uint32_t w1);
+// clear the columns of `mask` (w words) in every row of population [which]
+hipError_t launch_clear_columns(clotho_ctx* c, int which, uint32_t w, const uint64_t* d_mask);
+// host-directed mutations into population [which] (width w): mutation k sets
+// slot slots[k] in row rows[k] and writes the slot's key, effects and age
+hipError_t launch_add_mutations(clotho_ctx* c, int which, uint32_t w, const uint32_t* d_rows,
+                                const uint32_t* d_slots, const uint32_t* d_keys,
+                                const int32_t* d_eff, uint32_t n, uint32_t age);
+// ordered fixed / lost slots of reduced column masks (AND, OR) below M;
+// counts = {fixed, lost}
+hipError_t launch_classify_columns(const uint64_t* d_fix, const uint64_t* d_var, uint32_t words,
+                                   uint32_t M, uint32_t* d_fixed, uint32_t* d_lost,
+                                   uint32_t* d_counts, hipStream_t s);
+// per-gamete recombination of rows against the allele keys (sorted events per gamete)
+hipError_t launch_crossover_rows(clotho_ctx* c, const uint64_t* d_top, const uint64_t* d_bottom,
+                                 uint32_t w, const uint32_t* d_nev, const uint32_t* d_ev,
+                                 uint32_t stride, uint32_t gametes, uint64_t* d_out, uint32_t ow);
 // the generation's late launches: the reduction alone, then the free-list
 // compaction and the offspring fitness in one launch
 hipError_t launch_free_reduce(clotho_ctx* c);

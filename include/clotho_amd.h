@@ -109,8 +109,15 @@ typedef struct clotho_config {
                                  space analyses (QS/engine_parallel_pipeline_batched.hpp);
                                  0 = analysis after every generation; not with
                                  compact_every                                          */
-    uint32_t reserved[1];
+    uint32_t flags;           /* CLOTHO_FLAG_*                                           */
 } clotho_config;
+
+/* clotho_config.flags */
+#define CLOTHO_FLAG_HOST_MUTATIONS 1u /* generations draw no mutations of their own: new
+                                         alleles come from clotho_add_mutations (the
+                                         reference's base engine, where MutationGenerator
+                                         adds mutation_allocator's count after crossover);
+                                         mu still sets the fitness constants            */
 
 /* Shape and bookkeeping of the current (child) population after a generation. */
 typedef struct clotho_shape {
@@ -246,6 +253,59 @@ int clotho_sync(clotho_ctx* ctx);
  * every generation of clotho_run / clotho_generation (phase-level callers
  * call it themselves). */
 int clotho_compact(clotho_ctx* ctx);
+
+/* ---- the generic operator families ------------------------------------ */
+/* Parent individual of every offspring gamete of the next generation, in
+ * place of the fitness-proportional draw (Crossover<RNG,GS>::update(parents,
+ * mate_pairs, offspring), DS/crossover/crossover.hpp:49-72, mate pair i =
+ * (parents[2i], parents[2i+1])): gamete o recombines the two rows of
+ * individual parents[o], events and strand swap drawn as usual.  gametes
+ * must be twice the next generation's individuals and every entry below the
+ * current population size.  Applies to the next generation only. */
+int clotho_set_parents(clotho_ctx* ctx, const uint32_t* parents, uint32_t gametes);
+/* n new alleles on the current population
+ * (MutationGenerator<RNG,GS>::operator()(child, N), DS/mutation/mutation_generator.hpp:41-52):
+ * mutation k, in order, takes the next free slot (the free list, fixed
+ * ascending then lost ascending, then appended slots), sets it in row rows[k]
+ * and gives it position positions[k] (k 2^-32 in [0,1)), effects
+ * effects[k][T] (integer multiples of 2^q) and the age of the generation
+ * that produced the population.  A reused fixed slot is first cleared from
+ * every row (its allele leaves the population, as removeFixedAlleles does);
+ * rows widen when appended slots need it.  Traits and the free-space lists
+ * are recomputed, fitness is not (clotho_evaluate_fitness).  slots (may be
+ * NULL) receives each mutation's slot.  Synchronous. */
+int clotho_add_mutations(clotho_ctx* ctx, const uint32_t* rows, const double* positions,
+                         const double* effects, uint32_t n, uint32_t* slots);
+/* Fitness of every individual of the current population from its genome
+ * traits, phenotype = trait(2i) + trait(2i+1) (+ environmental noise of the
+ * generation that produced the population), with the configured metric
+ * (Fitness<phenotype_evaluator<...>>::update(pop, eval),
+ * DS/fitness/phenotype_evaluator_fitness.hpp:51-69).  Synchronous. */
+int clotho_evaluate_fitness(clotho_ctx* ctx);
+/* Per-gamete recombination of host rows against the current population's
+ * allele positions (common_crossover::operator()(p0b, p0e, p1b, p1e, ob, oe),
+ * DS/crossover/common_crossover_row_block_alignment.hpp:53-93, with the strand
+ * choice made by the caller): out[g] takes top[g] at slots with an even
+ * number of events e <= position, bottom[g] at odd ones, and is zero past
+ * `words` up to out_words; no events copies top.  event_keys[g * event_stride
+ * + e] (e < n_events[g], any order) are positions k 2^-32 as uint32.  Bits of
+ * slots at or beyond the allele count must be clear.  Synchronous. */
+int clotho_crossover_rows(clotho_ctx* ctx, const uint64_t* top, const uint64_t* bottom,
+                          uint32_t words, const uint32_t* n_events, const uint32_t* event_keys,
+                          uint32_t event_stride, uint32_t gametes, uint64_t* out,
+                          uint32_t out_words);
+
+/* Ordered free-space classification of reduced column masks, for rows
+ * accumulated on the host (free_space_accumulator_mt::finalize ->
+ * free_space_details::analyze_free_indices,
+ * DS/free_space/free_space_accumulator.hpp:142-151,
+ * DS/free_space/free_space_details.hpp:359-408): fixed = set bits of
+ * fixed_and, lost = set bits of ~variable_or, both below max_alleles, each
+ * ascending; counts = {fixed, lost}.  fixed and lost hold up to
+ * min(64 words, max_alleles) entries each.  Context-free, synchronous. */
+int clotho_classify_free_space(const uint64_t* fixed_and, const uint64_t* variable_or,
+                               uint32_t words, uint32_t max_alleles, uint32_t* fixed,
+                               uint32_t* lost, uint32_t* counts, int device);
 
 /* Kernel timing over the last clotho_run: milliseconds summed over the
  * generations for [0] whole generations, [1] draw, [2] recombine+mutate,

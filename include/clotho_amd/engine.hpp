@@ -21,11 +21,28 @@
 //   GeneralFitness         DS/fitness/general_fitness.hpp:30-284
 //   Engine<..., mi355x_pipeline>
 //                          QS/engine_parallel_pipeline.hpp:59-483
+// and the generic families of the reference's base engine
+// (QS/qtlsim_engine.hpp:123-160):
+//   Crossover<RNG,GS>::update, crossover_method<RNG,A,S>
+//                          DS/crossover/crossover.hpp:27-80,
+//                          DS/crossover/crossover_method_def.hpp:20-21
+//   MutationGenerator<RNG,GS>::operator()(child, N)
+//                          DS/mutation/mutation_generator.hpp:25-59
+//   TraitWeightAccumulator, phenotype_evaluator, Fitness<phenotype_evaluator<...>>::update
+//                          DS/phenotype_evaluator/trait_accumulator.hpp:67-84,
+//                          DS/phenotype_evaluator/phenotype_evaluator.hpp:52-76,
+//                          DS/fitness/phenotype_evaluator_fitness.hpp:25-72
+//   FreeSpaceAnalyzer<GS>::update, free_space_evaluator<Space,Size>
+//                          DS/free_space/free_space.hpp:29-78,
+//                          DS/free_space/free_space_evaluator_def.hpp:20-21,
+//                          DS/free_space/row_grouped_free_evaluator.hpp:121-199
 //
-// The per-gamete and per-thread granularity of the reference collapses into
-// whole-generation device phases: each adapter delegates to the phase entry
-// point that covers its work.  Errors from the C ABI throw std::runtime_error
-// (the reference asserts).
+// The per-gamete and per-thread granularity of the pipeline engine collapses
+// into whole-generation device phases: each adapter delegates to the phase
+// entry point that covers its work.  The generic families keep their own
+// granularity (mate pairs, explicit mutation counts, single gametes) through
+// the C ABI's host-directed entry points.  Errors from the C ABI throw
+// std::runtime_error (the reference asserts).
 #ifndef CLOTHO_AMD_ENGINE_HPP_
 #define CLOTHO_AMD_ENGINE_HPP_
 
@@ -37,6 +54,7 @@
 #include <map>
 #include <memory>
 #include <ostream>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -274,6 +292,8 @@ inline clotho_config make_config(property_tree& config) {
     // generations.intermediates (QS/engine_parallel_pipeline_batched.hpp:155-166), written back
     c.intermediates = (uint32_t)config.get("generations.intermediates", 0.0);
     if (!config.has("generations.intermediates")) config.put("generations.intermediates", 0.0);
+    // no reference key: generations draw no mutations, MutationGenerator adds them
+    c.flags = config.get("mutation.host_directed", 0.0) != 0.0 ? CLOTHO_FLAG_HOST_MUTATIONS : 0u;
     auto gen = fitness_toolkit::getInstance()->get_tool(config);
     c.fitness_kind = gen ? gen->device_kind() : CLOTHO_FIT_CONSTANT;
     c.fitness_scale = gen ? gen->device_scale() : 1.0;
@@ -313,6 +333,16 @@ public:
         check(clotho_download_traits(m_ctx, t.data(), t.size()), m_ctx, "download_traits");
         return t;
     }
+    std::vector<double> fitness() const {
+        std::vector<double> f(getIndividualCount());
+        check(clotho_download_fitness(m_ctx, f.data(), f.size()), m_ctx, "download_fitness");
+        return f;
+    }
+    // genetic_space accessors of the generic families
+    size_t sequence_count() const { return haploid_genome_count(); }
+    size_t individual_count() const { return getIndividualCount(); }
+    size_t allele_count() const { return getMaxAlleles(); }
+    unsigned int trait_count() const { return m_cfg.trait_count; }
 
 private:
     clotho_config m_cfg;
@@ -332,72 +362,208 @@ struct allele_space_view {
 // operator families
 // ---------------------------------------------------------------------------
 
-// mutation_allocator<RNG, IntType>: the reference draws M ~ Poisson(mu * N) on
-// the host; here the per-gamete Poisson(mu) counts are drawn on the device in
-// the draw phase.  allocate(N) returns the expected count mu * N; the drawn M
-// of the last generation is new_mutations() (shape().new_mutations).
+// mutation_allocator<RNG, IntType>: allocate(N) draws M ~ Poisson(mu * N) from
+// the caller's engine, as the reference does, for callers that size their own
+// mutation batch (MutationGenerator).  A pipeline context instead draws an
+// independent Poisson(mu) count per gamete on the device (the same law for
+// the total); drawn(pop) is the M of that population's last generation.
 template <class RNG, class IntType = size_t>
 class mutation_allocator {
 public:
     mutation_allocator(RNG* rng, property_tree& config)
         : m_rng(rng), m_mu(config.get("mutation.mu", 1e-4)) {}
-    IntType allocate(size_t N) const { return (IntType)std::llround(m_mu * (double)N); }
+    IntType allocate(size_t N) {
+        const double lambda = m_mu * (double)N;
+        if (!(lambda > 0.0)) return 0;
+        std::poisson_distribution<long long> dist(lambda);
+        return (IntType)dist(*m_rng);
+    }
+    static IntType drawn(const device_population& pop) {
+        return (IntType)pop.shape().new_mutations;
+    }
 
 private:
     RNG* m_rng;
     double m_mu;
 };
 
-// common_crossover<RNG, population, alleles>: recombination + mutation scatter
-// of every gamete of the generation (the draw phase supplies parents, events
-// and strand swaps).
+namespace detail {
+// Single gametes queued for one device call (clotho_crossover_rows): each
+// keeps its input rows, event keys and output location until flush().
+class gamete_queue {
+public:
+    void push(const uint64_t* top, const uint64_t* bottom, uint32_t words,
+              std::vector<uint32_t> keys, uint64_t* out, uint32_t out_words) {
+        if (!m_items.empty() && (words != m_words || out_words != m_out_words))
+            throw std::runtime_error("gamete_queue: rows of one batch share their widths");
+        m_words = words;
+        m_out_words = out_words;
+        m_items.push_back(item{top, bottom, std::move(keys), out});
+    }
+    size_t size() const { return m_items.size(); }
+    void flush(clotho_ctx* ctx) {
+        if (m_items.empty()) return;
+        const uint32_t G = (uint32_t)m_items.size(), W = m_words, OW = m_out_words;
+        size_t stride = 1;
+        for (const item& it : m_items) stride = std::max(stride, it.keys.size());
+        std::vector<uint64_t> top((size_t)G * W), bot((size_t)G * W), out((size_t)G * OW);
+        std::vector<uint32_t> nev(G), keys((size_t)G * stride, 0);
+        for (uint32_t g = 0; g < G; ++g) {
+            const item& it = m_items[g];
+            std::copy(it.top, it.top + W, top.begin() + (size_t)g * W);
+            std::copy(it.bottom, it.bottom + W, bot.begin() + (size_t)g * W);
+            nev[g] = (uint32_t)it.keys.size();
+            std::copy(it.keys.begin(), it.keys.end(), keys.begin() + (size_t)g * stride);
+        }
+        std::vector<item> items;
+        items.swap(m_items);
+        check(clotho_crossover_rows(ctx, top.data(), bot.data(), W, nev.data(), keys.data(),
+                                    (uint32_t)stride, G, out.data(), OW),
+              ctx, "crossover_rows");
+        for (uint32_t g = 0; g < G; ++g)
+            std::copy(out.begin() + (size_t)g * OW, out.begin() + (size_t)(g + 1) * OW,
+                      items[g].out);
+    }
+
+private:
+    struct item {
+        const uint64_t *top, *bottom;
+        std::vector<uint32_t> keys;
+        uint64_t* out;
+    };
+    std::vector<item> m_items;
+    uint32_t m_words = 0, m_out_words = 0;
+};
+}  // namespace detail
+
+// crossover_method<RNG, AlleleSpace, SequenceSpace>: one gamete of one parent
+// genome at a time, the generic recombination method
+// (DS/crossover/crossover_method_def.hpp:20-21; the row-grouped instance,
+// DS/crossover/row_grouped_crossover_method.hpp:108-372, draws its events with
+// small_crossover_event_generator, DS/generators/small_crossover_event_generator.hpp:99-154).
+// Events come from the caller's engine: E ~ Poisson(rho), E positions uniform
+// on [0, 1) as k 2^-32, a strand swap ~ Bernoulli(bias); the rows are
+// recombined on the device against the allele positions of the population
+// given to initAlleles, with the pipeline's tie rule (events <= position
+// count).  A genome is the individual's two consecutive rows.  Gametes run
+// one device call each, or one call per batch between begin_batch() and
+// flush().
+template <class RNG, class AlleleSpace, class SequenceSpace>
+class crossover_method {
+public:
+    typedef uint64_t block_type;
+    crossover_method(RNG* rng, property_tree& config)
+        : m_rng(rng),
+          m_rho(config.get("recombination.rho", 1e-4)),
+          m_bias(config.get("recombination.sequence_bias.p", 0.5)) {}
+    crossover_method(RNG* rng, double rho, double bias) : m_rng(rng), m_rho(rho), m_bias(bias) {}
+
+    void initAlleles(SequenceSpace& pop) { m_pop = &pop; }
+
+    // one gamete of the genome whose first row starts at `genome` (rows of
+    // p_step words): the recombined row is written to c (c_step words)
+    void crossover(const block_type* genome, size_t p_step, block_type* c, size_t c_step) {
+        const bool swap = std::bernoulli_distribution(m_bias)(*m_rng);
+        const block_type* first = genome;
+        const block_type* second = genome + p_step;
+        recombine(swap ? second : first, swap ? first : second, (uint32_t)p_step, c,
+                  (uint32_t)c_step);
+    }
+    // the recombination of two given strands, top first (no strand draw)
+    void recombine(const block_type* top, const block_type* bottom, uint32_t words,
+                   block_type* out, uint32_t out_words) {
+        if (!m_pop) throw std::runtime_error("crossover_method: initAlleles first");
+        const uint32_t E = m_rho > 0.0 ? (uint32_t)std::poisson_distribution<int>(m_rho)(*m_rng) : 0u;
+        std::uniform_int_distribution<uint32_t> key(0u, 0xFFFFFFFFu);
+        std::vector<uint32_t> keys(E);
+        for (uint32_t& k : keys) k = key(*m_rng);
+        m_events.push_back(keys);
+        m_queue.push(top, bottom, words, std::move(keys), out, out_words);
+        if (!m_batch) flush();
+    }
+    void begin_batch() { m_batch = true; }
+    void flush() {
+        if (m_pop) m_queue.flush(m_pop->ctx());
+        m_batch = false;
+    }
+    // event positions (k 2^-32 as uint32) of every gamete since the last
+    // clear_events(), in call order
+    const std::vector<std::vector<uint32_t>>& events() const { return m_events; }
+    void clear_events() { m_events.clear(); }
+    size_t pending() const { return m_queue.size(); }
+    double rho() const { return m_rho; }
+    double bias() const { return m_bias; }
+
+private:
+    RNG* m_rng;
+    double m_rho, m_bias;
+    SequenceSpace* m_pop = nullptr;
+    bool m_batch = false;
+    detail::gamete_queue m_queue;
+    std::vector<std::vector<uint32_t>> m_events;
+};
+
+// common_crossover<RNG, population, alleles>: operator()(Population&) is the
+// recombination + mutation scatter of every gamete of the generation (the draw
+// phase supplies parents, events and strand swaps); the reference's
+// per-gamete operator()(p0b, p0e, p1b, p1e, ob, oe)
+// (DS/crossover/common_crossover_row_block_alignment.hpp:53-93) draws E ~
+// Poisson(rho), E positions and the strand swap from the caller's engine and
+// recombines the given rows on the device against the allele positions of the
+// population given to bind(); gametes queue into one device call between
+// begin_batch() and flush(), else each runs at once.
 template <class RNG, class Population, class Alleles>
 class common_crossover {
 public:
+    typedef uint64_t block_type;
     common_crossover(RNG* rng, Alleles* alleles, double recomb_rate, double bias_rate)
-        : m_rng(rng), m_alleles(alleles), m_rho(recomb_rate), m_bias(bias_rate) {}
+        : m_rng(rng), m_alleles(alleles), m_method(rng, recomb_rate, bias_rate) {}
     void operator()(Population& pop) {
         check(clotho_phase_recombine_mutate(pop.ctx()), pop.ctx(), "phase_recombine_mutate");
+    }
+    void bind(Population& pop) { m_method.initAlleles(pop); }
+    void operator()(const block_type* p0_start, const block_type* p0_end,
+                    const block_type* p1_start, const block_type* p1_end, block_type* offspring,
+                    block_type* offspring_end) {
+        if (p1_end - p1_start != p0_end - p0_start || offspring_end - offspring < p0_end - p0_start)
+            throw std::runtime_error("common_crossover: parent rows of one width, offspring wider");
+        const bool swap = std::bernoulli_distribution(m_method.bias())(*m_rng);
+        m_swaps.push_back(swap);
+        m_method.recombine(swap ? p1_start : p0_start, swap ? p0_start : p1_start,
+                           (uint32_t)(p0_end - p0_start), offspring,
+                           (uint32_t)(offspring_end - offspring));
+    }
+    void begin_batch() { m_method.begin_batch(); }
+    void flush() { m_method.flush(); }
+    // strand swaps and event positions of every gamete since clear_events()
+    const std::vector<bool>& swaps() const { return m_swaps; }
+    const std::vector<std::vector<uint32_t>>& events() const { return m_method.events(); }
+    void clear_events() {
+        m_swaps.clear();
+        m_method.clear_events();
     }
 
 private:
     RNG* m_rng;
     Alleles* m_alleles;
-    double m_rho, m_bias;
+    crossover_method<RNG, Alleles, Population> m_method;
+    std::vector<bool> m_swaps;
 };
 
-// free_space_accumulator_mt<Block, Size> with the free_space_details accessors.
-template <class BlockType = uint64_t, class SizeType = unsigned int>
-class free_space_accumulator_mt {
+// free_space_details<Size> accessors over a fixed-then-lost index list
+// (DS/free_space/free_space_details.hpp:193-277).
+template <class SizeType = unsigned int>
+class free_space_details {
 public:
     typedef SizeType size_type;
     typedef const size_type* iterator;
-    explicit free_space_accumulator_mt(unsigned int /*thread count*/) {}
-    void resetBuffers(unsigned int /*block_count*/) {}
-    // device column OR/AND partials were accumulated by the recombination
-    // phase; finalize reduces them and compacts the fixed and lost lists.
-    void finalize(device_population& pop) {
-        check(clotho_phase_free_space(pop.ctx()), pop.ctx(), "phase_free_space");
-        refresh(pop);
-    }
-    void refresh(device_population& pop) {
-        const clotho_shape s = pop.shape();
-        m_indices.assign(s.free_count, 0);
-        std::vector<uint32_t> fx(std::max<uint32_t>(s.fixed_count, 1)),
-            ls(std::max<uint32_t>(s.lost_count, 1));
-        check(clotho_download_free_space(pop.ctx(), fx.data(), ls.data(),
-                                         std::max(fx.size(), ls.size())),
-              pop.ctx(), "download_free_space");
-        for (uint32_t i = 0; i < s.fixed_count; ++i) m_indices[i] = fx[i];
-        for (uint32_t i = 0; i < s.lost_count; ++i) m_indices[s.fixed_count + i] = ls[i];
-        m_fixed = s.fixed_count;
-        m_lost = s.lost_count;
-        m_size = s.alleles;
-    }
     size_type fixed_size() const { return m_fixed; }
     size_type lost_size() const { return m_lost; }
     size_type free_size() const { return m_fixed + m_lost; }
     size_type variable_count() const { return m_size - free_size(); }
+    size_type getFree(unsigned int i) const { return m_indices[i]; }
+    size_type getFixed(unsigned int i) const { return m_indices[i]; }
+    size_type getLost(unsigned int i) const { return m_indices[m_fixed + i]; }
     iterator fixed_begin() const { return m_indices.data(); }
     iterator fixed_end() const { return m_indices.data() + m_fixed; }
     iterator lost_begin() const { return m_indices.data() + m_fixed; }
@@ -405,9 +571,110 @@ public:
     iterator free_begin() const { return m_indices.data(); }
     iterator free_end() const { return m_indices.data() + m_fixed + m_lost; }
 
-private:
+    // the device analysis of the population's current rows
+    void refresh(const device_population& pop) {
+        const clotho_shape s = pop.shape();
+        std::vector<uint32_t> fx(std::max<uint32_t>(s.fixed_count, 1)),
+            ls(std::max<uint32_t>(s.lost_count, 1));
+        check(clotho_download_free_space(pop.ctx(), fx.data(), ls.data(),
+                                         std::max(fx.size(), ls.size())),
+              pop.ctx(), "download_free_space");
+        assign(fx.data(), s.fixed_count, ls.data(), s.lost_count, s.alleles);
+    }
+
+protected:
+    void assign(const uint32_t* fx, uint32_t nf, const uint32_t* ls, uint32_t nl, uint32_t size) {
+        m_indices.assign(fx, fx + nf);
+        m_indices.insert(m_indices.end(), ls, ls + nl);
+        m_fixed = nf;
+        m_lost = nl;
+        m_size = size;
+    }
     std::vector<size_type> m_indices;
     size_type m_fixed = 0, m_lost = 0, m_size = 0;
+};
+
+// free_space_accumulator_mt<Block, Size> with the free_space_details
+// accessors.  finalize(pop) reduces the device column partials the
+// recombination phase accumulated.  For rows produced on the host (the
+// per-gamete common_crossover), each thread ORs / ANDs its rows into its own
+// buffer (getThreadBuffer(i).update(row, len), as the reference's
+// thread_buffer does, DS/free_space/free_space_accumulator.hpp:53-192) and
+// finalize(max_alleles) reduces the buffers and classifies the columns on the
+// device (clotho_classify_free_space).
+template <class BlockType = uint64_t, class SizeType = unsigned int>
+class free_space_accumulator_mt : public free_space_details<SizeType> {
+public:
+    typedef SizeType size_type;
+    typedef BlockType block_type;
+
+    struct thread_buffer {
+        free_space_accumulator_mt* m_parent = nullptr;
+        block_type* m_thread_var_buffer = nullptr;
+        block_type* m_thread_fix_buffer = nullptr;
+        thread_buffer() = default;
+        thread_buffer(free_space_accumulator_mt* par, block_type* var, block_type* fix)
+            : m_parent(par), m_thread_var_buffer(var), m_thread_fix_buffer(fix) {}
+        void update(const block_type* buf, unsigned int len) {
+            m_parent->updateVariableBuffer(m_thread_var_buffer, buf, len);
+            m_parent->updateFixedBuffer(m_thread_fix_buffer, buf, len);
+        }
+    };
+    typedef thread_buffer buffer_type;
+
+    explicit free_space_accumulator_mt(unsigned int thread_count)
+        : m_thread_count(std::max(1u, thread_count)) {}
+    // block_count words per row; variable buffers start at 0, fixed at ~0
+    void resetBuffers(unsigned int block_count) {
+        m_blocks_per_row = block_count;
+        m_var.assign((size_t)m_thread_count * block_count, 0);
+        m_fix.assign((size_t)m_thread_count * block_count, ~(block_type)0);
+    }
+    buffer_type getThreadBuffer(unsigned int i) {
+        if (i >= m_thread_count) throw std::runtime_error("getThreadBuffer: thread out of range");
+        return buffer_type(this, m_var.data() + (size_t)i * m_blocks_per_row,
+                           m_fix.data() + (size_t)i * m_blocks_per_row);
+    }
+    void updateVariableBuffer(block_type* dest, const block_type* src, unsigned int n) {
+        for (unsigned int k = 0; k < n; ++k) dest[k] |= src[k];
+    }
+    // a row shorter than the buffer leaves the trailing fixed words clear
+    void updateFixedBuffer(block_type* dest, const block_type* src, unsigned int n) {
+        for (unsigned int k = 0; k < n; ++k) dest[k] &= src[k];
+        for (unsigned int k = n; k < m_blocks_per_row; ++k) dest[k] = 0;
+    }
+    // the thread buffers reduced, then fixed (AND) and lost (~OR) below
+    // max_alleles, classified on the device
+    void finalize(unsigned int max_alleles, int device = 0) {
+        const unsigned int W = m_blocks_per_row;
+        if (W == 0) {
+            this->assign(nullptr, 0, nullptr, 0, max_alleles);
+            return;
+        }
+        std::vector<block_type> var(m_var.begin(), m_var.begin() + W),
+            fix(m_fix.begin(), m_fix.begin() + W);
+        for (unsigned int t = 1; t < m_thread_count; ++t)
+            for (unsigned int k = 0; k < W; ++k) {
+                var[k] |= m_var[(size_t)t * W + k];
+                fix[k] &= m_fix[(size_t)t * W + k];
+            }
+        const size_t cap = std::max<size_t>(1, std::min<size_t>((size_t)W * 64, max_alleles));
+        std::vector<uint32_t> fx(cap), ls(cap), n(2, 0);
+        check(clotho_classify_free_space(fix.data(), var.data(), W, max_alleles, fx.data(),
+                                         ls.data(), n.data(), device),
+              nullptr, "classify_free_space");
+        this->assign(fx.data(), n[0], ls.data(), n[1], max_alleles);
+    }
+    // device column OR/AND partials were accumulated by the recombination
+    // phase; finalize reduces them and compacts the fixed and lost lists.
+    void finalize(device_population& pop) {
+        check(clotho_phase_free_space(pop.ctx()), pop.ctx(), "phase_free_space");
+        this->refresh(pop);
+    }
+
+private:
+    unsigned int m_thread_count, m_blocks_per_row = 0;
+    std::vector<block_type> m_var, m_fix;
 };
 
 // GeneralFitness: phenotype (traits) + fitness of the child population.
@@ -437,6 +704,237 @@ private:
     std::shared_ptr<ifitness_generator> m_gen;
     std::vector<double> m_fit;
     bool m_cached = false;
+};
+
+// ---------------------------------------------------------------------------
+// generic families of the reference's base engine (QS/qtlsim_engine.hpp:123-160):
+// the genetic space is a device_population, whose context holds the parent
+// population (current) and produces the offspring population in place.
+// ---------------------------------------------------------------------------
+
+// Crossover<RNG, GS>::update(parents, mate_pairs, offspring)
+// (DS/crossover/crossover.hpp:49-72): offspring i takes a gamete of each
+// individual of mate pair i.  Runs one device generation with those pairs
+// in place of the fitness-proportional draw (clotho_set_parents); events and
+// strand swaps are drawn on the device as in every generation.  parents and
+// offspring name one context.  A context configured with host-directed
+// mutations (mutation.host_directed) adds none of its own, so the
+// MutationGenerator that follows places the generation's mutations.
+template <class RNG, class GeneticSpaceType>
+class Crossover {
+public:
+    typedef GeneticSpaceType genetic_space_type;
+    typedef uint32_t individual_id_type;
+    typedef std::vector<std::pair<individual_id_type, individual_id_type>> mate_pair_type;
+
+    Crossover(RNG* rng, property_tree& /*config*/) : m_rng(rng) {}
+
+    void update(genetic_space_type* parental, mate_pair_type& parents,
+                genetic_space_type* offspring) {
+        if (parental != offspring)
+            throw std::runtime_error("Crossover::update: one context holds parents and offspring");
+        clotho_ctx* c = parental->ctx();
+        std::vector<uint32_t> flat(2 * parents.size());
+        for (size_t i = 0; i < parents.size(); ++i) {
+            flat[2 * i] = parents[i].first;
+            flat[2 * i + 1] = parents[i].second;
+        }
+        check(clotho_set_population_size(c, (uint32_t)parents.size()), c, "set_population_size");
+        check(clotho_set_parents(c, flat.data(), (uint32_t)flat.size()), c, "set_parents");
+        check(clotho_run(c, 1, 0), c, "clotho_run");
+    }
+
+private:
+    RNG* m_rng;
+};
+
+// MutationGenerator<RNG, GS>::operator()(child, N)
+// (DS/mutation/mutation_generator.hpp:41-52): N new alleles, each at a row
+// uniform on [0, sequence_count), in the next free slot, with an allele from
+// AlleleGenerator (position uniform on [0, 1) as k 2^-32; per trait neutral
+// with probability neutral.p, else Normal(traits.mean, traits.sigma), rounded
+// to the effect quantum 2^traits.quantum_log2 as every effect on the device
+// is; DS/phenotype_evaluator/trait_space_generator.hpp:118-129).  Draws come
+// from the caller's engine in the reference's order (row, position, traits);
+// the device places them (clotho_add_mutations) and recomputes traits and
+// free space.
+template <class RNG, class GeneticSpaceType>
+class MutationGenerator {
+public:
+    typedef GeneticSpaceType genetic_space_type;
+    MutationGenerator(RNG* rng, property_tree& config)
+        : m_rng(rng),
+          m_neutral(config.get("neutral.p", 0.5)),
+          m_mean(config.get("traits.mean", 0.0)),
+          m_sigma(config.get("traits.sigma", 1.0)),
+          m_q((int)config.get("traits.quantum_log2", -24.0)) {}
+
+    void operator()(genetic_space_type* child, size_t N) {
+        const uint32_t H = (uint32_t)child->sequence_count(), T = child->trait_count();
+        if (N == 0) return;
+        if (H == 0) throw std::runtime_error("MutationGenerator: empty population");
+        std::uniform_int_distribution<uint32_t> row(0u, H - 1), key(0u, 0xFFFFFFFFu);
+        std::bernoulli_distribution neutral(m_neutral);
+        std::normal_distribution<double> weight(m_mean, m_sigma);
+        const double scale = std::ldexp(1.0, -m_q), quantum = std::ldexp(1.0, m_q);
+        m_rows.resize(N);
+        m_positions.resize(N);
+        m_effects.assign(N * T, 0.0);
+        for (size_t k = 0; k < N; ++k) {
+            m_rows[k] = row(*m_rng);
+            m_positions[k] = (double)key(*m_rng) * 0x1p-32;
+            for (uint32_t t = 0; t < T; ++t) {
+                double w = 0.0;
+                if (!neutral(*m_rng)) {
+                    w = std::nearbyint(weight(*m_rng) * scale);
+                    w = std::min(std::max(w, -1073741823.0), 1073741823.0) * quantum;
+                }
+                m_effects[k * T + t] = w;
+            }
+        }
+        m_slots.assign(N, 0);
+        check(clotho_add_mutations(child->ctx(), m_rows.data(), m_positions.data(),
+                                   m_effects.data(), (uint32_t)N, m_slots.data()),
+              child->ctx(), "add_mutations");
+    }
+    // the draws and slots of the last call
+    const std::vector<uint32_t>& rows() const { return m_rows; }
+    const std::vector<double>& positions() const { return m_positions; }
+    const std::vector<double>& effects() const { return m_effects; }
+    const std::vector<uint32_t>& slots() const { return m_slots; }
+
+private:
+    RNG* m_rng;
+    double m_neutral, m_mean, m_sigma;
+    int m_q;
+    std::vector<uint32_t> m_rows, m_slots;
+    std::vector<double> m_positions, m_effects;
+};
+
+// TraitWeightAccumulator<GS> (DS/phenotype_evaluator/trait_accumulator.hpp:67-84):
+// per-genome trait weights, here the device's exact sums over each row.
+template <class GeneticSpaceType>
+class TraitWeightAccumulator {
+public:
+    typedef GeneticSpaceType genetic_space_type;
+    typedef const double* trait_iterator;
+    void update(genetic_space_type& gs) {
+        m_T = gs.trait_count();
+        m_weights = gs.genome_traits();
+    }
+    size_t trait_count() const { return m_T; }
+    size_t genome_count() const { return m_T ? m_weights.size() / m_T : 0; }
+    trait_iterator begin_trait_weight(size_t genome) const { return m_weights.data() + genome * m_T; }
+    trait_iterator end_trait_weight(size_t genome) const {
+        return m_weights.data() + (genome + 1) * m_T;
+    }
+
+private:
+    std::vector<double> m_weights;
+    size_t m_T = 0;
+};
+
+// phenotype_evaluator<linear_combination<TraitWeightAccumulator<GS>, double>>
+// (DS/phenotype_evaluator/phenotype_evaluator.hpp:52-76,
+// DS/phenotype_evaluator/linear_combination_method.hpp:50-63): individual i's
+// phenotype is the sum of its two genomes' trait weights.
+template <class TraitAccumType>
+class phenotype_evaluator {
+public:
+    typedef TraitAccumType trait_accumulator_type;
+    typedef typename TraitAccumType::genetic_space_type genetic_space_type;
+    typedef const double* phenotype_type;
+    void update(genetic_space_type* pop, trait_accumulator_type& accum) {
+        m_T = accum.trait_count();
+        m_N = pop->individual_count();
+        m_pheno.assign(m_N * m_T, 0.0);
+        for (size_t i = 0; i < m_N; ++i)
+            for (size_t t = 0; t < m_T; ++t)
+                m_pheno[i * m_T + t] =
+                    accum.begin_trait_weight(2 * i)[t] + accum.begin_trait_weight(2 * i + 1)[t];
+    }
+    size_t phenotype_count() const { return m_N; }
+    phenotype_type begin_phenotype(size_t i) const { return m_pheno.data() + i * m_T; }
+    phenotype_type end_phenotype(size_t i) const { return m_pheno.data() + (i + 1) * m_T; }
+
+private:
+    std::vector<double> m_pheno;
+    size_t m_N = 0, m_T = 0;
+};
+
+template <class Evaluator>
+class Fitness;
+
+// Fitness<phenotype_evaluator<...>>::update(pop, eval)
+// (DS/fitness/phenotype_evaluator_fitness.hpp:51-69): the fitness of every
+// individual with the generator fitness.name selects, evaluated on the device
+// from the population's traits (clotho_evaluate_fitness: the same metric over
+// the same phenotype, plus the environmental noise the context adds).
+template <class TraitAccumType>
+class Fitness<phenotype_evaluator<TraitAccumType>> {
+public:
+    typedef phenotype_evaluator<TraitAccumType> evaluator_type;
+    typedef typename evaluator_type::genetic_space_type genetic_space_type;
+    explicit Fitness(property_tree& config)
+        : m_fit_gen(fitness_toolkit::getInstance()->get_tool(config)) {
+        if (!m_fit_gen) fitness_toolkit::getInstance()->tool_configurations(config);
+    }
+    void update(genetic_space_type* pop, evaluator_type& eval) {
+        const size_t N = eval.phenotype_count();
+        if (N == 0) return;
+        if (N != pop->individual_count())
+            throw std::runtime_error("Fitness::update: phenotypes of another population");
+        check(clotho_evaluate_fitness(pop->ctx()), pop->ctx(), "evaluate_fitness");
+    }
+    std::shared_ptr<ifitness_generator> generator() const { return m_fit_gen; }
+
+private:
+    std::shared_ptr<ifitness_generator> m_fit_gen;
+};
+
+// FreeSpaceAnalyzer<GS, Size>::update(gs) (DS/free_space/free_space.hpp:58-73):
+// the fixed and lost slots of the genetic space's current rows, through the
+// free_space_details accessors.  Every operation that changes the rows (a
+// generation, an upload, clotho_add_mutations) leaves the device analysis of
+// the new rows in the context; update reads it.
+template <class GeneticSpaceType, class SizeType = unsigned int>
+class FreeSpaceAnalyzer : public free_space_details<SizeType> {
+public:
+    typedef GeneticSpaceType genetic_space_type;
+    FreeSpaceAnalyzer() = default;
+    explicit FreeSpaceAnalyzer(genetic_space_type& gs) { update(gs); }
+    void update(genetic_space_type& gs) { this->refresh(gs); }
+};
+
+// free_space_evaluator<Space, Size>::operator()(space, res, fixed_offset,
+// lost_offset, free_count, M) (DS/free_space/free_space_evaluator_def.hpp:20-21;
+// layout of the row-grouped instance, DS/free_space/row_grouped_free_evaluator.hpp:140-196):
+// word by word, the word's fixed slots then its lost slots below M are
+// appended at res[free_count++], and each also at res[fixed_offset++] or
+// res[lost_offset++].  Slots between the allele count and M are lost (no row
+// carries them).  Space is a device_population.
+template <class SpaceType, class SizeType = unsigned int>
+class free_space_evaluator {
+public:
+    typedef SizeType size_type;
+    typedef size_type* result_type;
+    void operator()(SpaceType& ss, result_type res, size_type& fixed_offset,
+                    size_type& lost_offset, size_type& free_count, size_type M) {
+        free_space_details<size_type> d;
+        d.refresh(ss);
+        const uint32_t A = ss.shape().alleles;
+        std::vector<uint32_t> fx(d.fixed_begin(), d.fixed_end()), ls(d.lost_begin(), d.lost_end());
+        for (uint32_t j = A; j < M; ++j) ls.push_back(j);
+        size_t a = 0, b = 0;
+        while (a < fx.size() || b < ls.size()) {
+            const uint32_t wa = a < fx.size() ? fx[a] / 64 : ~0u, wb = b < ls.size() ? ls[b] / 64 : ~0u;
+            const uint32_t w = std::min(wa, wb);
+            for (; a < fx.size() && fx[a] / 64 == w; ++a)
+                if (fx[a] < M) res[free_count++] = res[fixed_offset++] = fx[a];
+            for (; b < ls.size() && ls[b] / 64 == w; ++b)
+                if (ls[b] < M) res[free_count++] = res[lost_offset++] = ls[b];
+        }
+    }
 };
 
 // ---------------------------------------------------------------------------

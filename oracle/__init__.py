@@ -77,6 +77,11 @@ def lib():
     L.or_sim_set_population_size.restype = C.c_int
     L.or_sim_step.argtypes = [C.c_void_p]
     L.or_sim_step.restype = C.c_int
+    L.or_sim_set_parents.argtypes = [C.c_void_p, u32p, C.c_uint32]
+    L.or_sim_set_parents.restype = C.c_int
+    L.or_sim_add_mutations.argtypes = [C.c_void_p, u32p, f64p, f64p, C.c_uint32, u32p]
+    L.or_sim_add_mutations.restype = C.c_int
+    L.or_sim_evaluate_fitness.argtypes = [C.c_void_p]
     L.or_sim_compact.argtypes = [C.c_void_p]
     L.or_allele_counts.argtypes = [u64p, C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, u32p]
     L.or_sim_compact.restype = C.c_int
@@ -277,6 +282,26 @@ class OracleSim:
 
     def compact(self):
         lib().or_sim_compact(self.h)
+
+    def set_parents(self, parents):
+        parents = np.ascontiguousarray(parents, np.uint32)
+        if lib().or_sim_set_parents(self.h, parents, parents.size) != 0:
+            raise ValueError("oracle: bad parents")
+
+    def add_mutations(self, rows, positions, effects):
+        rows = np.ascontiguousarray(rows, np.uint32)
+        positions = np.ascontiguousarray(positions, np.float64)
+        effects = np.ascontiguousarray(effects, np.float64).reshape(-1)
+        if effects.size == 0:
+            effects = np.zeros(1)
+        slots = np.zeros(max(rows.size, 1), np.uint32)
+        rc = lib().or_sim_add_mutations(self.h, rows, positions, effects, rows.size, slots)
+        if rc != 0:
+            raise ValueError(f"oracle: add_mutations failed ({rc})")
+        return slots[: rows.size]
+
+    def evaluate_fitness(self):
+        lib().or_sim_evaluate_fitness(self.h)
 
     def shape(self):
         s = self._Shape()

@@ -552,6 +552,7 @@ struct or_sim {
     double eff_scale = 1.0, eff_quantum = 1.0;
     int threads = 1;
     double phase[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> given;  // parent individual per gamete of the next generation
 };
 
 extern "C" or_sim* or_sim_create(const clotho_config* cfg) {
@@ -566,7 +567,9 @@ extern "C" or_sim* or_sim_create(const clotho_config* cfg) {
     s->cdf_rho.resize(128);
     s->cdf_mu.resize(128);
     int kr = poisson_table(cfg->rho, s->cdf_rho.data(), 128);
-    int km = poisson_table(cfg->mu, s->cdf_mu.data(), 128);
+    // host mutations: no per-gamete mutation draws (Poisson(0) counts)
+    const double mu_drawn = (cfg->flags & CLOTHO_FLAG_HOST_MUTATIONS) ? 0.0 : cfg->mu;
+    int km = poisson_table(mu_drawn, s->cdf_mu.data(), 128);
     if (kr < 0 || km < 0) {
         delete s;
         return nullptr;
@@ -631,9 +634,14 @@ void offspring_batch(or_sim* s, const Selector& sel, const std::vector<uint32_t>
     double ev[128];
     auto t0 = std::chrono::steady_clock::now();
     // crossover(): one pair per offspring, gamete 2i from parent id0, 2i+1 from id1
+    // given mate pairs (Crossover::update, DS/crossover/crossover.hpp:49-72):
+    // gamete 2i from the first individual of pair i, 2i + 1 from the second
+    const bool given = !s->given.empty();
     for (uint32_t i = i0; i < i1; ++i) {
         Draw pd = draw(cfg.seed, s->gen, S_PARENTS, cfg.replicate, i, 0);
-        const uint32_t ids[2] = {sel.pick(k53(pd.x[0], pd.x[1])), sel.pick(k53(pd.x[2], pd.x[3]))};
+        const uint32_t ids[2] = {
+            given ? s->given[2 * i] : sel.pick(k53(pd.x[0], pd.x[1])),
+            given ? s->given[2 * i + 1] : sel.pick(k53(pd.x[2], pd.x[3]))};
         for (uint32_t g = 0; g < 2; ++g) {
             const uint32_t o = 2 * i + g;
             const uint32_t p = ids[g];
@@ -896,6 +904,7 @@ extern "C" int or_sim_step(or_sim* s) {
         s->A = all_size;
         ++s->gen;
         ++s->inter;
+        s->given.clear();
         auto c4 = std::chrono::steady_clock::now();
         s->phase[4] = std::chrono::duration<double>(c3 - c2).count();
         s->phase[5] = std::chrono::duration<double>(c4 - c3).count();
@@ -928,12 +937,113 @@ extern "C" int or_sim_step(or_sim* s) {
     }
     s->A = all_size;
     ++s->gen;
+    s->given.clear();
     if (cfg.compact_every) or_sim_compact(s);
     auto c4 = std::chrono::steady_clock::now();
     s->phase[4] = std::chrono::duration<double>(c3 - c2).count();
     s->phase[5] = std::chrono::duration<double>(c4 - c3).count();
     s->phase[6] = std::chrono::duration<double>(c1 - c0).count();
     return 0;
+}
+
+// Given mate pairs of the next generation (Crossover<RNG,GS>::update,
+// DS/crossover/crossover.hpp:49-72): parents[o] is the parent individual of
+// offspring gamete o.
+extern "C" int or_sim_set_parents(or_sim* s, const uint32_t* parents, uint32_t gametes) {
+    const uint32_t next_n = s->next_N ? s->next_N : s->N;
+    if (gametes != 2 * next_n) return -1;
+    for (uint32_t o = 0; o < gametes; ++o)
+        if (parents[o] >= s->N) return -1;
+    s->given.assign(parents, parents + gametes);
+    return 0;
+}
+
+// MutationGenerator<RNG,GS>::operator()(child, N) (DS/mutation/mutation_generator.hpp:41-52)
+// on the current population: mutation k takes the next free slot (fixed
+// ascending, lost ascending, then appended), is set in row rows[k] and gets
+// its position, effects and the age of the generation that produced the
+// population.  A reused fixed column is cleared from every row first (its
+// allele leaves the population, as removeFixedAlleles does).  Traits, the
+// all-neutral flag and the free lists are recomputed; fitness is not.
+extern "C" int or_sim_add_mutations(or_sim* s, const uint32_t* rows, const double* positions,
+                                    const double* effects, uint32_t n, uint32_t* slots_out) {
+    if (n == 0) return 0;
+    const uint32_t T = s->T, H = s->H;
+    for (uint32_t k = 0; k < n; ++k)
+        if (rows[k] >= H) return CLOTHO_ERR_ARG;
+    std::vector<uint32_t> free_slots(s->fixed_list);
+    free_slots.insert(free_slots.end(), s->lost_list.begin(), s->lost_list.end());
+    const uint32_t F = (uint32_t)free_slots.size(), A = s->A;
+    const uint64_t A1 = (uint64_t)A + (n > F ? n - F : 0);
+    const uint64_t s_cap = ((uint64_t)s->cfg.max_alleles + 511) / 512 * 512;
+    if (A1 > s_cap) return CLOTHO_ERR_CAPACITY;
+    const uint32_t W0 = s->wc, W1 = (uint32_t)(((A1 + 511) / 512) * 8);
+    if (W1 > W0) {  // re-stride to the wider rows
+        std::vector<uint64_t> wide((size_t)H * W1, 0);
+        for (uint32_t r = 0; r < H; ++r)
+            std::copy(s->child.begin() + (size_t)r * W0, s->child.begin() + (size_t)(r + 1) * W0,
+                      wide.begin() + (size_t)r * W1);
+        s->child.swap(wide);
+        s->wc = W1;
+    }
+    std::vector<uint32_t> slots(n);
+    for (uint32_t k = 0; k < n; ++k) slots[k] = k < F ? free_slots[k] : A + (k - F);
+    const uint32_t fixed_reused = std::min(std::min(n, F), (uint32_t)s->fixed_list.size());
+    for (uint32_t k = 0; k < fixed_reused; ++k) {
+        const uint64_t keep = ~((uint64_t)1 << (slots[k] % 64));
+        for (uint32_t r = 0; r < H; ++r) s->child[(size_t)r * W1 + slots[k] / 64] &= keep;
+    }
+    s->pos.resize(A1, 0.0);
+    s->age.resize(A1, 0);
+    s->eff.resize((size_t)A1 * T, 0.0);
+    const uint32_t age = s->gen > 0 ? s->gen - 1 : 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t slot = slots[k];
+        s->pos[slot] = positions[k];
+        s->age[slot] = age;
+        for (uint32_t t = 0; t < T; ++t) s->eff[(size_t)slot * T + t] = effects[(size_t)k * T + t];
+        s->child[(size_t)rows[k] * W1 + slot / 64] |= (uint64_t)1 << (slot % 64);
+    }
+    s->A = (uint32_t)A1;
+    s->M = n;
+    s->all_neutral = true;
+    for (double w : s->eff)
+        if (w != 0.0) {
+            s->all_neutral = false;
+            break;
+        }
+    s->traits.assign((size_t)H * T, 0.0);
+    or_phenotype(s->child.data(), H, W1, s->eff.data(), s->A, T, s->traits.data());
+    std::vector<uint32_t> fx(s->A + 1), ls(s->A + 1);
+    uint32_t nf = 0, nl = 0;
+    or_free_space(s->child.data(), H, W1, s->A, fx.data(), &nf, ls.data(), &nl);
+    s->fixed_list.assign(fx.begin(), fx.begin() + nf);
+    s->lost_list.assign(ls.begin(), ls.begin() + nl);
+    s->inter = 0;
+    if (slots_out) std::copy(slots.begin(), slots.end(), slots_out);
+    return 0;
+}
+
+// Fitness<phenotype_evaluator<...>>::update(pop, eval)
+// (DS/fitness/phenotype_evaluator_fitness.hpp:51-69): every individual of the
+// current population from its genome traits, with the noise of the generation
+// that produced the population.
+extern "C" void or_sim_evaluate_fitness(or_sim* s) {
+    const clotho_config& cfg = s->cfg;
+    const uint32_t N = s->N, T = s->T;
+    const uint32_t gen = s->gen > 0 ? s->gen - 1 : 0;
+    FitnessOp op;
+    op.generate(cfg, N);
+    std::vector<double> z(T > 0 ? T : 1);
+    s->fitness.assign(N, 1.0);
+    for (uint32_t i = 0; i < N; ++i) {
+        for (uint32_t t = 0; t < T; ++t)
+            z[t] = s->traits[(size_t)(2 * i) * T + t] + s->traits[(size_t)(2 * i + 1) * T + t];
+        if (cfg.env_sigma > 0.0)
+            for (uint32_t t = 0; t < T; ++t) z[t] = z[t] + env_noise(cfg, gen, i, t);
+        s->fitness[i] = op(z.data());
+    }
+    s->fitness_constant = all_equal(s->fitness.data(), N);
 }
 
 // Column compaction (no reference counterpart; the reference reuses free

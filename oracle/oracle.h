@@ -71,6 +71,13 @@ void or_sim_set_threads(or_sim* s, int threads);
 /* Individuals of the offspring population of every following generation. */
 int or_sim_set_population_size(or_sim* s, uint32_t n);
 int or_sim_step(or_sim* s);
+/* Given parent individual per offspring gamete of the next generation. */
+int or_sim_set_parents(or_sim* s, const uint32_t* parents, uint32_t gametes);
+/* n new alleles on the current population, mutation k in the next free slot. */
+int or_sim_add_mutations(or_sim* s, const uint32_t* rows, const double* positions,
+                         const double* effects, uint32_t n, uint32_t* slots);
+/* Fitness of the current population from its genome traits. */
+void or_sim_evaluate_fitness(or_sim* s);
 /* Remove every fixed or lost column of the current population (slots keep
  * their order and are renumbered; the free lists become empty). */
 int or_sim_compact(or_sim* s);

@@ -95,3 +95,26 @@ def test_exact_contraction_rejects_unquantisable_effects():
         freqs = np.full(512, 0.5)
         with pytest.raises(api.ClothoError, match="invalid argument"):
             api.trait_contract_synthetic(4, freqs, eff, api.TRAITS_EXACT_I8)
+
+
+def test_mutation_allocator_draws_poisson():
+    """mutation_allocator::allocate(N) draws M ~ Poisson(mu N) from the caller's
+    engine (include/clotho/data_spaces/mutation/mutation_allocator.hpp:41-47):
+    mean and variance both mu N, and a chi-square fit of the counts."""
+    import scipy.stats as st
+
+    gen = build.GENERIC_DEMO
+    if not os.path.exists(gen):
+        build.build_host_demo()
+    k, n, mu = 4000, 500, 0.02  # lambda = 10
+    out = subprocess.run([gen, "--allocator-sample", str(k), str(n), f"mutation.mu={mu}"],
+                         check=True, capture_output=True, text=True).stdout
+    x = np.array([int(v) for v in out.split()])
+    lam = mu * n
+    assert x.size == k
+    assert abs(x.mean() - lam) < 4 * np.sqrt(lam / k)
+    assert abs(x.var() - lam) < 4 * lam * np.sqrt(2.0 / k) + 0.5
+    obs = np.bincount(np.minimum(x, 20), minlength=21)[3:21]
+    exp = st.poisson.pmf(np.arange(3, 21), lam) * k
+    exp[-1] = st.poisson.sf(19, lam) * k
+    assert st.chisquare(obs, exp * obs.sum() / exp.sum()).pvalue > 1e-3

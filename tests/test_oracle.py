@@ -599,3 +599,123 @@ def test_environmental_noise_law():
     # a different trait index or generation is a different draw
     other = oracle.env_noise(cfg, 2, 100)
     assert not np.array_equal(other, noise[:100])
+
+
+# --------------------------------------------- generic operator families
+def test_oracle_given_parents_equal_the_drawn_ones():
+    """Crossover<RNG,GS>::update(parents, mate_pairs, offspring)
+    (include/clotho/data_spaces/crossover/crossover.hpp:49-72): gamete 2i
+    recombines pair i's first individual, 2i+1 its second.  Giving the
+    oracle exactly the pairs its own selection draws must reproduce the
+    undirected generation bit for bit; other pairs must be used as given."""
+    cfg = api.make_config(150, 1.5, 2.0, seed=31, traits=1, neutral_p=0.3, sigma=0.2,
+                          fitness=api.FIT_GAUSSIAN_FIXED, env_sigma=0.5, max_alleles=1 << 14)
+    a, b = oracle.OracleSim(cfg), oracle.OracleSim(cfg)
+    for _ in range(3):
+        a.step()
+        b.step()
+    # generation 4: b takes a's selection as given pairs
+    drawn = oracle.select(b.fitness(), cfg.seed, b.shape().generation)
+    b.set_parents(drawn)
+    a.step()
+    b.step()
+    assert np.array_equal(a.rows(), b.rows())
+    assert np.array_equal(a.gametes()[0] >> 1, drawn)
+    # arbitrary pairs: every gamete's strands come from its given individual
+    rng = np.random.default_rng(3)
+    pairs = rng.integers(0, 150, 300).astype(np.uint32)
+    b.set_parents(pairs)
+    b.step()
+    top, bot, _, _ = b.gametes()
+    assert np.array_equal(top >> 1, pairs) and np.array_equal(bot >> 1, pairs)
+    assert np.all((top ^ bot) == 1)
+    with pytest.raises(ValueError):
+        b.set_parents(np.zeros(299, np.uint32))
+    with pytest.raises(ValueError):
+        b.set_parents(np.full(300, 150, np.uint32))
+
+
+def test_oracle_add_mutations_reference_known_answer():
+    """The reference's mutation unit test (unittest/data_spaces/mutation_test.cpp:39-66):
+    20 empty genomes over 200 allele slots have 200 free slots; 15 mutations
+    leave 185, and the free-space analysis of the mutated population finds
+    185.  Also the closed form: each mutation sets exactly its bit, slots
+    follow the free list in order, popcount grows by 15."""
+    cfg = api.make_config(10, 0.0, 0.0, seed=1, traits=1, neutral_p=0.0, max_alleles=1 << 12,
+                          host_mutations=True)
+    orc = oracle.OracleSim(cfg)
+    words = 8
+    orc.set_population(np.zeros((20, words), np.uint64), np.arange(200) / 256.0,
+                       np.zeros((200, 1)))
+    s = orc.shape()
+    assert s.free_count == 200 and s.lost_count == 200 and s.alleles == 200
+    rng = np.random.default_rng(1234)
+    rows = rng.integers(0, 20, 15).astype(np.uint32)
+    pos = np.floor(rng.uniform(0, 1, 15) * 2**32) / 2**32
+    eff = np.round(rng.normal(0, 1, (15, 1)) * 2**24) / 2**24
+    slots = orc.add_mutations(rows, pos, eff)
+    assert list(slots) == list(range(15))  # the lost slots, ascending
+    s = orc.shape()
+    assert s.free_count == 185 and s.alleles == 200
+    got = orc.rows()
+    bits = np.unpackbits(got.view(np.uint8), axis=1, bitorder="little")
+    assert bits.sum() == 15
+    for k in range(15):
+        assert bits[rows[k], slots[k]] == 1
+    p, e, age = orc.alleles()
+    assert np.array_equal(p[slots], pos) and np.array_equal(e[slots, 0], eff[:, 0])
+    want = np.zeros(20)
+    np.add.at(want, rows, eff[:, 0])
+    assert np.array_equal(orc.traits()[:, 0], want)
+
+
+def test_oracle_add_mutations_fixed_reuse_and_widening():
+    """A reused fixed slot leaves every row before the new allele is set; more
+    mutations than free slots append columns and widen the rows."""
+    cfg = api.make_config(4, 0.0, 0.0, seed=1, traits=2, neutral_p=0.0, max_alleles=1 << 12,
+                          host_mutations=True)
+    orc = oracle.OracleSim(cfg)
+    rows = np.zeros((8, 8), np.uint64)
+    rows[:, 0] = 0b101  # slot 0 and 2 fixed, slot 1 lost
+    rows[0, 0] |= 0b1000  # slot 3 segregating
+    eff = np.full((4, 2), 0.5)
+    orc.set_population(rows, np.array([0.1, 0.2, 0.3, 0.4]), eff)
+    fx, ls = orc.free_space()
+    assert list(fx) == [0, 2] and list(ls) == [1]
+    n = 600  # 3 free slots, 597 appended: 601 slots need 16 words
+    mrows = (np.arange(n) % 8).astype(np.uint32)
+    pos = (np.arange(n) + 1) / 1024.0
+    meff = np.full((n, 2), 0.25)
+    slots = orc.add_mutations(mrows, pos, meff)
+    assert list(slots[:3]) == [0, 2, 1] and list(slots[3:6]) == [4, 5, 6]
+    s = orc.shape()
+    assert s.words == 16 and s.alleles == 601
+    got = orc.rows()
+    bits = np.unpackbits(got.view(np.uint8), axis=1, bitorder="little")[:, :601].astype(int)
+    want = np.zeros((8, 601), int)
+    want[0, 3] = 1
+    want[mrows, slots] = 1
+    assert np.array_equal(bits, want)
+    # traits: 0.25 per new allele, 0.5 for the one old segregating allele
+    t = orc.traits()
+    assert np.array_equal(t[:, 0], want.sum(1) * 0.25 + np.where(np.arange(8) == 0, 0.25, 0.0))
+
+
+def test_oracle_evaluate_fitness_closed_form():
+    """Fitness<phenotype_evaluator>::update: f_i = op(trait(2i) + trait(2i+1)
+    + noise of the producing generation), with the configured metric."""
+    cfg = api.make_config(64, 1.0, 1.0, seed=9, traits=2, neutral_p=0.2, sigma=0.4,
+                          fitness=api.FIT_MV_STABILIZING, fitness_scale=1.5, env_sigma=0.3,
+                          max_alleles=1 << 12)
+    orc = oracle.OracleSim(cfg)
+    for _ in range(3):
+        orc.step()
+    before = orc.fitness().copy()
+    orc.evaluate_fitness()
+    after = orc.fitness()
+    gen = orc.shape().generation - 1
+    t = orc.traits().reshape(64, 2, 2).sum(axis=1)
+    noise = np.stack([oracle.env_noise(cfg, gen, 64, trait=j) for j in range(2)], axis=1)
+    want = [oracle.fitness_value(cfg, 64, t[i] + noise[i]) for i in range(64)]
+    assert np.array_equal(after, np.array(want))
+    assert np.array_equal(after, before)  # the step computed the same thing
