@@ -1754,11 +1754,11 @@ struct PackI8 {
         return (P.w_cap + i8::kGroupWords - 1) / i8::kGroupWords;
     }
     __host__ __device__ static uint32_t blocks(const Params& P) {
-        return P.traits ? ((P.traits + 3) / 4) * ng_cap(P) * 4 : 0u;  // 1024 items per (nt, G)
+        return P.traits ? i8::kernel_tiles(P.traits) * ng_cap(P) * 4 : 0u;  // 1024 items per (nt, G)
     }
     __device__ static __forceinline__ void run(const Params& P, const Buffers& B, uint32_t,
                                                uint32_t blk) {
-        i8::pack_i8_block(B.effects, P.traits, (P.traits + 3) / 4, &B.state->wc,
+        i8::pack_i8_block(B.effects, P.traits, i8::kernel_tiles(P.traits), &B.state->wc,
                           &B.state->all_size, ng_cap(P), B.digits, blk);
     }
 };

@@ -171,7 +171,9 @@ extern "C" int clotho_trait_contract(const uint64_t* rows, uint32_t h, uint32_t 
                                      const double* effects, uint32_t sites, uint32_t traits,
                                      int mode, int split_terms, int repeats, double* z,
                                      double* avg_ms, int device) {
-    if (!rows || !effects || !z || h == 0 || w == 0 || traits == 0 || traits > 16 ||
+    // rows are loaded four words (two 16-byte loads) at a time: whole
+    // 512-bit row blocks, as the reference's row_block_alignment pads them
+    if (!rows || !effects || !z || h == 0 || w == 0 || w % 8 || traits == 0 || traits > 16 ||
         (size_t)w * 64 < sites || repeats < 1)
         return CLOTHO_ERR_ARG;
     if (mode != CLOTHO_TRAITS_PLAIN_FP32 && mode != CLOTHO_TRAITS_EXACT_FP64 &&

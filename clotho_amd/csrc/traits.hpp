@@ -23,7 +23,7 @@ hipError_t launch_traits_walk_fp64(const uint64_t* rows, uint32_t h, uint32_t w,
                                    hipStream_t s);
 // Exact traits of the pipeline (traits_mfma.hip): int8 digit planes of the
 // quantised effects, int32 MFMA accumulation, int64 recombination.
-uint32_t i8_ntiles(uint32_t traits);
+uint32_t i8_ntiles(uint32_t traits);  // N tiles of the kernel for `traits` (1, 2, 4, 8)
 size_t i8_packed_elems(uint32_t traits, uint32_t w_cap);  // uint4 elements
 hipError_t launch_pack_i8(const int32_t* eff, uint32_t traits, const uint32_t* p_words,
                           const uint32_t* p_alleles, uint32_t w_cap, uint4* packed,

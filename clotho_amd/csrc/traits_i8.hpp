@@ -28,6 +28,13 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p) 
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// N tiles (four traits each) the traits kernels multiply by: 1, 2, 4 or 8.
+// The digit planes are packed (zeros past the last trait) and allocated for
+// this many tiles, so every plane a kernel reads was written.
+__host__ __device__ inline uint32_t kernel_tiles(uint32_t traits) {
+    return traits <= 4 ? 1u : traits <= 8 ? 2u : traits <= 16 ? 4u : 8u;
+}
+
 // four genotype bits -> four int8 0/1 bytes
 __device__ inline uint32_t spread4(uint32_t nib) { return (nib * 0x00204081u) & 0x01010101u; }
 

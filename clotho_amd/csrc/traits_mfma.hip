@@ -191,7 +191,7 @@ float bf16_to_f(uint16_t b) {
 }  // namespace
 
 // ---- exact int8 path (used by the pipeline) -------------------------------
-uint32_t i8_ntiles(uint32_t traits) { return (traits + 3) / 4; }
+uint32_t i8_ntiles(uint32_t traits) { return i8::kernel_tiles(traits); }
 
 size_t i8_packed_elems(uint32_t traits, uint32_t w_cap) {
     const size_t ng = (w_cap + kGroupWords - 1) / kGroupWords;

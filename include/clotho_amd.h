@@ -354,7 +354,8 @@ void clotho_batch_set_timing(clotho_batch* batch, int enabled);
 /* ---- standalone trait contraction -------------------------------------- */
 /* z[h][t] = sum_j bit(h,j) * effects[j][t] over a device-resident genotype
  * matrix.  Host-buffer variant: uploads, runs `repeats` times, downloads z
- * (FP64 for every mode), reports the average kernel milliseconds. */
+ * (FP64 for every mode), reports the average kernel milliseconds.  words is a
+ * multiple of 8 (whole 512-bit row blocks). */
 int clotho_trait_contract(const uint64_t* rows, uint32_t haplotypes, uint32_t words,
                           const double* effects, uint32_t sites, uint32_t traits,
                           int mode, int split_terms, int repeats, double* z,
