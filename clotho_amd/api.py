@@ -130,6 +130,7 @@ EXPORTS = [
     "clotho_pairwise_differences", "clotho_device_memory",
     "clotho_set_parents", "clotho_add_mutations", "clotho_evaluate_fitness",
     "clotho_crossover_rows", "clotho_classify_free_space",
+    "clotho_trait_contract_exact_noalloc",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -185,6 +186,8 @@ def lib():
     L.clotho_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.clotho_classify_free_space.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp,
                                              C.c_int]
+    L.clotho_trait_contract_exact_noalloc.argtypes = [vp, C.c_uint32, C.c_uint32, vp,
+                                                      C.c_uint32, C.c_uint32, vp, vp]
     L.clotho_set_parents.argtypes = [vp, vp, C.c_uint32]
     L.clotho_add_mutations.argtypes = [vp, vp, vp, vp, C.c_uint32, vp]
     L.clotho_evaluate_fitness.argtypes = [vp]

@@ -9,7 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib", "libclotho_amd.so")
 
-SOURCES = ["capi.hip", "pipeline.hip", "batch.hip", "traits.hip", "traits_mfma.hip", "stream_copy.hip"]
+SOURCES = ["capi.hip", "pipeline.hip", "batch.hip", "traits.hip", "traits_mfma.hip",
+           "traits_noalloc.hip", "stream_copy.hip"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",

@@ -361,6 +361,29 @@ int clotho_trait_contract(const uint64_t* rows, uint32_t haplotypes, uint32_t wo
                           int mode, int split_terms, int repeats, double* z,
                           double* avg_ms, int device);
 
+/* ---- exact trait contraction without allocation ------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): clotho_trait_contract's exact int8 path over caller-owned
+ * device buffers, enqueued on `stream` (a hipStream_t; NULL is the default
+ * stream) and not waited for.  rows[haplotypes][words] (words a multiple of
+ * 8), quantised effects k[sites][traits] (int32, |k| <= 2^30 - 1; the effect
+ * is k 2^q for the caller's q) and the output sums[haplotypes][traits]
+ * (int64, the exact sum of k over each row's set bits below `sites`) are
+ * device pointers.  The effect digit planes live in a fixed device array of
+ * CLOTHO_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU race.  words * 64 up to CLOTHO_NOALLOC_MAX_SITES and traits up
+ * to CLOTHO_NOALLOC_MAX_TRAITS; beyond either the call returns
+ * CLOTHO_ERR_CAPACITY without enqueuing anything.  Calls no hipMalloc,
+ * hipFree or allocating routine. */
+#define CLOTHO_NOALLOC_MAX_SITES 131072
+#define CLOTHO_NOALLOC_MAX_TRAITS 16
+#define CLOTHO_NOALLOC_SCRATCH_BYTES (256 + 8388608)
+int clotho_trait_contract_exact_noalloc(const uint64_t* rows, uint32_t haplotypes,
+                                        uint32_t words, const int32_t* effects, uint32_t sites,
+                                        uint32_t traits, int64_t* sums, void* stream);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* STREAM-style device-to-device copy of `bytes` (read + write counted);
  * no reference counterpart: it measures the HBM bandwidth the roofline is
