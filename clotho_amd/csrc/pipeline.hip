@@ -1565,40 +1565,68 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x, uint32_t lane) {
     return x;
 }
 
+// Block = 16 waves over a tile of kStatTileWords words (two 128-byte lines
+// per row) and a chunk of 64-row groups.  Each group's 64 x 32 words are
+// loaded coalesced (16 lanes x 16 bytes per row) into LDS, the next group's
+// load in flight meanwhile; then wave v takes words 2v, 2v + 1 with lane = row
+// (a 33-word row pitch keeps those reads in distinct banks), adds each row's
+// popcounts and transposes each word for the column popcounts.
 __global__ void __launch_bounds__(64 * kStatWaves) k_allele_counts(
     const uint64_t* __restrict__ rows_in, uint32_t W, const uint32_t* __restrict__ sample_in,
     uint32_t n, uint32_t groups_per_chunk, uint32_t* __restrict__ col_part_in,
     uint32_t* __restrict__ row_part_in) {
+    constexpr uint32_t TW = kStatTileWords, PITCH = TW + 1;
+    __shared__ uint64_t s_tile[64 * PITCH];
     __shared__ uint32_t s_rc[kStatWaves][64];
     const auto rows = gbl(rows_in);
     const auto col_part = gbl(col_part_in);
     const auto row_part = gbl(row_part_in);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t ngroups = (W + kStatWaves - 1) / kStatWaves;
-    const uint32_t wg = blockIdx.x % ngroups, chunk = blockIdx.x / ngroups;
-    const uint32_t w = wg * kStatWaves + wave;
+    const uint32_t ntw = stat_word_tiles(W);
+    const uint32_t wt = blockIdx.x % ntw, chunk = blockIdx.x / ntw;
+    const uint32_t w0 = wt * TW;
     const uint32_t rg_total = (n + 63) / 64;
     const uint32_t g0 = chunk * groups_per_chunk, g1 = min(rg_total, g0 + groups_per_chunk);
-    uint32_t col = 0;
-    for (uint32_t g = g0; g < g1; ++g) {
-        const uint32_t k = g * 64 + lane;
-        uint64_t x = 0;
-        if (k < n && w < W) {
+    // loader: row q of the group, words w0 + 2p and w0 + 2p + 1
+    const uint32_t q = threadIdx.x >> 4, p = threadIdx.x & 15;
+    const bool wok = w0 + 2 * p < W;  // W is even: both words are in the row
+    auto load = [&](uint32_t g) {
+        const uint32_t k = g * 64 + q;
+        i8::u32x4 v = {0, 0, 0, 0};
+        if (g < g1 && k < n && wok) {
             const uint32_t r = sample_in ? gbl(sample_in)[k] : k;
-            x = rows[(size_t)r * W + w];
+            v = *(const __attribute__((address_space(1))) i8::u32x4*)(rows + (size_t)r * W + w0 +
+                                                                       2 * p);
         }
-        s_rc[wave][lane] = (uint32_t)__popcll(x);
-        col += (uint32_t)__popcll(transpose64(x, lane));
+        return v;
+    };
+    uint32_t col0 = 0, col1 = 0;
+    i8::u32x4 cur = load(g0);
+    for (uint32_t g = g0; g < g1; ++g) {
+        s_tile[q * PITCH + 2 * p] = ((uint64_t)cur.y << 32) | cur.x;
+        s_tile[q * PITCH + 2 * p + 1] = ((uint64_t)cur.w << 32) | cur.z;
         __syncthreads();
+        cur = load(g + 1);
+        const uint64_t x0 = s_tile[lane * PITCH + 2 * wave];
+        const uint64_t x1 = s_tile[lane * PITCH + 2 * wave + 1];
+        s_rc[wave][lane] = (uint32_t)(__popcll(x0) + __popcll(x1));
+        col0 += (uint32_t)__popcll(transpose64(x0, lane));
+        col1 += (uint32_t)__popcll(transpose64(x1, lane));
+        __syncthreads();
+        const uint32_t k = g * 64 + lane;
         if (wave == 0 && k < n) {
             uint32_t rc = 0;
-            for (int q = 0; q < kStatWaves; ++q) rc += s_rc[q][lane];
-            row_part[(size_t)wg * n + k] = rc;
+#pragma unroll
+            for (int v = 0; v < kStatWaves; ++v) rc += s_rc[v][lane];
+            row_part[(size_t)wt * n + k] = rc;
         }
-        __syncthreads();
     }
-    if (w < W) col_part[(size_t)chunk * W * 64 + (size_t)w * 64 + lane] = col;
+    const uint32_t w = w0 + 2 * wave;
+    if (w < W) {
+        col_part[(size_t)chunk * W * 64 + (size_t)w * 64 + lane] = col0;
+        col_part[(size_t)chunk * W * 64 + (size_t)(w + 1) * 64 + lane] = col1;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum_columns(const uint32_t* __restrict__ part,
@@ -2155,7 +2183,7 @@ hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W,
                                 const uint32_t* d_sample, uint32_t n, uint32_t* d_cols,
                                 uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,
                                 uint32_t chunks) {
-    const uint32_t ngroups = (W + kStatWaves - 1) / kStatWaves;
+    const uint32_t ngroups = stat_word_tiles(W);
     const uint32_t rg = (n + 63) / 64;
     const uint32_t per = (rg + chunks - 1) / chunks;
     if (W && n) {

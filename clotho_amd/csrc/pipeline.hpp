@@ -19,6 +19,12 @@ constexpr int kScanBlock = 256;   // items per block of every two-level scan
 constexpr double kEffectLimit = 1073741823.0;  // 2^30 - 1
 // allele-slot capacity limit: int8 digit sums over a row stay within int32
 constexpr uint32_t kMaxAlleleSlots = 1u << 24;
+// words per column tile of the population-statistics pass; row partials
+// are kept per tile
+constexpr uint32_t kStatTileWords = 32;
+__host__ __device__ inline uint32_t stat_word_tiles(uint32_t words) {
+    return (words + kStatTileWords - 1) / kStatTileWords;
+}
 
 // Scalars of the generation, kept in device memory so that a captured
 // hipGraph replays successive generations without host involvement.
@@ -239,7 +245,7 @@ hipError_t query_trait_residency(clotho_ctx* c);  // sets c->trait_resident
 // columns removed, slots renumbered in order); the caller flips cur.
 hipError_t launch_compact(clotho_ctx* c, int src);
 // Column and row margins of a row sample (d_sample == nullptr: rows 0..n-1);
-// partial buffers: d_col_part [chunks][W*64], d_row_part [ceil(W/16)][n].
+// partial buffers: d_col_part [chunks][W*64], d_row_part [stat_word_tiles(W)][n].
 hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W, uint32_t A,
                                 const uint32_t* d_sample, uint32_t n, uint32_t* d_cols,
                                 uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,

@@ -62,43 +62,52 @@ __global__ void __launch_bounds__(256) k_traits_i8(const uint64_t* __restrict__ 
 
 
 // Split-BF16 contraction, K chunked: block = 4 waves x RT row tiles, one
-// chunk of kChunkGroups 16-word groups.  The FP32 MFMA accumulator is
-// flushed into FP64 registers every kFlushBlocks K blocks (256 sites), so
-// FP32 rounding acts only on short partial sums; FP64 partial[chunk][row][t].
-// K block q (of 32 per group) takes byte q % 8 of the lane's word q / 8, i.e.
-// sites bf_site(G, q, lane / 16, i).
+// chunk of kChunkGroups 16-word groups.  K block q (of 32 per group) takes
+// byte q % 8 of the lane's word q / 8, i.e. sites bf_site(G, q, lane / 16, i);
+// bits expand to BF16 0/1 through a 256-entry LDS table (byte -> 8 BF16).
+// The B fragments (TERMS per K block, 1 KB each) are staged in LDS for the
+// whole block, kStageBlocks K blocks at a time, double buffered and moved by
+// direct global-to-LDS loads while the previous stage is multiplied: every
+// wave reads its B operands from LDS instead of each wave fetching them from
+// L2.  The FP32 accumulators are flushed into FP64 after every stage (256
+// sites), so FP32 rounding acts only on short partial sums; FP64
+// partial[chunk][row][t].
 constexpr uint32_t kChunkGroups = 16;
-constexpr uint32_t kFlushBlocks = 8;
+constexpr uint32_t kStageBlocks = 8;
 
 __host__ __device__ inline uint32_t bf_site(uint32_t G, uint32_t q, uint32_t g, uint32_t i) {
     return 64 * (16 * G + 4 * g + q / 8) + 8 * (q & 7) + i;
 }
 
 template <int RT, int TERMS>
-__global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict__ rows,
+__global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict__ rows_in,
                                                      uint32_t h, uint32_t W, uint32_t T,
-                                                     const uint4* __restrict__ packed,
+                                                     const uint4* __restrict__ packed_in,
                                                      double* __restrict__ partial) {
+    constexpr uint32_t QS = kStageBlocks;
+    constexpr uint32_t FR = QS * TERMS;       // 1 KB fragments per stage
+    constexpr uint32_t SPG = 32 / QS;         // stages per group
+    __shared__ uint4 s_b[2 * FR * 64];
+    __shared__ uint4 s_lut[256];
+    const auto rows = i8::as_global(rows_in);
+    const auto packed = i8::as_global(packed_in);
     const uint32_t NG = (W + 15) / 16;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t m = lane & 15, g = lane >> 4;
     const uint32_t base = (blockIdx.x * 4 + wave) * (16 * RT);
     const uint32_t G0 = blockIdx.y * kChunkGroups;
     const uint32_t G1 = min(NG, G0 + kChunkGroups);
-    // byte -> 8 BF16 (0.0 / 1.0) lookup, 4 KB of LDS
-    __shared__ uint4 s_lut[256];
     {
         const uint32_t b8 = threadIdx.x;
         s_lut[b8] = make_uint4(bits2_bf16(b8 & 3), bits2_bf16((b8 >> 2) & 3),
                                bits2_bf16((b8 >> 4) & 3), bits2_bf16(b8 >> 6));
     }
-    __syncthreads();
     double acc64[RT][4];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc64[r][i] = 0.0;
-    const uint64_t* rp[RT];
+    const __attribute__((address_space(1))) uint64_t* rp[RT];
     bool rv[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
@@ -106,32 +115,59 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
         rv[r] = row < h;
         rp[r] = rows + (size_t)(rv[r] ? row : 0) * W;
     }
-    for (uint32_t G = G0; G < G1; ++G) {
+    auto load_rows = [&](uint32_t G, i8::u32x4 (&x)[RT][2]) {
         const uint32_t w = 16 * G + 4 * g;
-        uint4 x[RT][2];
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             if (rv[r] && w < W) {
-                const uint4* src = reinterpret_cast<const uint4*>(rp[r] + w);
+                const i8::g_u32x4* src = (const i8::g_u32x4*)(rp[r] + w);
                 x[r][0] = src[0];
                 x[r][1] = src[1];
             } else {
-                x[r][0] = make_uint4(0, 0, 0, 0);
-                x[r][1] = make_uint4(0, 0, 0, 0);
+                x[r][0] = i8::u32x4{0, 0, 0, 0};
+                x[r][1] = i8::u32x4{0, 0, 0, 0};
             }
         }
-        const uint4* bp = packed + (size_t)G * 32 * TERMS * 64 + lane;
+    };
+    // stage k of the chunk: group G0 + k / SPG, K blocks (k % SPG) QS ..
+    // + QS - 1, every term: FR contiguous 1 KB fragments, wave v moving
+    // fragments v, v + 4, ...
+    const uint32_t n_stages = (G1 > G0 ? G1 - G0 : 0) * SPG;
+    auto stage_b = [&](uint32_t k) {
+        if (k < n_stages) {
+            const uint32_t G = G0 + k / SPG, qb = (k % SPG) * QS;
+            const auto src = packed + (((size_t)G * 32 + qb) * TERMS) * 64 + lane;
+            for (uint32_t i = wave; i < FR; i += 4)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void*)(src + (size_t)i * 64),
+                    (__attribute__((address_space(3))) void*)(s_b + (k & 1) * FR * 64 + i * 64),
+                    16, 0, 0);
+        }
+    };
+    // vmcnt(0) only: the direct-to-LDS loads and the next group's rows have landed
+    auto wait_loads = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
+    i8::u32x4 x[RT][2];
+    if (n_stages) load_rows(G0, x);
+    stage_b(0);
+    wait_loads();
+    __syncthreads();
+    for (uint32_t G = G0; G < G1; ++G) {
+        i8::u32x4 xn[RT][2];
+        if (G + 1 < G1) load_rows(G + 1, xn);
 #pragma unroll
-        for (uint32_t qb = 0; qb < 32; qb += kFlushBlocks) {
+        for (uint32_t st = 0; st < SPG; ++st) {
+            const uint32_t k = (G - G0) * SPG + st;
+            stage_b(k + 1);
             v4f acc[RT];
 #pragma unroll
             for (int r = 0; r < RT; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (uint32_t q = qb; q < qb + kFlushBlocks; ++q) {
+            for (uint32_t qq = 0; qq < QS; ++qq) {
+                const uint32_t q = st * QS + qq;
                 v8bf a[RT];
 #pragma unroll
                 for (int r = 0; r < RT; ++r) {
-                    const uint4 v = x[r][q >> 4];
+                    const i8::u32x4 v = x[r][q >> 4];
                     const uint32_t sel = (q >> 2) & 3;
                     const uint32_t dw = sel == 0 ? v.x : sel == 1 ? v.y : sel == 2 ? v.z : v.w;
                     const uint32_t b8 = (dw >> (8 * (q & 3))) & 0xFFu;
@@ -140,7 +176,7 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
                 }
 #pragma unroll
                 for (int tm = 0; tm < TERMS; ++tm) {
-                    const uint4 bq = bp[((size_t)q * TERMS + tm) * 64];
+                    const uint4 bq = s_b[(k & 1) * FR * 64 + (qq * TERMS + tm) * 64 + lane];
                     v8bf bv;
                     __builtin_memcpy(&bv, &bq, 16);
 #pragma unroll
@@ -153,6 +189,15 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
             for (int r = 0; r < RT; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) acc64[r][i] += (double)acc[r][i];
+            wait_loads();
+            __syncthreads();  // stage k + 1 complete; buffer k & 1 free
+        }
+        if (G + 1 < G1) {
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                x[r][0] = xn[r][0];
+                x[r][1] = xn[r][1];
+            }
         }
     }
 #pragma unroll
