@@ -45,7 +45,10 @@ def main():
     ap.add_argument("--sites", type=int, default=65_536)
     ap.add_argument("--traits", type=int, default=16)
     ap.add_argument("--repeats", type=int, default=10)
+    ap.add_argument("--modes", default="fp64,plain,walk,i8,bf16",
+                    help="comma-separated subset to run (fp64 is the error reference)")
     args = ap.parse_args()
+    modes = set(args.modes.split(","))
 
     from clotho_amd import api
 
@@ -54,6 +57,18 @@ def main():
     eff = effects_mvn(s, t)
     out = {"workload": f"trait contraction H={h} S={s} T={t}, f~U(0,1), MVN effects rho=0.3"}
 
+    if "fp64" not in modes:  # timing only: no error reference
+        for name, mode, terms in (("exact_i8", api.TRAITS_EXACT_I8, 3),
+                                  ("mfma_bf16x3", api.TRAITS_MFMA_BF16, 3)):
+            if name.split("_")[-1].startswith("i8") and "i8" not in modes:
+                continue
+            if name.startswith("mfma") and "bf16" not in modes:
+                continue
+            _, ms = api.trait_contract_synthetic(h, freqs, eff, mode, seed=42, split_terms=terms,
+                                                 repeats=args.repeats)
+            out[name + "_ms"] = ms
+        print(json.dumps(out), flush=True)
+        return
     z64, ms64 = api.trait_contract_synthetic(h, freqs, eff, api.TRAITS_EXACT_FP64, seed=42,
                                              repeats=2)
     out["fp64_walk_ms"] = ms64

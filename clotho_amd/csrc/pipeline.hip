@@ -1549,21 +1549,67 @@ __global__ void k_compact_state(Buffers B) {
 // ---------------------------------------------------------------------------
 constexpr int kStatWaves = 16;
 
-__device__ __forceinline__ uint64_t transpose64(uint64_t x, uint32_t lane) {
-    const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
-                               0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
-                               0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        const uint32_t sft = 32u >> i;
-        const uint64_t y = __shfl_xor(x, (int)sft, 64);
-        const uint64_t M = masks[i];
-        // rows with bit sft clear keep their low-half columns and take the
-        // partner's low-half columns into the high half; the partner the reverse
-        x = (lane & sft) ? ((x & ~M) | ((y & ~M) >> sft)) : ((x & M) | ((y & M) << sft));
-    }
-    return x;
+// x of lane ^ S: DPP quad permutes for S = 1, 2, swizzles for S = 4, 8, 16
+// (both stay inside 32-lane halves), a bpermute for S = 32.
+template <uint32_t S>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, uint32_t lane) {
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    if constexpr (S >= 4 && S <= 16)
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (int)((S << 10) | 0x1F));
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane ^ S) << 2), (int)v);
 }
+
+// One butterfly stage: rows l and l ^ S exchange the off-diagonal S x S
+// blocks selected by M (the low S bits of every 2S-bit block).  A row with
+// bit S clear keeps its M bits and takes its partner's M bits, shifted up,
+// into ~M; a row with it set keeps ~M and takes the partner's ~M bits,
+// shifted down.  Per lane the stage is fixed by K (the bits kept: M or ~M)
+// and the shift pair (S, 0) or (0, S), computed once per kernel.
+struct TransposeStage {
+    uint32_t k_lo, k_hi, up, down;
+};
+
+template <uint32_t S>
+__device__ __forceinline__ TransposeStage transpose_stage_of(uint32_t lane, uint64_t M) {
+    const uint32_t b = lane & S;
+    const uint64_t K = b ? ~M : M;
+    return TransposeStage{(uint32_t)K, (uint32_t)(K >> 32), S - b, b};
+}
+
+template <uint32_t S>
+__device__ __forceinline__ uint64_t transpose_stage(uint64_t x, uint32_t lane,
+                                                    const TransposeStage& t) {
+    const uint32_t lo = lane_xor<S>((uint32_t)x, lane);
+    const uint32_t hi = lane_xor<S>((uint32_t)(x >> 32), lane);
+    const uint64_t z = ((((uint64_t)hi << 32) | lo) << t.up) >> t.down;
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t zl = (uint32_t)z, zh = (uint32_t)(z >> 32);
+    const uint32_t rl = (xl & t.k_lo) | (zl & ~t.k_lo);
+    const uint32_t rh = (xh & t.k_hi) | (zh & ~t.k_hi);
+    return ((uint64_t)rh << 32) | rl;
+}
+
+struct Transpose64 {
+    TransposeStage s32, s16, s8, s4, s2, s1;
+    __device__ explicit Transpose64(uint32_t lane)
+        : s32(transpose_stage_of<32>(lane, 0x00000000FFFFFFFFull)),
+          s16(transpose_stage_of<16>(lane, 0x0000FFFF0000FFFFull)),
+          s8(transpose_stage_of<8>(lane, 0x00FF00FF00FF00FFull)),
+          s4(transpose_stage_of<4>(lane, 0x0F0F0F0F0F0F0F0Full)),
+          s2(transpose_stage_of<2>(lane, 0x3333333333333333ull)),
+          s1(transpose_stage_of<1>(lane, 0x5555555555555555ull)) {}
+    // 64 x 64 bit transpose across a wave: row l (lane l, bit j = column j)
+    // becomes column l (lane l, bit j = row j)
+    __device__ uint64_t operator()(uint64_t x, uint32_t lane) const {
+        x = transpose_stage<32>(x, lane, s32);
+        x = transpose_stage<16>(x, lane, s16);
+        x = transpose_stage<8>(x, lane, s8);
+        x = transpose_stage<4>(x, lane, s4);
+        x = transpose_stage<2>(x, lane, s2);
+        return transpose_stage<1>(x, lane, s1);
+    }
+};
 
 // Block = 16 waves over a tile of kStatTileWords words (two 128-byte lines
 // per row) and a chunk of 64-row groups.  Each group's 64 x 32 words are
@@ -1602,6 +1648,7 @@ __global__ void __launch_bounds__(64 * kStatWaves) k_allele_counts(
         return v;
     };
     uint32_t col0 = 0, col1 = 0;
+    const Transpose64 transpose64(lane);
     i8::u32x4 cur = load(g0);
     for (uint32_t g = g0; g < g1; ++g) {
         s_tile[q * PITCH + 2 * p] = ((uint64_t)cur.y << 32) | cur.x;
