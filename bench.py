@@ -144,22 +144,28 @@ def batched_vs_sequential(api, generations):
     cfgs, N, _, _ = replicate_configs(api, 256)
     ms = {}
     shapes = {}
+    grid_pct = None
     for mode in ("graph", "sequential"):
         b = api.Batch(cfgs)
-        ms[mode] = b.run(generations, mode=mode)
+        b.run(1, mode=mode)  # untimed first generation (and the tuner's gametes)
+        if mode == "graph":
+            grid_pct = b.autotune()
+        ms[mode] = b.run(generations - 1, mode=mode)
         shapes[mode] = [b.replicate(r).shape().as_dict() for r in range(len(cfgs))]
         b.close()
+    generations -= 1
     genomes = 2.0 * float(N.sum()) * generations
     return {
         "workload": f"256 replicates, N log-uniform [1e3, 5e4], rho U[0.5, 5], "
                     f"mu U[0.5, 5] x {MU_SCALE}, {CAPACITY} slots each, 1 trait, "
-                    f"{generations} generations from empty founders",
+                    f"generations 2..{generations + 1} from empty founders timed",
         "batched_ms_per_generation": ms["graph"] / generations,
         "sequential_ms_per_generation": ms["sequential"] / generations,
         "batched_genomes_per_s": genomes / (ms["graph"] * 1e-3),
         "sequential_genomes_per_s": genomes / (ms["sequential"] * 1e-3),
         "speedup": ms["sequential"] / ms["graph"],
         "shapes_equal": shapes["graph"] == shapes["sequential"],
+        "batched_grid_percent_of_default": grid_pct,
     }
 
 
@@ -275,7 +281,8 @@ def main():
     graph = not args.eager
     sim.run(args.warmup, graph=graph)
     chunks = sim.autotune()
-    log(f"[bench] autotuned recombination row chunks: {chunks}")
+    shape = sim.launch_shape()
+    log(f"[bench] autotuned launch shapes: {shape}")
     sim.run(1, graph=graph)
     sim.sync()
     t0 = time.perf_counter()
@@ -341,7 +348,7 @@ def main():
                                "neutral 1/f standing variation, rho=5, mu=0.01/gamete, "
                                "1 trait N(0,0.1^2), env N(0,1), fitness exp(-z^2/2)",
                    "haplotypes": h, "words": words, "rho": RHO, "mu": MU,
-                   "graph": graph, "row_chunks": chunks},
+                   "graph": graph, "row_chunks": chunks, "launch_shape": shape},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": traffic,
                      "kernel": "k_recombine", "kernel_ms": rec_ms,

@@ -130,7 +130,8 @@ EXPORTS = [
     "clotho_pairwise_differences", "clotho_device_memory",
     "clotho_set_parents", "clotho_add_mutations", "clotho_evaluate_fitness",
     "clotho_crossover_rows", "clotho_classify_free_space",
-    "clotho_trait_contract_exact_noalloc",
+    "clotho_trait_contract_exact_noalloc", "clotho_get_launch_shape", "clotho_batch_autotune",
+    "clotho_set_launch_shape",
 ]
 
 BATCH_MODES = {"graph": 0, "eager": 1, "sequential": 2}
@@ -174,6 +175,9 @@ def lib():
         getattr(L, n).argtypes = [vp]
     L.clotho_timings.argtypes = [vp, f64p, C.c_int]
     L.clotho_autotune.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.clotho_get_launch_shape.argtypes = [vp, vp]
+    L.clotho_set_launch_shape.argtypes = [vp, vp]
+    L.clotho_batch_autotune.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.clotho_set_timing.argtypes = [vp, C.c_int]
     L.clotho_set_timing.restype = None
     L.clotho_trait_contract.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32, C.c_uint32,
@@ -417,6 +421,23 @@ class Simulation:
                                                 stride, g, out.ctypes.data, ow), "crossover_rows")
         return out
 
+    def launch_shape(self):
+        """The launch shapes in use (dict of clotho_launch_shape)."""
+        out = (C.c_uint32 * 8)()
+        self._check(lib().clotho_get_launch_shape(self.ctx, out), "get_launch_shape")
+        keys = ("recombine_chunks", "traits_variant", "traits_variants", "compact_rows",
+                "compact_chunks")
+        return {k: int(out[i]) for i, k in enumerate(keys)}
+
+    def set_launch_shape(self, **shape):
+        """Apply launch shapes (keys of launch_shape(); missing keys keep theirs)."""
+        cur = self.launch_shape()
+        cur.update(shape)
+        arr = (C.c_uint32 * 8)(cur["recombine_chunks"], cur["traits_variant"],
+                               cur["traits_variants"], cur["compact_rows"],
+                               cur["compact_chunks"])
+        self._check(lib().clotho_set_launch_shape(self.ctx, arr), "set_launch_shape")
+
     def set_timing(self, on=True):
         lib().clotho_set_timing(self.ctx, 1 if on else 0)
 
@@ -554,6 +575,12 @@ class Batch:
         self._check(lib().clotho_batch_run(self.h, int(n), BATCH_MODES[mode], C.byref(ms)),
                     f"batch_run({mode})")
         return ms.value
+
+    def autotune(self):
+        """Persistent grid size of the batched launches, percent of the default."""
+        pct = C.c_uint32(0)
+        self._check(lib().clotho_batch_autotune(self.h, C.byref(pct)), "batch_autotune")
+        return pct.value
 
     def set_timing(self, on=True):
         lib().clotho_batch_set_timing(self.h, 1 if on else 0)

@@ -170,6 +170,42 @@ clotho_ctx* clotho_batch_replicate(clotho_batch* b, uint32_t r) {
     return (b && r < b->reps.size()) ? b->reps[r] : nullptr;
 }
 
+int clotho_batch_autotune(clotho_batch* b, uint32_t* percent) {
+    if (!b) return CLOTHO_ERR_ARG;
+    BATCH_TRY(b, hipSetDevice(b->device));
+    for (clotho_ctx* c : b->reps) BATCH_TRY(b, hipStreamSynchronize(c->stream));
+    int rc = sync_views(b);
+    if (rc != CLOTHO_OK) return rc;
+    const uint32_t par = (uint32_t)b->cur;
+    const uint32_t nums[3] = {1, 1, 2}, dens[3] = {2, 1, 1};
+    EventSet events;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BATCH_TRY(b, events.add(&e0));
+    BATCH_TRY(b, events.add(&e1));
+    float best[3] = {1e30f, 1e30f, 1e30f};
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < 3; ++i) {
+            b->L.grid_scale_num = nums[i];
+            b->L.grid_scale_den = dens[i];
+            BATCH_TRY(b, launch_batch_replay(b->L, par, b->stream));  // warm-up
+            BATCH_TRY(b, hipEventRecord(e0, b->stream));
+            for (int r = 0; r < 3; ++r) BATCH_TRY(b, launch_batch_replay(b->L, par, b->stream));
+            BATCH_TRY(b, hipEventRecord(e1, b->stream));
+            BATCH_TRY(b, hipEventSynchronize(e1));
+            float ms = 0.f;
+            BATCH_TRY(b, hipEventElapsedTime(&ms, e0, e1));
+            best[i] = std::min(best[i], ms);
+        }
+    int bi = 1;
+    for (int i = 0; i < 3; ++i)
+        if (best[i] < best[bi]) bi = i;
+    b->L.grid_scale_num = nums[bi];
+    b->L.grid_scale_den = dens[bi];
+    drop_graphs(b);  // captured launches bake the grid in
+    if (percent) *percent = 100u * nums[bi] / dens[bi];
+    return CLOTHO_OK;
+}
+
 void clotho_batch_set_timing(clotho_batch* b, int enabled) {
     if (b) b->timing.enabled = enabled != 0;
 }

@@ -1226,8 +1226,89 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
         }
     }
     c->p.chunks = best_chunks;
+
+    // The traits pass: every kernel shape over the current population (the
+    // pass writes the traits it already holds), fastest of two passes.
+    auto time_it = [&](auto&& launch, float* out_ms) -> int {
+        HIP_TRY(c, launch());  // warm-up
+        HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+        for (int r = 0; r < kReps; ++r) HIP_TRY(c, launch());
+        HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+        HIP_TRY(c, hipEventSynchronize(ev[1]));
+        HIP_TRY(c, hipEventElapsedTime(out_ms, ev[0], ev[1]));
+        return CLOTHO_OK;
+    };
+    if (c->p.traits) {
+        const uint32_t nv = trait_variants(c);
+        std::vector<float> tbest(nv + 1, 1e30f);
+        for (int pass = 0; pass < 2; ++pass)
+            for (uint32_t v = 1; v <= nv; ++v) {
+                c->trait_variant = v;
+                float ms = 0.f;
+                rc = time_it([&] { return launch_traits_current(c); }, &ms);
+                if (rc) return rc;
+                tbest[v] = std::min(tbest[v], ms);
+            }
+        uint32_t bv = 1;
+        for (uint32_t v = 2; v <= nv; ++v)
+            if (tbest[v] < tbest[bv]) bv = v;
+        c->trait_variant = bv;
+    }
+    // The column compaction of contexts that compact after every generation:
+    // rows per wave step and row chunks, the row pass timed from the current
+    // population into the spare buffer (its plan rebuilt first; every buffer
+    // the plan writes is rebuilt by the next generation before it is read).
+    if (c->cfg.compact_every) {
+        HIP_TRY(c, launch_compact_plan(c));
+        const uint32_t dflt = compact_chunks_of(c);
+        float bms = 1e30f;
+        uint32_t brows = c->compact_rows, bchunks = dflt;
+        for (uint32_t rows : {4u, 8u, 16u})
+            for (uint32_t ch : {std::max<uint32_t>(1, dflt / 2), dflt, dflt * 2}) {
+                c->compact_rows = rows;
+                c->compact_chunks = ch;
+                float ms = 0.f;
+                rc = time_it([&] { return launch_compact_rows(c, c->cur); }, &ms);
+                if (rc) return rc;
+                if (ms < bms) {
+                    bms = ms;
+                    brows = rows;
+                    bchunks = ch;
+                }
+            }
+        c->compact_rows = brows;
+        c->compact_chunks = bchunks;
+    }
     destroy_graphs(c);  // captured launches bake the shape in
     if (chosen_chunks) *chosen_chunks = best_chunks;
+    return CLOTHO_OK;
+}
+
+int clotho_set_launch_shape(clotho_ctx* c, const clotho_launch_shape* in) {
+    if (!c || !in) return CLOTHO_ERR_ARG;
+    const uint32_t nv = c->p.traits ? trait_variants(c) : 1u;
+    if (in->recombine_chunks < 1 || in->recombine_chunks > c->p.max_chunks ||
+        in->traits_variant < 1 || in->traits_variant > nv ||
+        (in->compact_rows != 4 && in->compact_rows != 8 && in->compact_rows != 16))
+        return fail(c, CLOTHO_ERR_ARG,
+                    "set_launch_shape: chunks 1.." + std::to_string(c->p.max_chunks) +
+                        ", traits shape 1.." + std::to_string(nv) + ", compaction rows 4/8/16");
+    c->p.chunks = in->recombine_chunks;
+    c->trait_variant = in->traits_variant;
+    c->compact_rows = in->compact_rows;
+    c->compact_chunks = in->compact_chunks;  // 0: the default
+    destroy_graphs(c);
+    return CLOTHO_OK;
+}
+
+int clotho_get_launch_shape(clotho_ctx* c, clotho_launch_shape* out) {
+    if (!c || !out) return CLOTHO_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->recombine_chunks = c->p.chunks;
+    out->traits_variant = c->trait_variant ? c->trait_variant : 1u;
+    out->traits_variants = c->p.traits ? trait_variants(c) : 0u;
+    out->compact_rows = c->compact_rows;
+    out->compact_chunks = compact_chunks_of(c);
     return CLOTHO_OK;
 }
 

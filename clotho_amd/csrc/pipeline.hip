@@ -1362,13 +1362,12 @@ __global__ void __launch_bounds__(kBlock) k_compact_alleles(Params P, Buffers B)
 // per step, their loads in flight together; otherwise a wave packs one row
 // at a time, reading the plan per row.
 constexpr int kCompactWaves = 4;
-constexpr int kCompactRows = 8;
 
 // The planned path of k_compact_rows: lane l holds the plan (keep mask,
 // compress masks, destination bit) of input words wa + l + 64 i, i < NWL, and
 // a wave packs kCompactRows rows per step starting at row o_first, stepping
 // by kCompactWaves * kCompactRows.
-template <int NWL, class In, class Out, class Keep, class Dst, class Mv>
+template <int NWL, int kCompactRows, class In, class Out, class Keep, class Dst, class Mv>
 __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out out, Keep keepm,
                                                      Dst dst, Mv mvg,
                                                      unsigned long long (*s_row)[64], uint32_t W,
@@ -1427,6 +1426,7 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
     }
 }
 
+template <int kCompactRows>
 __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, Buffers B,
                                                                      uint32_t src) {
     __shared__ unsigned long long s_out[kCompactWaves][kCompactRows][64];
@@ -1461,13 +1461,13 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
     const uint32_t r0 = chunk * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
     const uint32_t u = tile * 64 + lane;
     if (wb - wa <= 64) {
-        compact_rows_planned<1>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, wa, wb, lo_bit,
-                                r0 + wave * kCompactRows, r1);
+        compact_rows_planned<1, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2,
+                                              wa, wb, lo_bit, r0 + wave * kCompactRows, r1);
         return;
     }
     if (wb - wa <= 128) {
-        compact_rows_planned<2>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, wa, wb, lo_bit,
-                                r0 + wave * kCompactRows, r1);
+        compact_rows_planned<2, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2,
+                                              wa, wb, lo_bit, r0 + wave * kCompactRows, r1);
         return;
     }
     for (uint32_t o = r0 + wave; o < r1; o += kCompactWaves) {
@@ -1940,6 +1940,9 @@ CLOTHO_SINGLE(k_recombine_wide, RecombineWide)
 CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
 using Traits4Even = TraitsI8<5, 1, true>;  // T <= 4, grid within one round of resident blocks
 using Traits4 = TraitsI8<4, 1, false>;      // T <= 4
+using Traits4Two = TraitsI8<2, 1, false>;   // T <= 4, two row tiles per wave (autotuner)
+using Traits8Four = TraitsI8<4, 2, false>;  // T <= 8, four row tiles per wave (autotuner)
+using Traits32Two = TraitsI8<2, 8, false>;  // T <= 32, two row tiles per wave (autotuner)
 using Traits8 = TraitsI8<2, 2, false>;      // T <= 8
 using Traits16 = TraitsI8<4, 4, false>;     // T <= 16 (four row tiles: 5% faster than two at H = 1e6)
 using BatchTraits16 = TraitsI8<2, 4, false>;  // batched T <= 16: the rows per block of Traits8
@@ -1949,6 +1952,10 @@ CLOTHO_SINGLE(k_traits_i8_pop_fixed, Traits4)
 CLOTHO_SINGLE(k_traits_i8_pop_t8, Traits8)
 CLOTHO_SINGLE(k_traits_i8_pop_t16, Traits16)
 CLOTHO_SINGLE(k_traits_i8_pop_t32, Traits32)
+CLOTHO_SINGLE(k_traits_i8_pop_two, Traits4Two)
+CLOTHO_SINGLE(k_traits_i8_pop_t8_four, Traits8Four)
+CLOTHO_SINGLE(k_traits_i8_pop_t16_two, BatchTraits16)
+CLOTHO_SINGLE(k_traits_i8_pop_t32_two, Traits32Two)
 CLOTHO_SINGLE(k_free_reduce, FreeReduce)
 CLOTHO_SINGLE(k_free_compact, FreeCompact)
 CLOTHO_SINGLE(k_fitness, Fitness)
@@ -2052,7 +2059,10 @@ static hipError_t launch_traits_of(clotho_ctx* c, int which, bool pack) {
     const uint32_t sel = (uint32_t)which;
     if (pack) TRY_LAUNCH(launch_single<PackI8>(k_pack_i8_pop, c, sel));
     const uint32_t nt = (c->p.traits + 3) / 4;
+    const uint32_t v = c->trait_variant;  // 0: default shape; the autotuner may pick another
     if (nt <= 1) {
+        if (v == 2) return launch_single<Traits4>(k_traits_i8_pop_fixed, c, sel);
+        if (v == 3) return launch_single<Traits4Two>(k_traits_i8_pop_two, c, sel);
         // one round of the even kernel where it fits, else the fixed tiles
         if (!c->trait_resident) TRY_LAUNCH(query_trait_residency(c));
         const uint32_t cus = (uint32_t)std::max(1, c->prop.multiProcessorCount);
@@ -2060,10 +2070,23 @@ static hipError_t launch_traits_of(clotho_ctx* c, int which, bool pack) {
         if (c->p.trait_blocks) return launch_single<Traits4Even>(k_traits_i8_pop, c, sel);
         return launch_single<Traits4>(k_traits_i8_pop_fixed, c, sel);
     }
-    if (nt <= 2) return launch_single<Traits8>(k_traits_i8_pop_t8, c, sel);
-    if (nt <= 4) return launch_single<Traits16>(k_traits_i8_pop_t16, c, sel);
+    if (nt <= 2) {
+        if (v == 2) return launch_single<Traits8Four>(k_traits_i8_pop_t8_four, c, sel);
+        return launch_single<Traits8>(k_traits_i8_pop_t8, c, sel);
+    }
+    if (nt <= 4) {
+        if (v == 2) return launch_single<BatchTraits16>(k_traits_i8_pop_t16_two, c, sel);
+        return launch_single<Traits16>(k_traits_i8_pop_t16, c, sel);
+    }
+    if (v == 2) return launch_single<Traits32Two>(k_traits_i8_pop_t32_two, c, sel);
     return launch_single<Traits32>(k_traits_i8_pop_t32, c, sel);
 }
+
+// Launch-shape candidates of the traits pass for the context's trait count
+// (values of clotho_ctx::trait_variant; 1 is the default shape).
+uint32_t trait_variants(const clotho_ctx* c) { return (c->p.traits + 3) / 4 <= 1 ? 3u : 2u; }
+
+hipError_t launch_traits_current(clotho_ctx* c) { return launch_traits_of(c, c->cur, true); }
 
 hipError_t query_trait_residency(clotho_ctx* c) {
     int per_cu = 0;
@@ -2151,7 +2174,8 @@ template <class Op>
 hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_t s) {
     const uint32_t total = L.total[op];
     if (!total) return hipSuccess;
-    const uint32_t per_cu = Op::kThreads >= 1024 ? 2u : 8u;
+    const uint32_t per_cu = std::max<uint32_t>(
+        1u, (Op::kThreads >= 1024 ? 2u : 8u) * L.grid_scale_num / L.grid_scale_den);
     // An odd grid: virtual block v runs on block v mod grid, and replicate
     // ranges are multiples of the column-tile count (a power of two), so an
     // even grid would hand every live tile-0 block to the same quarter of
@@ -2164,6 +2188,23 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
 }
 
 }  // namespace
+
+// The batch's recombination and traits over the last generation's gametes,
+// parents [par], children into [1 - par] (the autotuner's timed replay).
+hipError_t launch_batch_replay(const BatchLaunch& L, uint32_t par, hipStream_t s) {
+    if (L.wide_events)
+        TRY_LAUNCH(launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s));
+    else
+        TRY_LAUNCH(launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s));
+    if (!L.traits) return hipSuccess;
+    const uint32_t chi = 1 - par;
+    TRY_LAUNCH(launch_batched<PackI8>(L, kOpPackI8, chi, s));
+    const uint32_t nt = (L.traits + 3) / 4;
+    if (nt <= 1) return launch_batched<Traits4>(L, kOpTraits, chi, s);
+    if (nt <= 2) return launch_batched<Traits8>(L, kOpTraits, chi, s);
+    if (nt <= 4) return launch_batched<BatchTraits16>(L, kOpTraits, chi, s);
+    return launch_batched<Traits32>(L, kOpTraits, chi, s);
+}
 
 hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream_t s,
                                    hipEvent_t* marks) {
@@ -2206,6 +2247,34 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
     return hipSuccess;
 }
 
+// Rows of the compaction, kCompactWaves waves of c->compact_rows rows per
+// step over P.tiles x compact_chunks blocks (default: about eight blocks per
+// compute unit over the column tiles at capacity).
+uint32_t compact_chunks_of(const clotho_ctx* c) {
+    if (c->compact_chunks) return c->compact_chunks;
+    const uint32_t cus = (uint32_t)std::max(1, c->prop.multiProcessorCount);
+    const uint32_t want = (8u * cus + c->p.tiles - 1) / c->p.tiles;
+    return std::max<uint32_t>(1, std::min<uint32_t>(want, (c->p.h + 255) / 256));
+}
+
+hipError_t launch_compact_rows(clotho_ctx* c, int src) {
+    const Params& P = c->p;
+    const dim3 grid(P.tiles * compact_chunks_of(c)), block(64 * kCompactWaves);
+    const uint32_t s = (uint32_t)src;
+    if (c->compact_rows == 4)
+        hipLaunchKernelGGL(k_compact_rows<4>, grid, block, 0, c->stream, P, c->b, s);
+    else if (c->compact_rows == 16)
+        hipLaunchKernelGGL(k_compact_rows<16>, grid, block, 0, c->stream, P, c->b, s);
+    else
+        hipLaunchKernelGGL(k_compact_rows<8>, grid, block, 0, c->stream, P, c->b, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_plan(clotho_ctx* c) {
+    hipLaunchKernelGGL(k_compact_plan, dim3(1), dim3(1024), 0, c->stream, c->p, c->b);
+    return hipGetLastError();
+}
+
 hipError_t launch_compact(clotho_ctx* c, int src) {
     const Params& P = c->p;
     const uint32_t s = (uint32_t)src;
@@ -2214,9 +2283,7 @@ hipError_t launch_compact(clotho_ctx* c, int src) {
     hipLaunchKernelGGL(k_compact_alleles, dim3((P.s_cap + kBlock - 1) / kBlock), dim3(kBlock), 0,
                        c->stream, P, c->b);
     TRY_LAUNCH(hipGetLastError());
-    hipLaunchKernelGGL(k_compact_rows, dim3(P.tiles * P.chunks), dim3(64 * kCompactWaves), 0,
-                       c->stream, P, c->b, s);
-    TRY_LAUNCH(hipGetLastError());
+    TRY_LAUNCH(launch_compact_rows(c, src));
     const size_t n = std::max<size_t>(std::max<size_t>(P.s_cap, (size_t)P.h * P.traits),
                                       std::max<size_t>(P.n, P.w_cap));
     hipLaunchKernelGGL(k_compact_finish, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock),

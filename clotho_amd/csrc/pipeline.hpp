@@ -157,6 +157,9 @@ struct BatchLaunch {
     uint32_t traits = 0;             // trait count shared by every replicate
     bool wide_events = false;        // some replicate keeps more than 64 events per gamete
     uint32_t cus = 1;                // compute units of the device
+    // persistent grid per compute unit: the default (2 blocks of 1024
+    // threads, 8 of 256) scaled by num / den (clotho_batch_autotune)
+    uint32_t grid_scale_num = 1, grid_scale_den = 1;
 };
 
 struct Timing {
@@ -204,6 +207,10 @@ struct clotho_ctx {
     size_t device_bytes = 0;    // sum of the context's device allocations
     uint32_t next_n = 0;        // offspring individuals of following generations (0: keep)
     uint32_t trait_resident = 0;  // resident blocks per unit of the even traits kernel (0: unknown)
+    // launch shapes the autotuner may change (0 / defaults: the built-in choice)
+    uint32_t trait_variant = 0;   // traits-pass kernel shape, 1 .. trait_variants()
+    uint32_t compact_rows = 8;    // rows per wave step of the column compaction: 4, 8, 16
+    uint32_t compact_chunks = 0;  // row chunks of the compaction grid (0: about 8 blocks per CU)
 };
 
 namespace clotho {
@@ -240,6 +247,14 @@ hipError_t launch_free_reduce(clotho_ctx* c);
 hipError_t launch_compact_fitness(clotho_ctx* c);
 hipError_t launch_commit(clotho_ctx* c);
 hipError_t launch_population_analysis(clotho_ctx* c);  // after an upload
+// traits pass over the current population (digit planes repacked); the
+// candidates of its launch shape; the compaction's plan and row pass alone
+// (the autotuner times them into the spare population buffer)
+hipError_t launch_traits_current(clotho_ctx* c);
+uint32_t trait_variants(const clotho_ctx* c);
+hipError_t launch_compact_plan(clotho_ctx* c);
+hipError_t launch_compact_rows(clotho_ctx* c, int src);
+uint32_t compact_chunks_of(const clotho_ctx* c);
 hipError_t query_trait_residency(clotho_ctx* c);  // sets c->trait_resident
 // Column compaction of population [src] into [1 - src] (fixed and lost
 // columns removed, slots renumbered in order); the caller flips cur.
@@ -269,4 +284,5 @@ int after_generation(const clotho_ctx* c, int cur);  // population index after a
 void batch_op_blocks(const Params& P, uint32_t* out);  // kOpCount block counts
 hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream_t s,
                                    hipEvent_t* marks);  // marks: 6 events or null
+hipError_t launch_batch_replay(const BatchLaunch& L, uint32_t par, hipStream_t s);
 }  // namespace clotho

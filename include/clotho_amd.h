@@ -318,6 +318,22 @@ void clotho_set_timing(clotho_ctx* ctx, int enabled);
  * times the coming generation's recombination for candidate row-chunk counts
  * on the resident population and keeps the fastest.  State is unchanged. */
 int clotho_autotune(clotho_ctx* ctx, uint32_t* chosen_chunks);
+/* The launch shapes in use (clotho_autotune tunes the recombination's row
+ * chunks, the traits pass's kernel shape among the candidates for the trait
+ * count, and, for contexts with compact_every, the compaction's rows per wave
+ * step and row chunks; no reference counterpart). */
+typedef struct clotho_launch_shape {
+    uint32_t recombine_chunks; /* row chunks of the recombination grid               */
+    uint32_t traits_variant;   /* traits-pass kernel shape, 1 = the built-in choice   */
+    uint32_t traits_variants;  /* shapes available for the trait count               */
+    uint32_t compact_rows;     /* rows per wave step of the column compaction        */
+    uint32_t compact_chunks;   /* row chunks of the compaction grid                  */
+    uint32_t reserved[3];
+} clotho_launch_shape;
+int clotho_get_launch_shape(clotho_ctx* ctx, clotho_launch_shape* out);
+/* Apply launch shapes (e.g. tuned in an earlier run); compact_chunks 0 takes
+ * the default; traits_variants is ignored.  Invalid shapes: CLOTHO_ERR_ARG. */
+int clotho_set_launch_shape(clotho_ctx* ctx, const clotho_launch_shape* in);
 
 /* ---- replicate batches ------------------------------------------------- */
 /* Many independent populations of uneven size and parameters advanced
@@ -350,6 +366,12 @@ int clotho_batch_run(clotho_batch* batch, uint32_t n_generations, int mode, doub
  * clotho_timings; returns the number of generations timed. */
 int clotho_batch_timings(clotho_batch* batch, double* ms, int cap);
 void clotho_batch_set_timing(clotho_batch* batch, int enabled);
+/* Size of the persistent grids: times the batch's recombination and traits
+ * launches over the last generation's gametes (written into the spare
+ * population buffers; visible state unchanged) for grids of 1/2, 1 and 2
+ * times the default blocks per compute unit and keeps the fastest;
+ * percent_of_default receives the choice (50, 100 or 200). */
+int clotho_batch_autotune(clotho_batch* batch, uint32_t* percent_of_default);
 
 /* ---- standalone trait contraction -------------------------------------- */
 /* z[h][t] = sum_j bit(h,j) * effects[j][t] over a device-resident genotype
