@@ -1111,32 +1111,36 @@ extern "C" void or_sim_shape(const or_sim* s, clotho_shape* out) {
     out->all_neutral = s->all_neutral;
     out->fitness_constant = s->fitness_constant;
 }
+// Copies of possibly empty vectors (an empty vector may hold no storage at all).
+template <class T>
+inline void copy_out(void* dst, const std::vector<T>& v, size_t n) {
+    if (dst && n) std::memcpy(dst, v.data(), n * sizeof(T));
+}
 extern "C" void or_sim_rows(const or_sim* s, uint64_t* out) {
-    std::memcpy(out, s->child.data(), s->child.size() * 8);
+    copy_out(out, s->child, s->child.size());
 }
 extern "C" void or_sim_alleles(const or_sim* s, double* positions, double* effects,
                                uint32_t* ages) {
-    if (positions) std::memcpy(positions, s->pos.data(), (size_t)s->A * 8);
-    if (effects) std::memcpy(effects, s->eff.data(), (size_t)s->A * s->T * 8);
-    if (ages) std::memcpy(ages, s->age.data(), (size_t)s->A * 4);
+    copy_out(positions, s->pos, s->A);
+    copy_out(effects, s->eff, (size_t)s->A * s->T);
+    copy_out(ages, s->age, s->A);
 }
 extern "C" void or_sim_free_space(const or_sim* s, uint32_t* fixed, uint32_t* lost) {
-    if (fixed) std::memcpy(fixed, s->fixed_list.data(), s->fixed_list.size() * 4);
-    if (lost) std::memcpy(lost, s->lost_list.data(), s->lost_list.size() * 4);
+    copy_out(fixed, s->fixed_list, s->fixed_list.size());
+    copy_out(lost, s->lost_list, s->lost_list.size());
 }
 extern "C" void or_sim_traits(const or_sim* s, double* traits) {
-    std::memcpy(traits, s->traits.data(), s->traits.size() * 8);
+    copy_out(traits, s->traits, s->traits.size());
 }
 extern "C" void or_sim_fitness(const or_sim* s, double* fitness) {
-    std::memcpy(fitness, s->fitness.data(), s->fitness.size() * 8);
+    copy_out(fitness, s->fitness, s->fitness.size());
 }
 extern "C" void or_sim_gametes(const or_sim* s, uint32_t* top_row, uint32_t* bottom_row,
                                uint32_t* events, uint32_t* mutations) {
-    const size_t n = s->top_row.size();
-    if (top_row) std::memcpy(top_row, s->top_row.data(), n * 4);
-    if (bottom_row) std::memcpy(bottom_row, s->bottom_row.data(), n * 4);
-    if (events) std::memcpy(events, s->n_events.data(), n * 4);
-    if (mutations) std::memcpy(mutations, s->n_mut.data(), s->n_mut.size() * 4);
+    copy_out(top_row, s->top_row, s->top_row.size());
+    copy_out(bottom_row, s->bottom_row, s->bottom_row.size());
+    copy_out(events, s->n_events, s->n_events.size());
+    copy_out(mutations, s->n_mut, s->n_mut.size());
 }
 extern "C" void or_sim_phase_seconds(const or_sim* s, double* out7) {
     std::memcpy(out7, s->phase, sizeof(s->phase));
