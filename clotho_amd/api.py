@@ -426,7 +426,7 @@ class Simulation:
         out = (C.c_uint32 * 8)()
         self._check(lib().clotho_get_launch_shape(self.ctx, out), "get_launch_shape")
         keys = ("recombine_chunks", "traits_variant", "traits_variants", "compact_rows",
-                "compact_chunks")
+                "compact_chunks", "compact_tile_words")
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def set_launch_shape(self, **shape):
@@ -435,7 +435,7 @@ class Simulation:
         cur.update(shape)
         arr = (C.c_uint32 * 8)(cur["recombine_chunks"], cur["traits_variant"],
                                cur["traits_variants"], cur["compact_rows"],
-                               cur["compact_chunks"])
+                               cur["compact_chunks"], cur["compact_tile_words"])
         self._check(lib().clotho_set_launch_shape(self.ctx, arr), "set_launch_shape")
 
     def set_timing(self, on=True):

@@ -68,6 +68,10 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-max", action="store_true", help="skip the largest-capacity allocation")
+    ap.add_argument("--compact-tile-words", type=int, default=0,
+                    help="compaction tile width after autotuning (48, 56, 60 or 64; 0 keeps the tuned one)")
+    ap.add_argument("--compact-rows", type=int, default=0,
+                    help="compaction rows per wave step after autotuning (4, 8 or 16; 0 keeps the tuned one)")
     args = ap.parse_args()
 
     from clotho_amd import api
@@ -85,6 +89,12 @@ def main():
         f"device bytes {dev_bytes / 2**30:.1f} GiB ({time.perf_counter() - t0:.1f}s)")
     sim.run(args.warmup, graph=True)
     chunks = sim.autotune()
+    if args.compact_tile_words:
+        sim.set_launch_shape(compact_tile_words=args.compact_tile_words)
+    if args.compact_rows:
+        sim.set_launch_shape(compact_rows=args.compact_rows)
+    shape = sim.launch_shape()
+    log(f"[large] launch shape {shape}")
     sim.run(1, graph=True)
     sim.sync()
     t0 = time.perf_counter()
@@ -139,7 +149,8 @@ def main():
         "config": {"N": N_IND, "haplotypes": h, "traits": TRAITS, "rho": RHO, "mu": MU,
                    "standing_slots": args.slots, "words_final": words,
                    "fitness": "exp(-|z|^2/2), 16 traits", "compaction": "every generation",
-                   "effects": f"MVN(rho=0.3) x {EFFECT_SD}", "row_chunks": chunks},
+                   "effects": f"MVN(rho=0.3) x {EFFECT_SD}", "row_chunks": chunks,
+                   "launch_shape": shape},
         "phase_ms": phases,
         "recombine_alg_gbps": rec_gbps,
         "hbm": {"device_bytes": dev_bytes, "device_gib": dev_bytes / 2**30,

@@ -1258,11 +1258,15 @@ int clotho_autotune(clotho_ctx* c, uint32_t* chosen_chunks) {
     // rows per wave step and row chunks, the row pass timed from the current
     // population into the spare buffer (its plan rebuilt first; every buffer
     // the plan writes is rebuilt by the next generation before it is read).
+    // The population was compacted at the end of the last generation, so the
+    // timed pass removes nothing: the tile width, whose merit depends on
+    // where columns are removed, is left as set.
     if (c->cfg.compact_every) {
         HIP_TRY(c, launch_compact_plan(c));
-        const uint32_t dflt = compact_chunks_of(c);
         float bms = 1e30f;
-        uint32_t brows = c->compact_rows, bchunks = dflt;
+        uint32_t brows = c->compact_rows, bchunks = c->compact_chunks;
+        c->compact_chunks = 0;
+        const uint32_t dflt = compact_chunks_of(c);
         for (uint32_t rows : {4u, 8u, 16u})
             for (uint32_t ch : {std::max<uint32_t>(1, dflt / 2), dflt, dflt * 2}) {
                 c->compact_rows = rows;
@@ -1289,14 +1293,19 @@ int clotho_set_launch_shape(clotho_ctx* c, const clotho_launch_shape* in) {
     const uint32_t nv = c->p.traits ? trait_variants(c) : 1u;
     if (in->recombine_chunks < 1 || in->recombine_chunks > c->p.max_chunks ||
         in->traits_variant < 1 || in->traits_variant > nv ||
-        (in->compact_rows != 4 && in->compact_rows != 8 && in->compact_rows != 16))
+        (in->compact_rows != 4 && in->compact_rows != 8 && in->compact_rows != 16) ||
+        (in->compact_tile_words != 0 && in->compact_tile_words != 48 &&
+         in->compact_tile_words != 56 && in->compact_tile_words != 60 &&
+         in->compact_tile_words != 64))
         return fail(c, CLOTHO_ERR_ARG,
                     "set_launch_shape: chunks 1.." + std::to_string(c->p.max_chunks) +
-                        ", traits shape 1.." + std::to_string(nv) + ", compaction rows 4/8/16");
+                        ", traits shape 1.." + std::to_string(nv) +
+                        ", compaction rows 4/8/16, compaction tile words 48/56/60/64");
     c->p.chunks = in->recombine_chunks;
     c->trait_variant = in->traits_variant;
     c->compact_rows = in->compact_rows;
     c->compact_chunks = in->compact_chunks;  // 0: the default
+    c->compact_tile_words = in->compact_tile_words ? in->compact_tile_words : 64u;
     destroy_graphs(c);
     return CLOTHO_OK;
 }
@@ -1309,6 +1318,7 @@ int clotho_get_launch_shape(clotho_ctx* c, clotho_launch_shape* out) {
     out->traits_variants = c->p.traits ? trait_variants(c) : 0u;
     out->compact_rows = c->compact_rows;
     out->compact_chunks = compact_chunks_of(c);
+    out->compact_tile_words = c->compact_tile_words;
     return CLOTHO_OK;
 }
 

@@ -1352,30 +1352,45 @@ __global__ void __launch_bounds__(kBlock) k_compact_alleles(Params P, Buffers B)
     for (uint32_t t = 0; t < P.traits; ++t) tmp[(size_t)n * P.traits + t] = eff[(size_t)j * P.traits + t];
 }
 
-// Rows: block = one 64-word tile of the compacted row x a chunk of rows.  The
-// input words whose kept bits fall in the tile are [wa, wb) (dst is
+// Rows: wave = one tile of TW <= 64 words of the compacted row over a chunk
+// of rows; block = kCompactWaves adjacent tiles over the same chunk.  The
+// input words whose kept bits fall in a tile are [wa, wb) (dst is
 // non-decreasing); every input word contributes its packed bits to at most
 // two output words of the tile, OR-ed into a per-wave LDS row (no global
-// atomics).  When the tile draws on at most 64 input words (few columns
-// removed) lane l keeps the plan of input word wa + l in registers (of wa + l
-// and wa + 64 + l up to 128 input words) and a wave packs kCompactRows rows
-// per step, their loads in flight together; otherwise a wave packs one row
-// at a time, reading the plan per row.
+// atomics).  When the tile draws on at most 64 input words lane l keeps the
+// plan of input word wa + l in registers (of wa + l and wa + 64 + l up to 128
+// input words) and the wave packs kCompactRows rows per step, their loads in
+// flight together; otherwise it packs one row at a time, reading the plan
+// per row.  With a fraction f of the columns removed a 64-word tile draws on
+// about 64 / (1 - f) + 1 input words, so any removal sends it down the
+// two-plan path, which compresses a second, nearly idle set of words;
+// narrower tiles stay on one plan but write rows in pieces that do not start
+// on 128-byte lines.  The autotuner picks the width.
 constexpr int kCompactWaves = 4;
+
+// Block x of a grid of nb runs on XCD x mod 8; the returned index is a
+// bijection of [0, nb) under which XCD k runs one contiguous range (XCDs
+// below nb mod 8 take one block more).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t x, uint32_t nb) {
+    const uint32_t q = nb / 8, r = nb % 8, xcd = x % 8, i = x / 8;
+    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
 
 // The planned path of k_compact_rows: lane l holds the plan (keep mask,
 // compress masks, destination bit) of input words wa + l + 64 i, i < NWL, and
-// a wave packs kCompactRows rows per step starting at row o_first, stepping
-// by kCompactWaves * kCompactRows.
+// the wave packs kCompactRows rows per step from row o_first, `step` rows
+// apart, up to row r1.
 template <int NWL, int kCompactRows, class In, class Out, class Keep, class Dst, class Mv>
 __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out out, Keep keepm,
                                                      Dst dst, Mv mvg,
                                                      unsigned long long (*s_row)[64], uint32_t W,
-                                                     uint32_t W2, uint32_t wa, uint32_t wb,
-                                                     uint32_t lo_bit, uint32_t o_first,
-                                                     uint32_t r1) {
+                                                     uint32_t W2, uint32_t tw, uint32_t wa,
+                                                     uint32_t wb, uint32_t lo_bit,
+                                                     uint32_t o_first, uint32_t r1,
+                                                     uint32_t step) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t u = lo_bit / 64 + lane;
+    const bool writes = lane < tw && u < W2;
     uint32_t wr[NWL], n[NWL];
     uint64_t keep[NWL], mv[NWL][6];
     int d[NWL];
@@ -1390,12 +1405,22 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
         for (int j = 0; j < 6; ++j) mv[i][j] = mvg[(size_t)j * P.w_cap + wr[i]];
         d[i] = (int)dst[wr[i]] - (int)lo_bit;  // tile-relative bit of the packed word
     }
-    for (uint32_t o0 = o_first; o0 < r1; o0 += kCompactWaves * kCompactRows) {
-        uint64_t xin[kCompactRows][NWL];
+    if (o_first >= r1) return;
+    // software pipeline: the rows of step s + 1 are loaded (row indices
+    // clamped into the chunk, so every load is unconditional and in bounds)
+    // before step s is packed, keeping loads in flight through the packing
+    uint64_t xin[kCompactRows][NWL];
+#pragma unroll
+    for (int k = 0; k < kCompactRows; ++k)
+#pragma unroll
+        for (int i = 0; i < NWL; ++i) xin[k][i] = in[(size_t)min(o_first + k, r1 - 1) * W + wr[i]];
+    for (uint32_t o0 = o_first; o0 < r1; o0 += step) {
+        uint64_t xnx[kCompactRows][NWL];
+        const uint32_t o1 = min(o0 + step, r1 - 1);
 #pragma unroll
         for (int k = 0; k < kCompactRows; ++k)
 #pragma unroll
-            for (int i = 0; i < NWL; ++i) xin[k][i] = in[(size_t)min(o0 + k, r1 - 1) * W + wr[i]];
+            for (int i = 0; i < NWL; ++i) xnx[k][i] = in[(size_t)min(o1 + k, r1 - 1) * W + wr[i]];
 #pragma unroll
         for (int k = 0; k < kCompactRows; ++k) s_row[k][lane] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -1418,31 +1443,53 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // every packed row into its own registers first, then the stores: a
+        // store's data registers may not be overwritten before the store
+        // completes, so stores sharing one register pair would each wait out
+        // the previous store's write
+        uint64_t packed[kCompactRows];
+#pragma unroll
+        for (int k = 0; k < kCompactRows; ++k) packed[k] = s_row[k][lane];
 #pragma unroll
         for (int k = 0; k < kCompactRows; ++k)
-            if (o0 + k < r1 && u < W2) out[(size_t)(o0 + k) * W2 + u] = s_row[k][lane];
+            if (o0 + k < r1 && writes) out[(size_t)(o0 + k) * W2 + u] = packed[k];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int k = 0; k < kCompactRows; ++k)
+#pragma unroll
+            for (int i = 0; i < NWL; ++i) xin[k][i] = xnx[k][i];
     }
 }
 
 template <int kCompactRows>
 __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, Buffers B,
-                                                                     uint32_t src) {
+                                                                     uint32_t src, uint32_t tw,
+                                                                     uint32_t tiles) {
     __shared__ unsigned long long s_out[kCompactWaves][kCompactRows][64];
     const auto st = gbl(B.state);
     const uint32_t V = uniform_read(&st->kept), W = uniform_read(&st->wp);
     const uint32_t W2 = ((V + 511) / 512) * 8;
-    const uint32_t tile = blockIdx.x % P.tiles, chunk = blockIdx.x / P.tiles;
-    if (tile * 64 >= W2) return;  // uniform for the block
+    // The input windows of neighbouring output tiles overlap by a word or
+    // two, so their edge lines are read by both.  A block's waves take
+    // kCompactWaves adjacent tiles over the same rows in the same order,
+    // and blocks, which go to the eight XCDs (each with its own L2)
+    // round-robin, are numbered so that each XCD runs a contiguous range of
+    // (chunk, tile group) blocks: the second read of an edge line finds it
+    // in the unit's L1 or the XCD's L2 instead of fetching it again.
+    const uint32_t tgroups = (tiles + kCompactWaves - 1) / kCompactWaves;
+    const uint32_t vb = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t tg = vb % tgroups, chunk = vb / tgroups;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t tile = tg * kCompactWaves + wave;
+    if (tile >= tiles || tile * tw >= W2) return;  // uniform for the wave
     const auto in = gbl(B.rows[src]);
     const auto out = gbl(B.rows[1 - src]);
     const auto keepm = gbl(B.fix_part);
     const auto dst = gbl(B.word_cnt);
     const auto mvg = gbl(B.pm);
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t lo_bit = tile * 4096, hi_bit = lo_bit + 4096;
+    const uint32_t lo_bit = tile * tw * 64, hi_bit = lo_bit + tw * 64;
     // wa = last word with dst <= lo_bit, wb = first word with dst >= hi_bit
     uint32_t a = 0, b = W;
     while (b - a > 1) {
@@ -1457,20 +1504,21 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
         if (dst[mid] >= hi_bit) b = mid; else a = mid + 1;
     }
     const uint32_t wb = a;
-    const uint32_t rows_per_chunk = (P.h + gridDim.x / P.tiles - 1) / (gridDim.x / P.tiles);
+    const uint32_t chunks = gridDim.x / tgroups;
+    const uint32_t rows_per_chunk = (P.h + chunks - 1) / chunks;
     const uint32_t r0 = chunk * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
-    const uint32_t u = tile * 64 + lane;
+    const uint32_t u = tile * tw + lane;
     if (wb - wa <= 64) {
-        compact_rows_planned<1, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2,
-                                              wa, wb, lo_bit, r0 + wave * kCompactRows, r1);
+        compact_rows_planned<1, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, tw,
+                                              wa, wb, lo_bit, r0, r1, kCompactRows);
         return;
     }
     if (wb - wa <= 128) {
-        compact_rows_planned<2, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2,
-                                              wa, wb, lo_bit, r0 + wave * kCompactRows, r1);
+        compact_rows_planned<2, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, tw,
+                                              wa, wb, lo_bit, r0, r1, kCompactRows);
         return;
     }
-    for (uint32_t o = r0 + wave; o < r1; o += kCompactWaves) {
+    for (uint32_t o = r0; o < r1; ++o) {
         s_out[wave][0][lane] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1496,7 +1544,7 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (u < W2) out[(size_t)o * W2 + u] = s_out[wave][0][lane];
+        if (lane < tw && u < W2) out[(size_t)o * W2 + u] = s_out[wave][0][lane];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -2248,25 +2296,36 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
 }
 
 // Rows of the compaction, kCompactWaves waves of c->compact_rows rows per
-// step over P.tiles x compact_chunks blocks (default: about eight blocks per
-// compute unit over the column tiles at capacity).
+// step over (column tiles of c->compact_tile_words words at capacity) x
+// compact_chunks blocks (default: about eight blocks per compute unit).
+static uint32_t compact_tiles_of(const clotho_ctx* c) {
+    return (c->p.w_cap + c->compact_tile_words - 1) / c->compact_tile_words;
+}
+
+// blocks of kCompactWaves adjacent tiles per row chunk
+static uint32_t compact_tile_groups_of(const clotho_ctx* c) {
+    return (compact_tiles_of(c) + kCompactWaves - 1) / kCompactWaves;
+}
+
 uint32_t compact_chunks_of(const clotho_ctx* c) {
     if (c->compact_chunks) return c->compact_chunks;
     const uint32_t cus = (uint32_t)std::max(1, c->prop.multiProcessorCount);
-    const uint32_t want = (8u * cus + c->p.tiles - 1) / c->p.tiles;
+    const uint32_t groups = compact_tile_groups_of(c);
+    const uint32_t want = (8u * cus + groups - 1) / groups;
     return std::max<uint32_t>(1, std::min<uint32_t>(want, (c->p.h + 255) / 256));
 }
 
 hipError_t launch_compact_rows(clotho_ctx* c, int src) {
     const Params& P = c->p;
-    const dim3 grid(P.tiles * compact_chunks_of(c)), block(64 * kCompactWaves);
+    const uint32_t tw = c->compact_tile_words, tiles = compact_tiles_of(c);
+    const dim3 grid(compact_tile_groups_of(c) * compact_chunks_of(c)), block(64 * kCompactWaves);
     const uint32_t s = (uint32_t)src;
     if (c->compact_rows == 4)
-        hipLaunchKernelGGL(k_compact_rows<4>, grid, block, 0, c->stream, P, c->b, s);
+        hipLaunchKernelGGL(k_compact_rows<4>, grid, block, 0, c->stream, P, c->b, s, tw, tiles);
     else if (c->compact_rows == 16)
-        hipLaunchKernelGGL(k_compact_rows<16>, grid, block, 0, c->stream, P, c->b, s);
+        hipLaunchKernelGGL(k_compact_rows<16>, grid, block, 0, c->stream, P, c->b, s, tw, tiles);
     else
-        hipLaunchKernelGGL(k_compact_rows<8>, grid, block, 0, c->stream, P, c->b, s);
+        hipLaunchKernelGGL(k_compact_rows<8>, grid, block, 0, c->stream, P, c->b, s, tw, tiles);
     return hipGetLastError();
 }
 

@@ -211,6 +211,7 @@ struct clotho_ctx {
     uint32_t trait_variant = 0;   // traits-pass kernel shape, 1 .. trait_variants()
     uint32_t compact_rows = 8;    // rows per wave step of the column compaction: 4, 8, 16
     uint32_t compact_chunks = 0;  // row chunks of the compaction grid (0: about 8 blocks per CU)
+    uint32_t compact_tile_words = 64;  // compacted words per compaction wave: 48, 56, 60 or 64
 };
 
 namespace clotho {

@@ -328,11 +328,13 @@ typedef struct clotho_launch_shape {
     uint32_t traits_variants;  /* shapes available for the trait count               */
     uint32_t compact_rows;     /* rows per wave step of the column compaction        */
     uint32_t compact_chunks;   /* row chunks of the compaction grid                  */
-    uint32_t reserved[3];
+    uint32_t compact_tile_words; /* compacted words per compaction wave: 48, 56, 60, 64 (default 64; not autotuned) */
+    uint32_t reserved[2];
 } clotho_launch_shape;
 int clotho_get_launch_shape(clotho_ctx* ctx, clotho_launch_shape* out);
-/* Apply launch shapes (e.g. tuned in an earlier run); compact_chunks 0 takes
- * the default; traits_variants is ignored.  Invalid shapes: CLOTHO_ERR_ARG. */
+/* Apply launch shapes (e.g. tuned in an earlier run); compact_chunks 0 and
+ * compact_tile_words 0 take the defaults; traits_variants is ignored.
+ * Invalid shapes: CLOTHO_ERR_ARG. */
 int clotho_set_launch_shape(clotho_ctx* ctx, const clotho_launch_shape* in);
 
 /* ---- replicate batches ------------------------------------------------- */

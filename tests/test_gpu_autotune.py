@@ -28,6 +28,7 @@ def test_autotuned_shapes_keep_parity(traits, compact):
         print(f"generation {g + 1}: {shape}", flush=True)
         assert 1 <= shape["traits_variant"] <= shape["traits_variants"]
         assert shape["compact_rows"] in (4, 8, 16) and shape["compact_chunks"] >= 1
+        assert shape["compact_tile_words"] == 64  # not tuned: stays as set
         seen.add(shape["traits_variant"])
         compare_state(sim, orc, f"after autotune {g + 1}")
     for g in range(3):  # the tuned shapes, eager and replayed
@@ -37,19 +38,23 @@ def test_autotuned_shapes_keep_parity(traits, compact):
         compare_state(sim, orc, f"tuned generation {g + 1}")
 
 
-@pytest.mark.parametrize("traits,variant,rows", [(3, 1, 4), (3, 2, 16), (3, 3, 8), (6, 2, 8),
-                                                (12, 2, 4), (20, 2, 16)])
-def test_each_shape_is_exact(traits, variant, rows):
+@pytest.mark.parametrize("traits,variant,rows,tile_words",
+                         [(3, 1, 4, 64), (3, 2, 16, 60), (3, 3, 8, 56), (6, 2, 8, 48),
+                          (12, 2, 4, 56), (20, 2, 16, 64)])
+def test_each_shape_is_exact(traits, variant, rows, tile_words):
     """Every shape the tuner can pick computes the same generation: traits
-    kernels of each trait range, compaction rows per step and chunk counts."""
+    kernels of each trait range, compaction rows per step, tile widths and
+    chunk counts."""
     cfg = api.make_config(900, 3.0, 2.0, seed=71 + traits, traits=traits, neutral_p=0.1,
                           sigma=0.3, fitness=api.FIT_MV_STABILIZING, fitness_scale=2.0,
                           max_alleles=1 << 16, compact=True)
     sim, orc = api.Simulation(cfg), oracle.OracleSim(cfg)
     sim.set_launch_shape(traits_variant=variant, compact_rows=rows, compact_chunks=3,
-                         recombine_chunks=5)
+                         compact_tile_words=tile_words, recombine_chunks=5)
     with pytest.raises(api.ClothoError):
         sim.set_launch_shape(traits_variant=sim.launch_shape()["traits_variants"] + 1)
+    with pytest.raises(api.ClothoError):
+        sim.set_launch_shape(compact_tile_words=40)
     for g in range(4):
         sim.run(1, graph=(g >= 2))
         orc.step()

@@ -101,11 +101,14 @@ def test_compaction_invariant_at_bench_shape():
     assert a.rows().shape[0] == h
 
 
-@pytest.mark.parametrize("removed", [0.01, 0.5])
-def test_compact_row_paths(removed):
-    """Four tiles of input words.  With 1% of the columns removed every output
-    tile draws on at most 64 input words (plan in registers, rows in groups);
-    with half removed the tiles draw on about 128 (plan read per row)."""
+@pytest.mark.parametrize("tile_words", [64, 60, 56, 48])
+@pytest.mark.parametrize("removed", [0.01, 0.04, 0.5])
+def test_compact_row_paths(removed, tile_words):
+    """Four tiles of input words.  With 1% of the columns removed an output
+    tile of 64 words draws on at most 64 input words (plan in registers, rows
+    in groups), with 4% on up to 128 (two plans) unless the tile is narrower;
+    with half removed the tiles draw on about 128 or more (plan read per
+    row).  Every tile width gives the oracle's rows."""
     rng = np.random.default_rng(int(removed * 100) + 11)
     n, alleles = 150, 16000
     words = ((alleles + 511) // 512) * 8
@@ -124,9 +127,11 @@ def test_compact_row_paths(removed):
     sim, orc = api.Simulation(cfg), oracle.OracleSim(cfg)
     sim.set_population(rows, pos, eff)
     orc.set_population(rows, pos, eff)
+    sim.set_launch_shape(compact_tile_words=tile_words)
+    assert sim.launch_shape()["compact_tile_words"] == tile_words
     sim.compact()
     orc.compact()
-    compare_state(sim, orc, f"after compact ({removed:.0%} removed)")
+    compare_state(sim, orc, f"after compact ({removed:.0%} removed, {tile_words}-word tiles)")
     sim.run(2)
     orc.step()
     orc.step()
