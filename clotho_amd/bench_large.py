@@ -62,6 +62,47 @@ def standing_variation(slots, seed=42):
     return freqs, positions, effects
 
 
+def compaction_spans(api, slots, generations):
+    """Where a compaction after each generation removes columns, and the
+    input words each output tile of the compaction's row pass draws on: a
+    tile of tw words whose kept bits come from at most 64 input words takes
+    the one-plan path, up to 128 the two-plan path, beyond that the per-row
+    path.  The generations run without compact_every and are compacted
+    explicitly after their free-space lists are read, which is the same
+    sequence of states."""
+    sim = api.Simulation(config(api, slots + 8 * 4096, compact=False))
+    freqs, positions, effects = standing_variation(slots)
+    sim.set_synthetic(freqs, positions, effects, seed=42)
+    for g in range(generations):
+        sim.run(1, graph=False)
+        s = sim.shape()
+        fixed, lost = sim.free_space()
+        keep = np.ones(s.words * 64, bool)
+        keep[s.alleles:] = False
+        keep[np.asarray(fixed, np.int64)] = False
+        keep[np.asarray(lost, np.int64)] = False
+        cnt = keep.reshape(s.words, 64).sum(1)
+        dst = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+        kept = int(cnt.sum())
+        w2 = (kept + 511) // 512 * 8
+        rec = {"generation": g + 1, "alleles": s.alleles, "words": s.words,
+               "removed": s.alleles - kept, "removed_by_eighth":
+               [round(float(x.mean()), 4) for x in np.array_split(~keep[:s.alleles], 8)]}
+        for tw in (64, 60, 56, 48):
+            paths = [0, 0, 0]
+            widest = 0
+            for t in range((w2 + tw - 1) // tw):
+                wa = int(np.searchsorted(dst, t * tw * 64, side="right")) - 1
+                wb = int(np.searchsorted(dst, (t + 1) * tw * 64, side="left"))
+                widest = max(widest, wb - wa)
+                paths[0 if wb - wa <= 64 else 1 if wb - wa <= 128 else 2] += 1
+            rec[f"tiles_{tw}"] = {"one_plan": paths[0], "two_plans": paths[1],
+                                  "per_row": paths[2], "widest_span": widest}
+        print(json.dumps(rec), flush=True)
+        sim.compact()
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--slots", type=int, default=262_144)
@@ -72,10 +113,17 @@ def main():
                     help="compaction tile width after autotuning (48, 56, 60 or 64; 0 keeps the tuned one)")
     ap.add_argument("--compact-rows", type=int, default=0,
                     help="compaction rows per wave step after autotuning (4, 8 or 16; 0 keeps the tuned one)")
+    ap.add_argument("--spans", type=int, default=0,
+                    help="instead of the benchmark: G generations compacted one by one, printing "
+                         "where columns are removed and how many input words each compaction "
+                         "tile draws on")
     args = ap.parse_args()
 
     from clotho_amd import api
 
+    if args.spans:
+        compaction_spans(api, args.slots, args.spans)
+        return
     free0, total = api.device_memory(0)
     h = 2 * N_IND
     cap = args.slots + 8 * 4096
