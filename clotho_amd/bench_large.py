@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--no-max", action="store_true", help="skip the largest-capacity allocation")
     ap.add_argument("--compact-tile-words", type=int, default=0,
                     help="compaction tile width after autotuning (48, 56, 60 or 64; 0 keeps the tuned one)")
+    ap.add_argument("--traits-variant", type=int, default=0,
+                    help="traits kernel shape after autotuning (1..traits_variants; 0 keeps the tuned one)")
     ap.add_argument("--compact-rows", type=int, default=0,
                     help="compaction rows per wave step after autotuning (4, 8 or 16; 0 keeps the tuned one)")
     ap.add_argument("--spans", type=int, default=0,
@@ -141,6 +143,8 @@ def main():
         sim.set_launch_shape(compact_tile_words=args.compact_tile_words)
     if args.compact_rows:
         sim.set_launch_shape(compact_rows=args.compact_rows)
+    if args.traits_variant:
+        sim.set_launch_shape(traits_variant=args.traits_variant)
     shape = sim.launch_shape()
     log(f"[large] launch shape {shape}")
     sim.run(1, graph=True)

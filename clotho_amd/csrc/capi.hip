@@ -728,9 +728,8 @@ int clotho_allele_counts(clotho_ctx* c, const uint32_t* rows, uint32_t n_rows,
     if (rows)
         for (uint32_t i = 0; i < n; ++i)
             if (rows[i] >= H) return fail(c, CLOTHO_ERR_ARG, "allele_counts: row out of range");
-    const uint32_t ngroups = stat_word_tiles(W), rg = (n + 63) / 64;
-    const uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>(
-                                                      rg, 2048 / std::max<uint32_t>(1, ngroups)));
+    const uint32_t ngroups = stat_word_tiles(W);
+    const uint32_t chunks = stat_chunks(W, n);
     uint32_t *d_sample = nullptr, *d_cols = nullptr, *d_rows = nullptr, *d_cp = nullptr,
              *d_rp = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -1595,7 +1595,7 @@ __global__ void k_compact_state(Buffers B) {
 // column's carriers among the 64 rows.  Partials per (row chunk, column) and
 // (word group, row) are summed by two small kernels (no atomics).
 // ---------------------------------------------------------------------------
-constexpr int kStatWaves = 16;
+constexpr int kStatWaves = (int)kStatBlockWaves;
 
 // x of lane ^ S: DPP quad permutes for S = 1, 2, swizzles for S = 4, 8, 16
 // (both stay inside 32-lane halves), a bpermute for S = 32.
@@ -1659,68 +1659,154 @@ struct Transpose64 {
     }
 };
 
-// Block = 16 waves over a tile of kStatTileWords words (two 128-byte lines
-// per row) and a chunk of 64-row groups.  Each group's 64 x 32 words are
-// loaded coalesced (16 lanes x 16 bytes per row) into LDS, the next group's
-// load in flight meanwhile; then wave v takes words 2v, 2v + 1 with lane = row
-// (a 33-word row pitch keeps those reads in distinct banks), adds each row's
-// popcounts and transposes each word for the column popcounts.
+// Carry-save adder on bit planes: (hi, lo) = a + b + c per bit position.
+__device__ __forceinline__ void csa(uint64_t& hi, uint64_t& lo, uint64_t a, uint64_t b,
+                                    uint64_t c) {
+    const uint64_t u = a ^ b;
+    hi = (a & b) | (u & c);
+    lo = u ^ c;
+}
+
+// Block = kStatWaves waves over one tile of kStatTileWords = 64 words (512
+// bytes per row) and a chunk of at most kStatChunkRows sampled rows, split
+// between the waves and, within a wave, between its halves: lane l holds
+// words 2 (l % 32) and 2 (l % 32) + 1 of the tile (one 16-byte load per row)
+// for the rows of its half-wave h = l / 32 (at most 255).  Rows are loaded 8
+// per half at a time, the next 8 in flight while the current 8 are counted.
+// Column margins: each lane keeps, per word, a bit-sliced counter of 8 planes
+// (plane i, bit j = bit i of the number of its rows carrying column j), fed 8
+// rows at a time through a carry-save (Harley-Seal) tree: 7 adders per 8 rows
+// for the ones, twos and fours planes, and the eights carry rippled into the
+// planes above.  At the end of the chunk a word's 8 counters (kStatWaves
+// waves x 2 halves) x 8 planes are exactly 64 x 64 bits: one transpose across
+// a wave leaves lane j = column j holding the 8 counts of column j as bytes,
+// summed by two v_sad_u8.  Row margins: each lane's per-row popcounts of its
+// two words are summed over the 32 lanes of its half by a reduce-scatter
+// butterfly (8 rows -> 4 -> 2 -> 1 value per lane, then two full exchanges).
+// Partials per (row chunk, column) and (word tile, row) are summed by two
+// small kernels (no atomics).  kSample: rows come from sample_in (all eight
+// row indices of a batch are loaded before its rows; a per-row choice
+// between the two would make every row load wait for the previous ones).
+template <bool kSample>
 __global__ void __launch_bounds__(64 * kStatWaves) k_allele_counts(
     const uint64_t* __restrict__ rows_in, uint32_t W, const uint32_t* __restrict__ sample_in,
-    uint32_t n, uint32_t groups_per_chunk, uint32_t* __restrict__ col_part_in,
+    uint32_t n, uint32_t rows_per_chunk, uint32_t* __restrict__ col_part_in,
     uint32_t* __restrict__ row_part_in) {
-    constexpr uint32_t TW = kStatTileWords, PITCH = TW + 1;
-    __shared__ uint64_t s_tile[64 * PITCH];
-    __shared__ uint32_t s_rc[kStatWaves][64];
+    constexpr uint32_t TW = kStatTileWords, kContrib = 2 * kStatWaves, kPlanes = 8;
+    static_assert(kContrib * kPlanes == 64, "a word's counters fill one 64 x 64 transpose");
+    __shared__ uint64_t s_pl[TW][kContrib * kPlanes + 1];  // +1: spread the writers' banks
     const auto rows = gbl(rows_in);
     const auto col_part = gbl(col_part_in);
     const auto row_part = gbl(row_part_in);
+    const auto sample = gbl(sample_in);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t h = lane >> 5, pr = lane & 31;
     const uint32_t ntw = stat_word_tiles(W);
     const uint32_t wt = blockIdx.x % ntw, chunk = blockIdx.x / ntw;
     const uint32_t w0 = wt * TW;
-    const uint32_t rg_total = (n + 63) / 64;
-    const uint32_t g0 = chunk * groups_per_chunk, g1 = min(rg_total, g0 + groups_per_chunk);
-    // loader: row q of the group, words w0 + 2p and w0 + 2p + 1
-    const uint32_t q = threadIdx.x >> 4, p = threadIdx.x & 15;
-    const bool wok = w0 + 2 * p < W;  // W is even: both words are in the row
-    auto load = [&](uint32_t g) {
-        const uint32_t k = g * 64 + q;
-        i8::u32x4 v = {0, 0, 0, 0};
-        if (g < g1 && k < n && wok) {
-            const uint32_t r = sample_in ? gbl(sample_in)[k] : k;
-            v = *(const __attribute__((address_space(1))) i8::u32x4*)(rows + (size_t)r * W + w0 +
-                                                                       2 * p);
-        }
-        return v;
-    };
-    uint32_t col0 = 0, col1 = 0;
-    const Transpose64 transpose64(lane);
-    i8::u32x4 cur = load(g0);
-    for (uint32_t g = g0; g < g1; ++g) {
-        s_tile[q * PITCH + 2 * p] = ((uint64_t)cur.y << 32) | cur.x;
-        s_tile[q * PITCH + 2 * p + 1] = ((uint64_t)cur.w << 32) | cur.z;
-        __syncthreads();
-        cur = load(g + 1);
-        const uint64_t x0 = s_tile[lane * PITCH + 2 * wave];
-        const uint64_t x1 = s_tile[lane * PITCH + 2 * wave + 1];
-        s_rc[wave][lane] = (uint32_t)(__popcll(x0) + __popcll(x1));
-        col0 += (uint32_t)__popcll(transpose64(x0, lane));
-        col1 += (uint32_t)__popcll(transpose64(x1, lane));
-        __syncthreads();
-        const uint32_t k = g * 64 + lane;
-        if (wave == 0 && k < n) {
-            uint32_t rc = 0;
+    const uint32_t r0 = min(n, chunk * rows_per_chunk), r1 = min(n, r0 + rows_per_chunk);
+    const uint32_t per_wave = (r1 - r0 + kStatWaves - 1) / kStatWaves;
+    const uint32_t rw0 = min(r1, r0 + wave * per_wave), rw1 = min(r1, rw0 + per_wave);
+    const uint32_t wlocal = 2 * pr;
+    const bool wok = w0 + wlocal < W;  // W is even: both words are in the row
+    const uint32_t wsrc = wok ? w0 + wlocal : 0;
+    uint64_t pl0[kPlanes], pl1[kPlanes];
 #pragma unroll
-            for (int v = 0; v < kStatWaves; ++v) rc += s_rc[v][lane];
-            row_part[(size_t)wt * n + k] = rc;
+    for (uint32_t i = 0; i < kPlanes; ++i) pl0[i] = pl1[i] = 0;
+    // the half's rows are rw0 + 2 i + h; batches of 8 per half, every load
+    // unconditional (rows past the wave's range re-read its first row and are
+    // zeroed when counted)
+    const uint32_t half_rows = (rw1 - rw0 + 1) / 2;
+    auto load8 = [&](uint32_t i0, i8::u32x4 (&x)[8]) {
+        uint32_t r[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const uint32_t k = rw0 + 2 * (i0 + t) + h;
+            const uint32_t kc = k < rw1 ? k : rw0;
+            r[t] = kSample ? sample[kc] : kc;
         }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            x[t] = *(const __attribute__((address_space(1))) i8::u32x4*)(rows + (size_t)r[t] * W +
+                                                                         wsrc);
+    };
+    i8::u32x4 x[8];
+    if (half_rows) load8(0, x);
+    for (uint32_t i0 = 0; i0 < half_rows; i0 += 8) {
+        i8::u32x4 xn[8];
+        if (i0 + 8 < half_rows) load8(i0 + 8, xn);
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (!(rw0 + 2 * (i0 + t) + h < rw1 && wok)) x[t] = i8::u32x4{0, 0, 0, 0};
+        // row margins: popcounts of the lane's two words, summed over the half
+        uint32_t v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            v[t] = (uint32_t)(__builtin_popcount(x[t].x) + __builtin_popcount(x[t].y) +
+                              __builtin_popcount(x[t].z) + __builtin_popcount(x[t].w));
+        {
+            const bool b16 = pr & 16, b8 = pr & 8, b4 = pr & 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t keep = b16 ? v[j + 4] : v[j], send = b16 ? v[j] : v[j + 4];
+                v[j] = keep + lane_xor<16>(send, lane);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t keep = b8 ? v[j + 2] : v[j], send = b8 ? v[j] : v[j + 2];
+                v[j] = keep + lane_xor<8>(send, lane);
+            }
+            {
+                const uint32_t keep = b4 ? v[1] : v[0], send = b4 ? v[0] : v[1];
+                v[0] = keep + lane_xor<4>(send, lane);
+            }
+            v[0] += lane_xor<2>(v[0], lane);
+            v[0] += lane_xor<1>(v[0], lane);
+            const uint32_t t = (b16 ? 4u : 0u) + (b8 ? 2u : 0u) + (b4 ? 1u : 0u);
+            const uint32_t k = rw0 + 2 * (i0 + t) + h;
+            if ((pr & 3) == 0 && k < rw1) row_part[(size_t)wt * n + k] = v[0];
+        }
+        // column margins: 8 rows into the bit-sliced counters of each word
+#pragma unroll
+        for (int wd = 0; wd < 2; ++wd) {
+            uint64_t d[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                d[t] = wd ? (((uint64_t)x[t].w << 32) | x[t].z) : (((uint64_t)x[t].y << 32) | x[t].x);
+            uint64_t* pl = wd ? pl1 : pl0;
+            uint64_t tA, tB, fA, fB, e;
+            csa(tA, pl[0], pl[0], d[0], d[1]);
+            csa(tB, pl[0], pl[0], d[2], d[3]);
+            csa(fA, pl[1], pl[1], tA, tB);
+            csa(tA, pl[0], pl[0], d[4], d[5]);
+            csa(tB, pl[0], pl[0], d[6], d[7]);
+            csa(fB, pl[1], pl[1], tA, tB);
+            csa(e, pl[2], pl[2], fA, fB);
+#pragma unroll
+            for (uint32_t i = 3; i < kPlanes; ++i) {
+                const uint64_t c = pl[i] & e;
+                pl[i] ^= e;
+                e = c;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[t] = xn[t];
     }
-    const uint32_t w = w0 + 2 * wave;
-    if (w < W) {
-        col_part[(size_t)chunk * W * 64 + (size_t)w * 64 + lane] = col0;
-        col_part[(size_t)chunk * W * 64 + (size_t)(w + 1) * 64 + lane] = col1;
+    // a word's counters into LDS: row (contributor c = 2 wave + h, plane i)
+    const uint32_t c = 2 * wave + h;
+#pragma unroll
+    for (uint32_t i = 0; i < kPlanes; ++i) {
+        s_pl[wlocal][c * kPlanes + i] = pl0[i];
+        s_pl[wlocal + 1][c * kPlanes + i] = pl1[i];
+    }
+    __syncthreads();
+    const Transpose64 transpose64(lane);
+    for (uint32_t wl = wave; wl < TW; wl += kStatWaves) {
+        const uint64_t tr = transpose64(s_pl[wl][lane], lane);  // byte c: counter c's count
+        const uint32_t cnt = __builtin_amdgcn_sad_u8((uint32_t)tr, 0u, 0u) +
+                             __builtin_amdgcn_sad_u8((uint32_t)(tr >> 32), 0u, 0u);
+        if (w0 + wl < W) col_part[(size_t)chunk * W * 64 + (size_t)(w0 + wl) * 64 + lane] = cnt;
     }
 }
 
@@ -1729,8 +1815,17 @@ __global__ void __launch_bounds__(kBlock) k_sum_columns(const uint32_t* __restri
                                                         uint32_t A, uint32_t* __restrict__ out) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= A) return;
-    uint32_t s = 0;
-    for (uint32_t c = 0; c < chunks; ++c) s += gbl(part)[(size_t)c * W * 64 + j];
+    const auto p = gbl(part) + j;
+    const size_t stride = (size_t)W * 64;
+    uint32_t s = 0, c = 0;
+    for (; c + 8 <= chunks; c += 8) {  // eight independent loads in flight
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[(c + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < chunks; ++c) s += p[c * stride];
     gbl(out)[j] = s;
 }
 
@@ -2357,11 +2452,16 @@ hipError_t launch_allele_counts(clotho_ctx* c, const uint64_t* rows, uint32_t W,
                                 uint32_t* d_rows, uint32_t* d_col_part, uint32_t* d_row_part,
                                 uint32_t chunks) {
     const uint32_t ngroups = stat_word_tiles(W);
-    const uint32_t rg = (n + 63) / 64;
-    const uint32_t per = (rg + chunks - 1) / chunks;
+    const uint32_t per = n ? (n + chunks - 1) / chunks : 0u;
+    if (per > kStatChunkRows) return hipErrorInvalidValue;  // counters hold 255 rows per half
     if (W && n) {
-        hipLaunchKernelGGL(k_allele_counts, dim3(ngroups * chunks), dim3(64 * kStatWaves), 0,
-                           c->stream, rows, W, d_sample, n, per, d_col_part, d_row_part);
+        if (d_sample)
+            hipLaunchKernelGGL(k_allele_counts<true>, dim3(ngroups * chunks), dim3(64 * kStatWaves),
+                               0, c->stream, rows, W, d_sample, n, per, d_col_part, d_row_part);
+        else
+            hipLaunchKernelGGL(k_allele_counts<false>, dim3(ngroups * chunks),
+                               dim3(64 * kStatWaves), 0, c->stream, rows, W, d_sample, n, per,
+                               d_col_part, d_row_part);
         TRY_LAUNCH(hipGetLastError());
     }
     if (A) {

@@ -19,11 +19,28 @@ constexpr int kScanBlock = 256;   // items per block of every two-level scan
 constexpr double kEffectLimit = 1073741823.0;  // 2^30 - 1
 // allele-slot capacity limit: int8 digit sums over a row stay within int32
 constexpr uint32_t kMaxAlleleSlots = 1u << 24;
-// words per column tile of the population-statistics pass; row partials
-// are kept per tile
-constexpr uint32_t kStatTileWords = 32;
+// words per column tile of the population-statistics pass (row partials are
+// kept per tile), its waves per block, and the most rows of one of its row
+// chunks (4 waves x 2 half-waves x 255 rows: its 8-plane bit-sliced counters
+// count to 255)
+constexpr uint32_t kStatTileWords = 64;
+constexpr uint32_t kStatBlockWaves = 4;
+constexpr uint32_t kStatChunkRows = 2040;
 __host__ __device__ inline uint32_t stat_word_tiles(uint32_t words) {
     return (words + kStatTileWords - 1) / kStatTileWords;
+}
+// row chunks of the statistics pass over n rows of `words` words: at most
+// 2048 blocks (two rounds of four resident blocks per compute unit: a few
+// blocks past a round would run alone), each chunk at most kStatChunkRows
+// rows and at least one row
+inline uint32_t stat_chunks(uint32_t words, uint32_t n) {
+    const uint32_t tiles = stat_word_tiles(words) ? stat_word_tiles(words) : 1u;
+    uint32_t chunks = 2048u / tiles;
+    if (chunks < 1u) chunks = 1u;
+    const uint32_t need = (n + kStatChunkRows - 1) / kStatChunkRows;
+    if (chunks < need) chunks = need;
+    const uint32_t most = n ? n : 1u;
+    return chunks < most ? chunks : most;
 }
 
 // Scalars of the generation, kept in device memory so that a captured
