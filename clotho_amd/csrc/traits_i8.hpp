@@ -119,13 +119,18 @@ __device__ inline WaveTiles wave_tiles_even(uint32_t block, uint32_t wave, uint3
 // through a 256-entry LDS table (byte -> 8 bytes).
 //
 // traits_i8_wave is one wave's share: rows base .. base + 16 cnt - 1 (cnt <=
-// RT, base and cnt wave-uniform); s_b and s_lut are the block's tables.
+// RT, base and cnt wave-uniform); s_b0, s_b1 (the two stage buffers) and
+// s_lut are the block's tables.  The stage buffers are two LDS objects, not
+// halves of one: a read of one buffer is then known not to touch the other,
+// which direct-to-LDS loads are filling, and needs no wait for those loads
+// (with an even tile count the buffer of every stage is a constant).
 template <int RT, int NT>
 __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows_in, uint32_t h,
                                                uint32_t W, uint32_t T,
                                                const uint4* __restrict__ packed_in,
                                                int64_t* __restrict__ traits_in, uint32_t base,
-                                               uint32_t cnt, u32x4* s_b, uint2* s_lut) {
+                                               uint32_t cnt, u32x4* s_b0, u32x4* s_b1,
+                                               uint2* s_lut) {
     static_assert(NT == 1 || NT == 2 || NT == 4 || NT == 8, "N tiles: 1, 2, 4 or 8");
     constexpr int QS = 16 / NT;  // q-blocks per stage
     const auto rows = as_global(rows_in);
@@ -168,16 +173,23 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
     // stage k = (group k / NT, part k mod NT) into buffer k & 1: piece i =
     // (tile i / QS, q-block part QS + i mod QS), 1 KB each, four per wave
     const uint32_t n_stages = NG * NT;
+    // four pieces per wave in a loop of known trip count (the block has four
+    // waves) over a wave-uniform index: a loop the compiler takes as
+    // divergent, of unknown length, leaves it unable to count the loads in
+    // flight, and every later wait for a load becomes a wait for all of them
+    const uint32_t wave_s = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave);
     auto stage_b = [&](uint32_t k) {
         if (k < n_stages) {
             const uint32_t G = k / NT, q0 = (k % NT) * QS;
-            for (uint32_t i = wave; i < 16; i += 4) {
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t i = wave_s + 4 * j;
                 const uint32_t nt = i / QS, q = q0 + i % QS;
                 const g_u32x4* src =
                     (const g_u32x4*)(packed + (((size_t)nt * NG + G) * 16 + q) * 64) + lane;
                 __builtin_amdgcn_global_load_lds(
                     (const __attribute__((address_space(1))) void*)src,
-                    (__attribute__((address_space(3))) void*)(s_b + (k & 1) * 16 * 64 + i * 64),
+                    (__attribute__((address_space(3))) void*)(((k & 1) ? s_b1 : s_b0) + i * 64),
                     16, 0, 0);
             }
         }
@@ -213,7 +225,7 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
                 }
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
-                    const u32x4 bq = s_b[(k & 1) * 16 * 64 + (nt * QS + qq) * 64 + lane];
+                    const u32x4 bq = ((k & 1) ? s_b1 : s_b0)[(nt * QS + qq) * 64 + lane];
                     const i8x4 bv = i8x4{(int)bq.x, (int)bq.y, (int)bq.z, (int)bq.w};
 #pragma unroll
                     for (int r = 0; r < RT; ++r)
@@ -258,18 +270,20 @@ __device__ __forceinline__ void traits_i8_block(const uint64_t* __restrict__ row
                                                 const uint32_t* p_words, uint32_t T,
                                                 const uint4* __restrict__ packed,
                                                 int64_t* __restrict__ traits, WaveTiles wt) {
-    __shared__ u32x4 s_b[2 * 16 * 64];
+    __shared__ u32x4 s_b0[16 * 64];
+    __shared__ u32x4 s_b1[16 * 64];
     __shared__ uint2 s_lut[256];
     const uint32_t W = (uint32_t)__builtin_amdgcn_readfirstlane((int)*as_global(p_words));
     const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(wt.cnt, (uint32_t)RT));
     const uint32_t base = 16u * (uint32_t)__builtin_amdgcn_readfirstlane((int)wt.t0);
     if constexpr (kSplit && RT > 1) {
         if (cnt < (uint32_t)RT) {
-            traits_i8_wave<RT - 1, NT>(rows, h, W, T, packed, traits, base, cnt, s_b, s_lut);
+            traits_i8_wave<RT - 1, NT>(rows, h, W, T, packed, traits, base, cnt, s_b0, s_b1,
+                                       s_lut);
             return;
         }
     }
-    traits_i8_wave<RT, NT>(rows, h, W, T, packed, traits, base, cnt, s_b, s_lut);
+    traits_i8_wave<RT, NT>(rows, h, W, T, packed, traits, base, cnt, s_b0, s_b1, s_lut);
 }
 
 }  // namespace i8

@@ -87,7 +87,13 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
     constexpr uint32_t QS = kStageBlocks;
     constexpr uint32_t FR = QS * TERMS;       // 1 KB fragments per stage
     constexpr uint32_t SPG = 32 / QS;         // stages per group
-    __shared__ uint4 s_b[2 * FR * 64];
+    static_assert(FR % 4 == 0 && SPG % 2 == 0, "stage pieces per wave, stage parity per group");
+    // the two stage buffers are two LDS objects, not halves of one: a read of
+    // one is then known not to touch the other, which direct-to-LDS loads are
+    // filling, and needs no wait for those loads (SPG is even, so the buffer
+    // of every stage is a constant)
+    __shared__ uint4 s_b0[FR * 64];
+    __shared__ uint4 s_b1[FR * 64];
     __shared__ uint4 s_lut[256];
     const auto rows = i8::as_global(rows_in);
     const auto packed = i8::as_global(packed_in);
@@ -133,15 +139,22 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
     // + QS - 1, every term: FR contiguous 1 KB fragments, wave v moving
     // fragments v, v + 4, ...
     const uint32_t n_stages = (G1 > G0 ? G1 - G0 : 0) * SPG;
+    // FR / 4 pieces per wave in a loop of known trip count over a
+    // wave-uniform index (a loop of unknown length leaves the compiler unable
+    // to count the loads in flight)
+    const uint32_t wave_s = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave);
     auto stage_b = [&](uint32_t k) {
         if (k < n_stages) {
             const uint32_t G = G0 + k / SPG, qb = (k % SPG) * QS;
             const auto src = packed + (((size_t)G * 32 + qb) * TERMS) * 64 + lane;
-            for (uint32_t i = wave; i < FR; i += 4)
+#pragma unroll
+            for (uint32_t j = 0; j < FR / 4; ++j) {
+                const uint32_t i = wave_s + 4 * j;
                 __builtin_amdgcn_global_load_lds(
                     (const __attribute__((address_space(1))) void*)(src + (size_t)i * 64),
-                    (__attribute__((address_space(3))) void*)(s_b + (k & 1) * FR * 64 + i * 64),
+                    (__attribute__((address_space(3))) void*)(((k & 1) ? s_b1 : s_b0) + i * 64),
                     16, 0, 0);
+            }
         }
     };
     // vmcnt(0) only: the direct-to-LDS loads and the next group's rows have landed
@@ -176,7 +189,7 @@ __global__ void __launch_bounds__(256) k_traits_bf16(const uint64_t* __restrict_
                 }
 #pragma unroll
                 for (int tm = 0; tm < TERMS; ++tm) {
-                    const uint4 bq = s_b[(k & 1) * FR * 64 + (qq * TERMS + tm) * 64 + lane];
+                    const uint4 bq = ((k & 1) ? s_b1 : s_b0)[(qq * TERMS + tm) * 64 + lane];
                     v8bf bv;
                     __builtin_memcpy(&bv, &bq, 16);
 #pragma unroll
