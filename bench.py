@@ -145,6 +145,8 @@ def batched_vs_sequential(api, generations):
     ms = {}
     shapes = {}
     grid_pct = None
+    phases = None
+    rec_bytes = None
     for mode in ("graph", "sequential"):
         b = api.Batch(cfgs)
         b.run(1, mode=mode)  # untimed first generation (and the tuner's gametes)
@@ -152,9 +154,24 @@ def batched_vs_sequential(api, generations):
             grid_pct = b.autotune()
         ms[mode] = b.run(generations - 1, mode=mode)
         shapes[mode] = [b.replicate(r).shape().as_dict() for r in range(len(cfgs))]
+        if mode == "graph":
+            # per-phase kernel time of the batched launches after the timed
+            # run (eager, with timing events), and the recombination's
+            # algorithmic bytes at those widths: two parent rows read and one
+            # child row written per offspring haplotype of every replicate
+            words = np.array([s["words"] for s in shapes[mode]], dtype=np.float64)
+            rec_bytes = float((24.0 * words * 2.0 * N).sum())
+            b.set_timing(True)
+            b.run(3, mode="eager")
+            n, ph = b.timings()
+            b.set_timing(False)
+            phases = {k: round(float(ph[i]) / max(n, 1), 4) for i, k in
+                      enumerate(["generation", "draw", "recombine", "traits", "free_space",
+                                 "fitness"])}
         b.close()
     generations -= 1
     genomes = 2.0 * float(N.sum()) * generations
+    rec_gbps = rec_bytes / (phases["recombine"] * 1e-3) / 1e9 if phases and phases["recombine"] else None
     return {
         "workload": f"256 replicates, N log-uniform [1e3, 5e4], rho U[0.5, 5], "
                     f"mu U[0.5, 5] x {MU_SCALE}, {CAPACITY} slots each, 1 trait, "
@@ -166,6 +183,10 @@ def batched_vs_sequential(api, generations):
         "speedup": ms["sequential"] / ms["graph"],
         "shapes_equal": shapes["graph"] == shapes["sequential"],
         "batched_grid_percent_of_default": grid_pct,
+        "batched_phase_ms_after_run": phases,
+        "batched_recombine_alg_bytes": rec_bytes,
+        "batched_recombine_alg_gbps": rec_gbps,
+        "batched_recombine_frac_of_8000": rec_gbps / 8000.0 if rec_gbps else None,
     }
 
 
