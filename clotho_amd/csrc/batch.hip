@@ -26,6 +26,7 @@ struct clotho_batch {
     std::vector<RepView> host_views;  // last uploaded copy
     RepView* d_views = nullptr;
     uint32_t* d_offs = nullptr;
+    uint32_t* d_live_offs = nullptr;  // [R + 1], written on the device each generation
     BatchLaunch L;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int cur = 0;
@@ -77,6 +78,7 @@ int sync_views(clotho_batch* b) {
     if (!b->d_views) {
         BATCH_TRY(b, hipMalloc((void**)&b->d_views, sizeof(RepView) * R));
         BATCH_TRY(b, hipMalloc((void**)&b->d_offs, sizeof(uint32_t) * offs.size()));
+        BATCH_TRY(b, hipMalloc((void**)&b->d_live_offs, sizeof(uint32_t) * (R + 1)));
     }
     BATCH_TRY(b, hipMemcpy(b->d_views, v.data(), sizeof(RepView) * R, hipMemcpyHostToDevice));
     BATCH_TRY(b, hipMemcpy(b->d_offs, offs.data(), sizeof(uint32_t) * offs.size(),
@@ -85,6 +87,7 @@ int sync_views(clotho_batch* b) {
     BatchLaunch& L = b->L;
     L.views = b->d_views;
     L.offs = b->d_offs;
+    L.live_offs = b->d_live_offs;
     L.R = R;
     for (int k = 0; k < kOpCount; ++k) L.total[k] = offs[(size_t)k * (R + 1) + R];
     L.traits = v[0].P.traits;
@@ -157,6 +160,7 @@ void clotho_batch_destroy(clotho_batch* b) {
     drop_graphs(b);
     if (b->d_views) (void)hipFree(b->d_views);
     if (b->d_offs) (void)hipFree(b->d_offs);
+    if (b->d_live_offs) (void)hipFree(b->d_live_offs);
     for (clotho_ctx* c : b->reps) clotho_destroy(c);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -326,9 +326,9 @@ int clotho_create(const clotho_config* cfg, int device, clotho_ctx** out) {
     rc |= dalloc(c, &B.ages, P.s_cap);
     rc |= dalloc(c, &B.free_list, P.s_cap + 1);
     rc |= dalloc(c, &B.fixed_bits, P.w_cap);
-    rc |= dalloc(c, &B.sorted, (size_t)P.w_cap * 64);
-    rc |= dalloc(c, &B.pm, (size_t)P.w_cap * 65);
-    rc |= dalloc(c, &B.bucket, (size_t)P.w_cap * 64);
+    rc |= dalloc(c, &B.sorted, table_elems(P.w_cap));
+    rc |= dalloc(c, &B.pm, prefix_elems(P.w_cap));
+    rc |= dalloc(c, &B.bucket, table_elems(P.w_cap));
     rc |= dalloc(c, &B.digits, i8_packed_elems(std::max<uint32_t>(1, P.traits), P.w_cap));
     rc |= dalloc(c, &B.mut_count, h_cap);
     rc |= dalloc(c, &B.mut_local, h_cap);

@@ -568,7 +568,7 @@ struct ParentScatterOp {
 // Per-word order statistics, once per generation: the 64 position keys of
 // word w sorted ascending (sorted[w][k]), prefix masks pm[w][c] = bits of the
 // c smallest keys, and a 256-bucket table bucket[w][b] = #{k : key_k < b<<24}
-// (four buckets per dword).  For a gamete event e, c(e) = #{k : key_k < e}
+// (four buckets per dword), each stored tile-major (table_at, prefix_at).  For a gamete event e, c(e) = #{k : key_k < e}
 // is bucket[e>>24] plus the few keys of that bucket below e, and
 // {j in w : key_j >= e} = ~pm[w][c(e)]; the crossover mask of the word,
 // {j : odd #{events e <= key_j}} (block_crossover + region_classifier's
@@ -604,14 +604,14 @@ struct WordSort {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const uint32_t j = s_perm[wave][lane];  // slot bit of the lane-th smallest key
-        sorted[(size_t)w * 64 + lane] = s_key[wave][j];
+        sorted[table_at(w, lane)] = s_key[wave][j];
         uint64_t x = (uint64_t)1 << j;
         for (int d = 1; d < 64; d <<= 1) {
             const uint64_t y = __shfl_up(x, d, 64);
             if ((int)lane >= d) x |= y;
         }
-        pm[(size_t)w * 65 + lane + 1] = x;
-        if (lane == 0) pm[(size_t)w * 65] = 0;
+        pm[prefix_at(w, lane + 1)] = x;
+        if (lane == 0) pm[prefix_at(w, 0)] = 0;
         // buckets 4 lane .. 4 lane + 3: count of keys below b << 24
         uint32_t packed = 0;
         for (uint32_t q = 0; q < 4; ++q) {
@@ -620,7 +620,7 @@ struct WordSort {
             for (uint32_t k = 0; k < 64; ++k) cnt += s_key[wave][k] < bound ? 1u : 0u;
             packed |= cnt << (8 * q);  // cnt <= 64 fits a byte
         }
-        bucket[(size_t)w * 64 + lane] = packed;
+        bucket[table_at(w, lane)] = packed;
     }
 };
 
@@ -667,6 +667,58 @@ __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
+// A block's LDS copy of column tile bx's order-statistic tables (the
+// tile-major images of table_at / prefix_at): every 16-byte load is issued
+// before the first LDS write, so the fill costs one round trip, not one per
+// table row.  Columns at or past wc read as no keys: sorted all ones, bucket
+// and prefix masks zero.  fw < 64 (narrow rows, tile 0; a power of two of at
+// least 4): column l holds word l mod fw, the tile's first fw words repeated
+// across the 64 columns.
+__device__ __forceinline__ void load_word_tables(const Buffers& B, uint32_t bx, uint32_t wc,
+                                                 uint32_t fw, uint32_t* s_bucket,
+                                                 uint32_t* s_sorted, uint64_t* s_pm) {
+    static_assert(kRecBlock == 1024, "one 16-byte table piece per thread");
+    using i8::u32x4;
+    using i8::g_u32x4;
+    uint32_t t = threadIdx.x;
+    // opaque here, so the per-lane table offsets are computed per call: hoisted
+    // out of a persistent block's loop they outlive it and spill to scratch
+    asm volatile("" : "+v"(t));
+    const uint32_t w0 = bx * 64;
+    // sorted and bucket: 64 rows x 64 columns of dwords, four columns per thread
+    const uint32_t k = t >> 4, c = (t & 15u) * 4u, sc = c & (fw - 1);
+    const u32x4 vs = *(const g_u32x4*)(gbl(B.sorted) + table_at(w0 + sc, k));
+    const u32x4 vb = *(const g_u32x4*)(gbl(B.bucket) + table_at(w0 + sc, k));
+    // prefix masks: 65 rows x 64 columns of qwords, two columns per piece;
+    // pieces t, t + 1024 and (first 32 threads) t + 2048 of 2080
+    u32x4 vp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint32_t x = min(t + 1024u * i, 65u * 32u - 1u);  // in bounds; the store is guarded
+        vp[i] = *(const g_u32x4*)(gbl(B.pm) + prefix_at(w0 + (((x & 31u) * 2u) & (fw - 1)), x >> 5));
+    }
+    u32x4 ms, mb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool ok = w0 + sc + e < wc;
+        ms[e] = ok ? vs[e] : 0xFFFFFFFFu;
+        mb[e] = ok ? vb[e] : 0u;
+    }
+    *(u32x4*)(s_sorted + k * 64 + c) = ms;
+    *(u32x4*)(s_bucket + k * 64 + c) = mb;
+    if (t < 64) *(u32x4*)(s_sorted + 64 * 64 + t * 4) = u32x4{~0u, ~0u, ~0u, ~0u};  // rows 64..67
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i == 2 && t >= 32) break;
+        const uint32_t x = t + 1024u * i, col = (x & 31u) * 2u;
+        const uint32_t sw = w0 + (col & (fw - 1));
+        u32x4 v = vp[i];
+        if (sw >= wc) v.x = v.y = 0u;
+        if (sw + 1 >= wc) v.z = v.w = 0u;
+        *(u32x4*)(s_pm + (x >> 5) * 64 + col) = v;
+    }
+}
+
 // Memory operations of the row loop are issued unconditionally (indices
 // clamped in bounds, the child store through a range-checked buffer resource
 // that drops the lanes past the row), so the number of vector memory
@@ -677,24 +729,27 @@ __device__ __forceinline__ uint32_t lane_read(uint32_t v, uint32_t l) {
 // a replicate of 8-word rows keeps every lane busy instead of one in eight.
 // The LDS tables hold the tile's FW words repeated across the 64 columns, so
 // lane l still reads its own bank column.  For events j FW .. j FW + FW - 1
-// each lane holds key j FW + (l mod FW) of its row (the first FW keys are
+// each lane holds key j FW + (l mod FW) of its row (the first 2 FW keys are
 // loaded with the parent words, later ones only when some row of the step
 // has that many events) and takes key x of its row from lane
 // f FW + x mod FW; rows of a step share the event loop up to the largest
 // event count, lookups past a row's own count take rank 64 (pm[64] = all
 // ones), and since every lane does the same even number of lookups the XOR
 // of the prefix masks is still the XOR of the complements of the real events.
-template <uint32_t FW>
+// One code path serves FW = 8, 16 and 32 (lg = log2 FW): a persistent
+// batched block holds every path of the kernel in one register allocation of
+// at most 64 registers, and one narrow path keeps it free of spills.
 __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers& B, uint32_t sel,
-                                                 uint32_t by, uint32_t* s_bucket,
+                                                 uint32_t by, uint32_t lg, uint32_t* s_bucket,
                                                  uint32_t* s_sorted, uint64_t* s_pm) {
-    constexpr uint32_t NF = 64 / FW;   // rows per wave step
+    const uint32_t FW = 1u << lg;     // words per row slot
+    const uint32_t NF = 64u >> lg;    // rows per wave step
+    // event-key registers loaded with the parent words: keys 0 .. 2 FW - 1
+    // (more than 8 crossover events per gamete is common at rho = 5)
+    constexpr uint32_t NK = 2;
     const auto st = gbl(B.state);
     const auto parent = gbl(B.rows[sel]);
     const auto child = gbl(B.rows[1 - sel]);
-    const auto sorted = gbl(B.sorted);
-    const auto pm = gbl(B.pm);
-    const auto bucket = gbl(B.bucket);
     const auto free_list = gbl(B.free_list);
     const auto fixed_bits = gbl(B.fixed_bits);
     const auto top_in = gbl(B.top);
@@ -707,19 +762,18 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
     const uint32_t A = uniform_read(&st->alleles), F = uniform_read(&st->free_count),
                    S = uniform_read(&st->all_size), fb = uniform_read(&st->free_base);
     const uint32_t ev_cap = P.ev_cap;  // <= 64 on this path
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t f = lane / FW, w = lane % FW;
+    // opaque thread index: in a persistent block's loop the lane-derived
+    // values of each row width would otherwise be hoisted and held throughout
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t f = lane >> lg, w = lane & (FW - 1);
     const bool valid = w < wc;
     const bool inpar = valid && w < wp;
     const uint32_t col = wp ? min(w, wp - 1) : 0;
 
-    for (uint32_t k = wave; k < 68; k += kRecWaves)
-        s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
-    for (uint32_t q = wave; q < 64; q += kRecWaves)
-        s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
-    for (uint32_t c = wave; c < 65; c += kRecWaves)
-        s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
+    load_word_tables(B, 0, wc, FW, s_bucket, s_sorted, s_pm);
     const uint64_t keep = inpar ? ~fixed_bits[w] : 0;
     __syncthreads();
 
@@ -729,30 +783,48 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
     const uint32_t nsteps = (c1 - c0 + NF - 1) / NF;
     uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-    auto wave_max = [](uint32_t v) {
-#pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) v = max(v, (uint32_t)__shfl_xor((int)v, (int)d, 64));
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+    // A step: the gamete records of its rows (event and mutation counts, top
+    // row; the top row last, so the wait for it covers the record), then the
+    // parent words and the first 2 FW event keys of each row.  (A two-deep
+    // software pipeline over steps measured no faster and, in the batched
+    // kernel's 64 registers, spilled.)
+    struct Rec {
+        uint32_t nev, cnt, top;
     };
-
-    for (uint32_t step = wave; step < nsteps; step += kRecWaves) {
+    struct Par {
+        uint64_t t, b;
+        uint32_t kv[NK];  // event keys x FW + w of the lane's row
+    };
+    auto clamp_row = [&](uint32_t step) {
         const uint32_t row = c0 + step * NF + f;
-        const bool live = row < c1;
-        const uint32_t r = live ? row : c0;  // in bounds (c0 < c1 here)
-        const uint32_t top = top_in[r];
-        const uint32_t ne = live ? nev_in[r] : 0u;
-        const uint32_t nm = live ? count[r] : 0u;
-        const uint32_t k0 = local[r] + boff[r / kBlock];
-        const uint64_t t = parent[(size_t)top * wp + col];
-        const uint64_t b = parent[(size_t)(top ^ 1u) * wp + col];
-        const auto ev = events + (size_t)r * ev_cap;
-        uint32_t kv = ev[min(w, ev_cap - 1)];
-
-        const uint32_t max_ne = wave_max(ne);
-        uint64_t m = 0;
-        for (uint32_t j = 0; j * FW < max_ne; ++j) {
-            if (j) kv = ev[min(j * FW + w, ev_cap - 1)];
-            for (uint32_t x = 0; x < FW && j * FW + x < max_ne; x += kEventGroup) {
+        return row < c1 ? row : c0;  // in bounds: the loop runs only when c0 < c1
+    };
+    auto load_rec = [&](uint32_t step, Rec& q) {
+        const uint32_t r = clamp_row(step);
+        q.nev = nev_in[r];
+        q.cnt = count[r];
+        q.top = top_in[r];
+    };
+    auto load_par = [&](uint32_t step, const Rec& q, Par& pr) {
+        // opaque column indices: the addresses are formed here, per call,
+        // rather than hoisted and held as 64-bit pointers across the loop
+        uint32_t cl = col, wl = w;
+        asm volatile("" : "+v"(cl), "+v"(wl));
+        pr.t = parent[(size_t)q.top * wp + cl];
+        pr.b = parent[(size_t)(q.top ^ 1u) * wp + cl];
+        const auto ev = events + (size_t)clamp_row(step) * ev_cap;
+#pragma unroll
+        for (uint32_t x = 0; x < NK; ++x) pr.kv[x] = ev[min(x * FW + wl, ev_cap - 1)];
+    };
+    auto process = [&](uint32_t step, const Rec& q, const Par& pr) {
+        const uint32_t rbase = c0 + step * NF;  // first row of the step (< c1)
+        const bool live = rbase + f < c1;
+        const uint32_t ne = live ? q.nev : 0u;
+        const uint32_t nm = live ? q.cnt : 0u;
+        // loop bounds: while some row of the step has events (mutations) left.
+        // kv holds events j FW .. j FW + FW - 1 of the lane's row.
+        auto classify = [&](uint32_t kv, uint32_t j, uint64_t& m) {
+            for (uint32_t x = 0; x < FW && __ballot(j * FW + x < ne); x += kEventGroup) {
                 uint32_t key[kEventGroup], p[kEventGroup];
                 bool more[kEventGroup];
 #pragma unroll
@@ -766,7 +838,7 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) sk[i][q] = s_sorted[(p[i] + q) * 64 + lane];
+                    for (int u = 0; u < 3; ++u) sk[i][u] = s_sorted[(p[i] + u) * 64 + lane];
 #pragma unroll
                 for (int i = 0; i < kEventGroup; ++i) {
                     const bool real = j * FW + x + i < ne;
@@ -783,24 +855,50 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
                 for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
                 m ^= xm;
             }
-        }
-        uint64_t c = ((t & ~m) | (b & m)) & keep;
-        const uint32_t max_nm = wave_max(nm);
-        for (uint32_t k = 0; k < max_nm; ++k) {
+        };
+        uint64_t m = 0;
+        // the first NK FW events: keys already in registers, no load here (a
+        // key that might come from a load issued in this loop would make every
+        // use wait for all loads in flight, the prefetched ones included)
+#pragma unroll
+        for (uint32_t j = 0; j < NK; ++j)
+            if (__ballot(j * FW < ne)) classify(pr.kv[j], j, m);
+        // further events (rare): keys loaded here
+        for (uint32_t j = NK; __ballot(j * FW < ne); ++j)
+            classify(events[(size_t)clamp_row(step) * ev_cap + min(j * FW + w, ev_cap - 1)], j, m);
+        uint64_t c = ((pr.t & ~m) | (pr.b & m)) & keep;
+        // new mutations (rare at these rates: their offsets load on demand)
+        const bool any_nm = __ballot(nm != 0) != 0;
+        const uint32_t k0 = any_nm ? local[clamp_row(step)] + boff[clamp_row(step) / kBlock] : 0u;
+        for (uint32_t k = 0; __ballot(k < nm); ++k) {
             if (k < nm) {
                 const uint32_t kk = k0 + k;
                 const uint32_t slot = kk < F ? free_list[fb + kk] : A + (kk - F);
                 if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
             }
         }
-        if (live && valid) child[(size_t)row * wc + w] = c;
-        if (live) {
-            acc_or |= c;
-            acc_and &= c;
-        }
+        // the step's rows through a range-checked buffer resource: lanes of
+        // rows past the chunk or of words past the row store out of range and
+        // are dropped, so the store is issued on every path
+        const uint32_t nrows = min(NF, c1 - rbase);
+        const __amdgpu_buffer_rsrc_t out = __builtin_amdgcn_make_buffer_rsrc(
+            child + (size_t)rbase * wc, 0, (int)(nrows * wc * 8u), kBufferWord3);
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b64(v, out, (live && valid) ? (f * wc + w) * 8u : 0x80000000u,
+                                              0, kCachePolicyNT);
+        acc_or |= live ? c : 0;
+        acc_and &= live ? c : ~(uint64_t)0;
+    };
+
+    for (uint32_t step = wave; step < nsteps; step += kRecWaves) {
+        Rec q;
+        Par pr;
+        load_rec(step, q);
+        load_par(step, q, pr);
+        process(step, q, pr);
     }
     // rows of one word live in lanes w, w + FW, ...: fold them, then the waves
-#pragma unroll
     for (uint32_t d = FW; d < 64; d <<= 1) {
         acc_or |= (uint64_t)__shfl_xor((long long)acc_or, (int)d, 64);
         acc_and &= (uint64_t)__shfl_xor((long long)acc_and, (int)d, 64);
@@ -832,9 +930,6 @@ struct Recombine {
         const auto st = gbl(B.state);
         const auto parent = gbl(B.rows[sel]);
         const auto child = gbl(B.rows[1 - sel]);
-        const auto sorted = gbl(B.sorted);
-        const auto pm = gbl(B.pm);
-        const auto bucket = gbl(B.bucket);
         const auto free_list = gbl(B.free_list);
         const auto fixed_bits = gbl(B.fixed_bits);
         const auto top_in = gbl(B.top);
@@ -848,18 +943,18 @@ struct Recombine {
         const auto by_parent = (const __attribute__((address_space(1))) i8::u32x4*)B.by_parent;
         const bool ordered = B.by_parent != nullptr;
         const uint32_t bx = blk % P.tiles, by = blk / P.tiles;
-        __shared__ uint32_t s_bucket[64 * 64];
-        __shared__ uint32_t s_sorted[68 * 64];
-        __shared__ uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
+        __shared__ __align__(16) uint32_t s_bucket[64 * 64];
+        __shared__ __align__(16) uint32_t s_sorted[68 * 64];
+        __shared__ __align__(16) uint64_t s_pm[65 * 64];  // reused for the OR/AND reduction at the end
 
         const uint32_t wc = uniform_read(&st->wc);
         const uint32_t w0 = bx * 64;
         if (w0 >= wc) return;  // uniform for the block
         if constexpr (kSibs == 1 && !kWideEvents) {
             // narrow rows: several rows per wave (tile 0 is the only live tile)
-            if (wc <= 8) return recombine_folded<8>(P, B, sel, by, s_bucket, s_sorted, s_pm);
-            if (wc <= 16) return recombine_folded<16>(P, B, sel, by, s_bucket, s_sorted, s_pm);
-            if (wc <= 32) return recombine_folded<32>(P, B, sel, by, s_bucket, s_sorted, s_pm);
+            if (wc <= 32)
+                return recombine_folded(P, B, sel, by, wc <= 8 ? 3u : wc <= 16 ? 4u : 5u, s_bucket,
+                                        s_sorted, s_pm);
         }
         const uint32_t wp = uniform_read(&st->wp);
         const uint32_t lane = threadIdx.x & 63;
@@ -872,12 +967,7 @@ struct Recombine {
                        S = uniform_read(&st->all_size), fb = uniform_read(&st->free_base);
         const uint32_t ev_cap = P.ev_cap;  // >= 4, multiple of 4, <= 128
 
-        for (uint32_t k = wave; k < 68; k += kRecWaves)
-            s_sorted[k * 64 + lane] = (k < 64 && valid) ? sorted[(size_t)w * 64 + k] : 0xFFFFFFFFu;
-        for (uint32_t q = wave; q < 64; q += kRecWaves)
-            s_bucket[q * 64 + lane] = valid ? bucket[(size_t)w * 64 + q] : 0u;
-        for (uint32_t c = wave; c < 65; c += kRecWaves)
-            s_pm[c * 64 + lane] = valid ? pm[(size_t)w * 65 + c] : 0;
+        load_word_tables(B, bx, wc, 64u, s_bucket, s_sorted, s_pm);
         const uint64_t keep = inpar ? ~fixed_bits[w] : 0;  // also clears columns >= wp
         __syncthreads();
 
@@ -2118,6 +2208,63 @@ CLOTHO_SINGLE(k_compact_fitness, CompactFitness)
 // enough parent loads in flight.
 constexpr uint32_t kBatchOffsLds = 1024;  // replicate offsets kept in LDS up to this R
 
+// Ops whose batched offsets count live column tiles only (k_batch_live_offsets):
+// the local index u of a replicate maps to tile u mod lt, row chunk u / lt,
+// lt = ceil(wc / 64), i.e. to Op block (u / lt) tiles + u mod lt.
+template <class Op>
+struct LiveTiles {
+    static constexpr bool value = false;
+};
+template <bool W, int K>
+struct LiveTiles<Recombine<W, K>> {
+    static constexpr bool value = true;
+};
+
+// Live virtual blocks of the batched recombination.  A replicate's blocks at
+// capacity cover every column tile of its allele capacity, but its child rows
+// are wc words wide this generation, so only ceil(wc / 64) tiles hold words
+// (a block of a tile past wc does nothing).  One block scans live tiles x row
+// chunks over the replicates into out[0 .. R] (exclusive, out[R] = total), so
+// the persistent blocks of the recombination visit live tiles only.
+__global__ void __launch_bounds__(1024) k_batch_live_offsets(const RepView* __restrict__ views,
+                                                             uint32_t R, uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < R; base += 1024) {
+        const uint32_t r = base + tid;
+        uint32_t n = 0;
+        if (r < R) {
+            const RepView& rv = views[r];
+            const uint32_t wc = gbl(rv.B.state)->wc;
+            n = min((wc + 63) / 64, rv.P.tiles) * rv.P.chunks;
+        }
+        uint32_t x = n;  // inclusive scan within the wave
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) s_wave[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t t = lane < 16 ? s_wave[lane] : 0u;
+            for (uint32_t d = 1; d < 16; d <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)t, d, 64);
+                if (lane >= d) t += y;
+            }
+            if (lane < 16) s_wave[lane] = t;  // inclusive wave totals
+        }
+        __syncthreads();
+        if (r < R) out[r] = s_carry + (wave ? s_wave[wave - 1] : 0u) + x - n;
+        __syncthreads();
+        if (tid == 0) s_carry += s_wave[15];
+        __syncthreads();
+    }
+    if (tid == 0) out[R] = s_carry;
+}
+
 template <class Op>
 __global__ void __launch_bounds__(Op::kThreads)
     __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value))) k_batched(const RepView* __restrict__ views,
@@ -2143,7 +2290,12 @@ __global__ void __launch_bounds__(Op::kThreads)
         }
         lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: descriptor reads stay scalar
         const RepView& rv = views[lo];
-        Op::run(rv.P, rv.B, sel, v - offs[lo]);
+        uint32_t blk = v - (in_lds ? s_offs[lo] : offs[lo]);
+        if constexpr (LiveTiles<Op>::value) {
+            const uint32_t lt = (uniform_read(&gbl(rv.B.state)->wc) + 63) / 64;  // >= 1 here
+            blk = (blk / lt) * rv.P.tiles + blk % lt;
+        }
+        Op::run(rv.P, rv.B, sel, blk);
         __syncthreads();
     }
 }
@@ -2315,7 +2467,7 @@ namespace {
 
 template <class Op>
 hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_t s) {
-    const uint32_t total = L.total[op];
+    const uint32_t total = L.total[op];  // at capacity: the most live blocks
     if (!total) return hipSuccess;
     const uint32_t per_cu = std::max<uint32_t>(
         1u, (Op::kThreads >= 1024 ? 2u : 8u) * L.grid_scale_num / L.grid_scale_den);
@@ -2325,9 +2477,18 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
     // the grid while the rest only meet tiles past the live width.
     uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
     if (grid > 1 && !(grid & 1u)) --grid;
-    hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::kThreads), 0, s, L.views,
-                       L.offs + (size_t)op * (L.R + 1), L.R, sel);
+    const uint32_t* offs = LiveTiles<Op>::value ? L.live_offs : L.offs + (size_t)op * (L.R + 1);
+    hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::kThreads), 0, s, L.views, offs, L.R, sel);
     return hipGetLastError();
+}
+
+// The recombination's live-tile offsets of this generation (child widths are
+// set by the draws), then the recombination.
+hipError_t launch_batched_recombine(const BatchLaunch& L, uint32_t par, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_live_offsets, dim3(1), dim3(1024), 0, s, L.views, L.R, L.live_offs);
+    TRY_LAUNCH(hipGetLastError());
+    if (L.wide_events) return launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s);
+    return launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s);
 }
 
 }  // namespace
@@ -2335,10 +2496,7 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
 // The batch's recombination and traits over the last generation's gametes,
 // parents [par], children into [1 - par] (the autotuner's timed replay).
 hipError_t launch_batch_replay(const BatchLaunch& L, uint32_t par, hipStream_t s) {
-    if (L.wide_events)
-        TRY_LAUNCH(launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s));
-    else
-        TRY_LAUNCH(launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s));
+    TRY_LAUNCH(launch_batched_recombine(L, par, s));
     if (!L.traits) return hipSuccess;
     const uint32_t chi = 1 - par;
     TRY_LAUNCH(launch_batched<PackI8>(L, kOpPackI8, chi, s));
@@ -2362,10 +2520,7 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
     TRY_LAUNCH(launch_batched<Gametes>(L, kOpGametes, par, s));
     TRY_LAUNCH(mark(1));
     TRY_LAUNCH(launch_batched<WordSort>(L, kOpWordSort, par, s));
-    if (L.wide_events)
-        TRY_LAUNCH(launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s));
-    else
-        TRY_LAUNCH(launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s));
+    TRY_LAUNCH(launch_batched_recombine(L, par, s));
     TRY_LAUNCH(mark(2));
     if (L.traits) {
         const uint32_t chi = 1 - par;

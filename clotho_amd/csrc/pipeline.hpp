@@ -43,6 +43,22 @@ inline uint32_t stat_chunks(uint32_t words, uint32_t n) {
     return chunks < most ? chunks : most;
 }
 
+// The per-word order-statistic tables of the recombination (sorted keys,
+// bucket counts, prefix masks) are stored tile-major: the table of a 64-word
+// column tile is one contiguous [row][64 words] image, the layout a
+// recombination block keeps in LDS, so a block fills its LDS copy with
+// coalesced reads (a word-major table puts each lane of a read on its own
+// cache line).  Entry k of word w, and prefix mask c of word w:
+__host__ __device__ inline size_t table_at(uint32_t w, uint32_t k) {
+    return (size_t)(w >> 6) * (64u * 64u) + k * 64u + (w & 63u);
+}
+__host__ __device__ inline size_t prefix_at(uint32_t w, uint32_t c) {
+    return (size_t)(w >> 6) * (65u * 64u) + c * 64u + (w & 63u);
+}
+// elements of the sorted / bucket tables and of the prefix-mask table
+inline size_t table_elems(uint32_t w_cap) { return (size_t)((w_cap + 63) / 64) * 64u * 64u; }
+inline size_t prefix_elems(uint32_t w_cap) { return (size_t)((w_cap + 63) / 64) * 65u * 64u; }
+
 // Scalars of the generation, kept in device memory so that a captured
 // hipGraph replays successive generations without host involvement.
 struct GenState {
@@ -106,9 +122,10 @@ struct Buffers {
     uint32_t* free_list;      // [s_cap] fixed ascending then lost ascending; entries
                               // from free_base on are unused
     uint64_t* fixed_bits;     // [w_cap] AND of all rows, idx < all_size
-    uint32_t* sorted;         // [w_cap][64] position keys of each word, ascending
-    uint64_t* pm;             // [w_cap][65] bits of the c smallest keys of each word
-    uint32_t* bucket;         // [w_cap][64] per word: #{keys < b<<24}, four b per dword
+    // per-word order statistics in tile-major order (table_at / prefix_at)
+    uint32_t* sorted;         // [tiles][64 k][64 w] position keys of each word, ascending
+    uint64_t* pm;             // [tiles][65 c][64 w] bits of the c smallest keys of each word
+    uint32_t* bucket;         // [tiles][64 q][64 w] per word: #{keys < b<<24}, four b per dword
     uint4* digits;            // [ceil(T/4)][w_cap][64] int8 digit planes of the effects
     uint32_t* mut_count;      // [h]
     uint32_t* mut_local;      // [h] exclusive scan inside a block
@@ -169,6 +186,7 @@ enum BatchOp : int {
 struct BatchLaunch {
     const RepView* views = nullptr;  // device [R]
     const uint32_t* offs = nullptr;  // device [kOpCount][R + 1]
+    uint32_t* live_offs = nullptr;   // device [R + 1]: the recombination's live blocks
     uint32_t R = 0;
     uint32_t total[kOpCount] = {};   // offs[k][R]
     uint32_t traits = 0;             // trait count shared by every replicate
