@@ -780,123 +780,135 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
     const uint32_t rows_per_chunk = (P.h + P.chunks - 1) / P.chunks;
     const uint32_t c0 = min(P.h, by * rows_per_chunk);
     const uint32_t c1 = min(P.h, c0 + rows_per_chunk);
-    const uint32_t nsteps = (c1 - c0 + NF - 1) / NF;
     uint64_t acc_or = 0, acc_and = ~(uint64_t)0;
 
-    // A step: the gamete records of its rows (event and mutation counts, top
-    // row; the top row last, so the wait for it covers the record), then the
-    // parent words and the first 2 FW event keys of each row.  (A two-deep
-    // software pipeline over steps measured no faster and, in the batched
-    // kernel's 64 registers, spilled.)
-    struct Rec {
-        uint32_t nev, cnt, top;
-    };
+    // Each wave owns a contiguous segment of the chunk's rows (a multiple of
+    // NF rows).  The gamete records of 64 rows (event and mutation counts,
+    // top row) are loaded with one vector load, lane q holding row q of the
+    // batch; a step of NF rows reads its rows' records from those registers
+    // with shuffles, so only the parent words and event keys of a step are
+    // loads of their own.  Steps alternate between two register sets: the
+    // loads of step k + 2 are issued as soon as step k is done and stay in
+    // flight while step k + 1 is classified.
+    const uint32_t seg = ((c1 - c0 + kRecWaves - 1) / kRecWaves + NF - 1) & ~(NF - 1);
+    const uint32_t s0 = min(c1, c0 + wave * seg);
+    const uint32_t nrow = min(c1, s0 + seg) - s0;  // rows of this wave
     struct Par {
         uint64_t t, b;
         uint32_t kv[NK];  // event keys x FW + w of the lane's row
     };
-    auto clamp_row = [&](uint32_t step) {
-        const uint32_t row = c0 + step * NF + f;
-        return row < c1 ? row : c0;  // in bounds: the loop runs only when c0 < c1
-    };
-    auto load_rec = [&](uint32_t step, Rec& q) {
-        const uint32_t r = clamp_row(step);
-        q.nev = nev_in[r];
-        q.cnt = count[r];
-        q.top = top_in[r];
-    };
-    auto load_par = [&](uint32_t step, const Rec& q, Par& pr) {
-        // opaque column indices: the addresses are formed here, per call,
-        // rather than hoisted and held as 64-bit pointers across the loop
-        uint32_t cl = col, wl = w;
-        asm volatile("" : "+v"(cl), "+v"(wl));
-        pr.t = parent[(size_t)q.top * wp + cl];
-        pr.b = parent[(size_t)(q.top ^ 1u) * wp + cl];
-        const auto ev = events + (size_t)clamp_row(step) * ev_cap;
-#pragma unroll
-        for (uint32_t x = 0; x < NK; ++x) pr.kv[x] = ev[min(x * FW + wl, ev_cap - 1)];
-    };
-    auto process = [&](uint32_t step, const Rec& q, const Par& pr) {
-        const uint32_t rbase = c0 + step * NF;  // first row of the step (< c1)
-        const bool live = rbase + f < c1;
-        const uint32_t ne = live ? q.nev : 0u;
-        const uint32_t nm = live ? q.cnt : 0u;
-        // loop bounds: while some row of the step has events (mutations) left.
-        // kv holds events j FW .. j FW + FW - 1 of the lane's row.
-        auto classify = [&](uint32_t kv, uint32_t j, uint64_t& m) {
-            for (uint32_t x = 0; x < FW && __ballot(j * FW + x < ne); x += kEventGroup) {
-                uint32_t key[kEventGroup], p[kEventGroup];
-                bool more[kEventGroup];
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i)
-                    key[i] = (uint32_t)__shfl((int)kv, (int)(f * FW + ((x + i) & (FW - 1))), 64);
-                const uint8_t* sb = reinterpret_cast<const uint8_t*>(s_bucket);
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i)
-                    p[i] = sb[lane * 4u + (((key[i] >> 26) << 8) | ((key[i] >> 24) & 3u))];
-                uint32_t sk[kEventGroup][3];
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i)
-#pragma unroll
-                    for (int u = 0; u < 3; ++u) sk[i][u] = s_sorted[(p[i] + u) * 64 + lane];
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i) {
-                    const bool real = j * FW + x + i < ne;
-                    const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u);
-                    more[i] = real && sk[i][2] < key[i];
-                    p[i] = real ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
-                }
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i)
-                    if (more[i])
-                        while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
-                uint64_t xm = 0;
-#pragma unroll
-                for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
-                m ^= xm;
-            }
-        };
-        uint64_t m = 0;
-        // the first NK FW events: keys already in registers, no load here (a
-        // key that might come from a load issued in this loop would make every
-        // use wait for all loads in flight, the prefetched ones included)
-#pragma unroll
-        for (uint32_t j = 0; j < NK; ++j)
-            if (__ballot(j * FW < ne)) classify(pr.kv[j], j, m);
-        // further events (rare): keys loaded here
-        for (uint32_t j = NK; __ballot(j * FW < ne); ++j)
-            classify(events[(size_t)clamp_row(step) * ev_cap + min(j * FW + w, ev_cap - 1)], j, m);
-        uint64_t c = ((pr.t & ~m) | (pr.b & m)) & keep;
-        // new mutations (rare at these rates: their offsets load on demand)
-        const bool any_nm = __ballot(nm != 0) != 0;
-        const uint32_t k0 = any_nm ? local[clamp_row(step)] + boff[clamp_row(step) / kBlock] : 0u;
-        for (uint32_t k = 0; __ballot(k < nm); ++k) {
-            if (k < nm) {
-                const uint32_t kk = k0 + k;
-                const uint32_t slot = kk < F ? free_list[fb + kk] : A + (kk - F);
-                if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
-            }
-        }
-        // the step's rows through a range-checked buffer resource: lanes of
-        // rows past the chunk or of words past the row store out of range and
-        // are dropped, so the store is issued on every path
-        const uint32_t nrows = min(NF, c1 - rbase);
-        const __amdgpu_buffer_rsrc_t out = __builtin_amdgcn_make_buffer_rsrc(
-            child + (size_t)rbase * wc, 0, (int)(nrows * wc * 8u), kBufferWord3);
-        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-        const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, out, (live && valid) ? (f * wc + w) * 8u : 0x80000000u,
-                                              0, kCachePolicyNT);
-        acc_or |= live ? c : 0;
-        acc_and &= live ? c : ~(uint64_t)0;
-    };
 
-    for (uint32_t step = wave; step < nsteps; step += kRecWaves) {
-        Rec q;
-        Par pr;
-        load_rec(step, q);
-        load_par(step, q, pr);
-        process(step, q, pr);
+    for (uint32_t j0 = 0; j0 < nrow; j0 += 64) {
+        const uint32_t nb = min(64u, nrow - j0);  // rows of this record batch
+        const uint32_t nst = (nb + NF - 1) >> (6 - lg);  // its steps
+        const uint32_t rr = s0 + j0 + min(lane, nb - 1);  // in bounds
+        const uint32_t g_nev = nev_in[rr], g_cnt = count[rr], g_top = top_in[rr];
+        // record lane of the lane's row in step k (clamped into the batch)
+        auto rec_lane = [&](uint32_t k) { return min(k * NF + f, nb - 1); };
+        auto load_step = [&](uint32_t k, Par& pr) {
+            const uint32_t q = rec_lane(min(k, nst - 1));
+            const uint32_t top = (uint32_t)__shfl((int)g_top, (int)q, 64);
+            // opaque column indices: the addresses are formed here, per call,
+            // rather than hoisted and held as 64-bit pointers across the loop
+            uint32_t cl = col, wl = w;
+            asm volatile("" : "+v"(cl), "+v"(wl));
+            pr.t = parent[(size_t)top * wp + cl];
+            pr.b = parent[(size_t)(top ^ 1u) * wp + cl];
+            const auto ev = events + (size_t)(s0 + j0 + q) * ev_cap;
+#pragma unroll
+            for (uint32_t x = 0; x < NK; ++x) pr.kv[x] = ev[min(x * FW + wl, ev_cap - 1)];
+        };
+        auto process = [&](uint32_t k, const Par& pr) {
+            const uint32_t q = rec_lane(k);
+            const bool live = k * NF + f < nb;
+            // shuffles with every lane active: a lane reading a record must
+            // not depend on the source lane being live itself
+            const uint32_t nev_q = (uint32_t)__shfl((int)g_nev, (int)q, 64);
+            const uint32_t cnt_q = (uint32_t)__shfl((int)g_cnt, (int)q, 64);
+            const uint32_t ne = live ? nev_q : 0u;
+            const uint32_t nm = live ? cnt_q : 0u;
+            const uint32_t row = s0 + j0 + q;
+            // loop bounds: while some row of the step has events (mutations)
+            // left.  kv holds events j FW .. j FW + FW - 1 of the lane's row.
+            auto classify = [&](uint32_t kv, uint32_t j, uint64_t& m) {
+                for (uint32_t x = 0; x < FW && __ballot(j * FW + x < ne); x += kEventGroup) {
+                    uint32_t key[kEventGroup], p[kEventGroup];
+                    bool more[kEventGroup];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i)
+                        key[i] = (uint32_t)__shfl((int)kv, (int)(f * FW + ((x + i) & (FW - 1))), 64);
+                    const uint8_t* sb = reinterpret_cast<const uint8_t*>(s_bucket);
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i)
+                        p[i] = sb[lane * 4u + (((key[i] >> 26) << 8) | ((key[i] >> 24) & 3u))];
+                    uint32_t sk[kEventGroup][3];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i)
+#pragma unroll
+                        for (int u = 0; u < 3; ++u) sk[i][u] = s_sorted[(p[i] + u) * 64 + lane];
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) {
+                        const bool real = j * FW + x + i < ne;
+                        const uint32_t c = (sk[i][0] < key[i] ? 1u : 0u) + (sk[i][1] < key[i] ? 1u : 0u);
+                        more[i] = real && sk[i][2] < key[i];
+                        p[i] = real ? p[i] + c + (more[i] ? 1u : 0u) : 64u;
+                    }
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i)
+                        if (more[i])
+                            while (p[i] < 64 && s_sorted[p[i] * 64 + lane] < key[i]) ++p[i];
+                    uint64_t xm = 0;
+#pragma unroll
+                    for (int i = 0; i < kEventGroup; ++i) xm ^= s_pm[p[i] * 64 + lane];
+                    m ^= xm;
+                }
+            };
+            uint64_t m = 0;
+            // the first NK FW events: keys already in registers, no load here
+            // (a key that might come from a load issued in this loop would make
+            // every use wait for all loads in flight, the prefetched ones included)
+#pragma unroll
+            for (uint32_t j = 0; j < NK; ++j)
+                if (__ballot(j * FW < ne)) classify(pr.kv[j], j, m);
+            // further events (rare): keys loaded here
+            for (uint32_t j = NK; __ballot(j * FW < ne); ++j)
+                classify(events[(size_t)row * ev_cap + min(j * FW + w, ev_cap - 1)], j, m);
+            uint64_t c = ((pr.t & ~m) | (pr.b & m)) & keep;
+            // new mutations (rare at these rates: their offsets load on demand)
+            const bool any_nm = __ballot(nm != 0) != 0;
+            const uint32_t k0 = any_nm ? local[row] + boff[row / kBlock] : 0u;
+            for (uint32_t k = 0; __ballot(k < nm); ++k) {
+                if (k < nm) {
+                    const uint32_t kk = k0 + k;
+                    const uint32_t slot = kk < F ? free_list[fb + kk] : A + (kk - F);
+                    if (slot < S && (slot >> 6) == w) c ^= (uint64_t)1 << (slot & 63);
+                }
+            }
+            // the step's rows through a range-checked buffer resource: lanes of
+            // rows past the batch or of words past the row store out of range
+            // and are dropped, so the store is issued on every path
+            const uint32_t rbase = s0 + j0 + k * NF;  // first row of the step
+            const uint32_t nrows = min(NF, nb - k * NF);
+            const __amdgpu_buffer_rsrc_t out = __builtin_amdgcn_make_buffer_rsrc(
+                child + (size_t)rbase * wc, 0, (int)(nrows * wc * 8u), kBufferWord3);
+            typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+            const u32x2 v = {(uint32_t)c, (uint32_t)(c >> 32)};
+            __builtin_amdgcn_raw_buffer_store_b64(v, out, (live && valid) ? (f * wc + w) * 8u : 0x80000000u,
+                                                  0, kCachePolicyNT);
+            acc_or |= live ? c : 0;
+            acc_and &= live ? c : ~(uint64_t)0;
+        };
+
+        Par pa, pb;
+        load_step(0, pa);
+        load_step(1, pb);
+        for (uint32_t k = 0; k < nst; k += 2) {
+            process(k, pa);
+            load_step(k + 2, pa);
+            if (k + 1 >= nst) break;
+            process(k + 1, pb);
+            load_step(k + 3, pb);
+        }
     }
     // rows of one word live in lanes w, w + FW, ...: fold them, then the waves
     for (uint32_t d = FW; d < 64; d <<= 1) {
