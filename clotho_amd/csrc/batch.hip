@@ -63,6 +63,10 @@ int sync_views(clotho_batch* b) {
         v[r].P = b->reps[r]->p;
         v[r].B = b->reps[r]->b;
         v[r].B.by_parent = nullptr;  // replicates recombine in child order
+        // batched recombination blocks take kBatchChunkDiv times the rows of
+        // a replicate's own launch: the replicates supply the parallelism,
+        // and every block pays a fixed set-up (offset search, tables, partials)
+        v[r].P.chunks = std::max<uint32_t>(1, (v[r].P.chunks + kBatchChunkDiv - 1) / kBatchChunkDiv);
     }
     if (b->d_views && b->host_views.size() == R &&
         std::memcmp(v.data(), b->host_views.data(), sizeof(RepView) * R) == 0)

@@ -2481,6 +2481,9 @@ template <class Op>
 hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_t s) {
     const uint32_t total = L.total[op];  // at capacity: the most live blocks
     if (!total) return hipSuccess;
+    // blocks per compute unit; a grid past residency (the 256-thread traits
+    // blocks hold four per CU) starts its last blocks late, which measured
+    // faster than a resident grid: they take the virtual blocks left over
     const uint32_t per_cu = std::max<uint32_t>(
         1u, (Op::kThreads >= 1024 ? 2u : 8u) * L.grid_scale_num / L.grid_scale_den);
     // An odd grid: virtual block v runs on block v mod grid, and replicate
