@@ -2538,11 +2538,22 @@ template <class Op>
 hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_t s) {
     const uint32_t total = L.total[op];  // at capacity: the most live blocks
     if (!total) return hipSuccess;
-    // blocks per compute unit; a grid past residency (the 256-thread traits
-    // blocks hold four per CU) starts its last blocks late, which measured
-    // faster than a resident grid: they take the virtual blocks left over
-    const uint32_t per_cu = std::max<uint32_t>(
+    // blocks per compute unit.  A grid past residency starts its last blocks
+    // late: for the traits blocks (four resident per CU) that measured faster
+    // than a resident grid, since the late blocks take the virtual blocks
+    // left over, but the recombination (two resident per CU, LDS-bound) lost
+    // 7% at twice its resident grid, so its grid stays resident.
+    uint32_t per_cu = std::max<uint32_t>(
         1u, (Op::kThreads >= 1024 ? 2u : 8u) * L.grid_scale_num / L.grid_scale_den);
+    if constexpr (LiveTiles<Op>::value) {
+        static const uint32_t resident = [] {
+            int n = 0;
+            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(k_batched<Op>), Op::kThreads, 0);
+            return e == hipSuccess && n > 0 ? (uint32_t)n : 0u;
+        }();
+        if (resident) per_cu = std::min(per_cu, resident);
+    }
     // An odd grid: virtual block v runs on block v mod grid, and replicate
     // ranges are multiples of the column-tile count (a power of two), so an
     // even grid would hand every live tile-0 block to the same quarter of
