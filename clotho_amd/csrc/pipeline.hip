@@ -409,9 +409,16 @@ __device__ inline uint32_t pick_parent(const Params& P, bool constant, uint64_t 
 // Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
 // count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
 // (common_crossover::operator(), fill_events), and the top/bottom parent rows.
-// kScanTotals: each block scans the selection block totals into LDS itself
-// (up to kTotalsLds of them), in place of a separate SelBlockscan launch.
-template <bool kScanTotals>
+// Selection block totals (kTotals): kTotalsGlobal searches the scanned totals
+// in global memory; kTotalsScan has each block scan the raw totals into LDS
+// itself (up to kTotalsLds of them), in place of a separate SelBlockscan
+// launch; kTotalsCopy copies totals a SelBlockscan launch already scanned into
+// LDS (the batched launches), so the block search is a chain of LDS reads,
+// not of round trips.  The event-count table is read from LDS too: the
+// inversion walks it with one dependent read per step (up to ~10 steps at
+// rho = 5).
+constexpr int kTotalsGlobal = 0, kTotalsScan = 1, kTotalsCopy = 2;
+template <int kTotals>
 struct GametesOp {
     static constexpr int kThreads = kBlock;
     __host__ __device__ static uint32_t blocks(const Params& P) { return P.mut_blocks; }
@@ -426,14 +433,18 @@ struct GametesOp {
         const auto bot_out = gbl(B.bottom);
         const auto nev_out = gbl(B.n_events);
         const auto events = gbl(B.events);
-        __shared__ double s_cb[kScanTotals ? kTotalsLds : 1];
+        __shared__ double s_cb[kTotals != kTotalsGlobal ? kTotalsLds : 1];
         __shared__ double s_lds[kWaves];
+        __shared__ double s_cdf[kMaxCdf];
         const bool constant = uniform_read(&st->fit_constant) != 0;
-        bool staged = false;
+        const uint32_t nb = parent_blocks(P);
+        const bool staged = kTotals != kTotalsGlobal && nb <= kTotalsLds && !constant;
+        for (uint32_t i = threadIdx.x; i < P.n_cdf_rho; i += kThreads) s_cdf[i] = cdf_rho[i];
+        if (kTotals == kTotalsCopy && staged)
+            for (uint32_t i = threadIdx.x; i < nb; i += kThreads) s_cb[i] = cum_block[i];
+        __syncthreads();
         double scanned_total = 0.0;
-        if constexpr (kScanTotals) {
-            const uint32_t nb = parent_blocks(P);
-            staged = nb <= kTotalsLds && !constant;
+        if constexpr (kTotals == kTotalsScan) {
             if (staged) scanned_total = block_scan_totals<double, true>(cum_block, nb, s_cb, s_lds);
         }
         const uint32_t o = blk * kBlock + threadIdx.x;
@@ -444,12 +455,13 @@ struct GametesOp {
         const uint64_t k = (o & 1) ? k53(pd.x[2], pd.x[3]) : k53(pd.x[0], pd.x[1]);
         // given parents (one generation): Crossover::update's mate pairs
         const bool given = uniform_read(&st->given) != 0;
+        const uint64_t total = kTotals == kTotalsScan ? (uint64_t)scanned_total : st->sel_total;
         const uint32_t p =
             given ? min(gbl(B.given_parents)[o], P.n_par - 1)
-            : staged ? pick_parent(P, false, (uint64_t)scanned_total, k, cum_local, (const double*)s_cb)
+            : staged ? pick_parent(P, false, total, k, cum_local, (const double*)s_cb)
                      : pick_parent(P, constant, st->sel_total, k, cum_local, cum_block);
         const Draw4 g = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 0);
-        const uint32_t ne = poisson_draw(cdf_rho, P.n_cdf_rho, u53(g.x[0], g.x[1]));
+        const uint32_t ne = poisson_draw((const double*)s_cdf, P.n_cdf_rho, u53(g.x[0], g.x[1]));
         const bool swap = u53(g.x[2], g.x[3]) < P.bias;
         top_out[o] = swap ? 2 * p + 1 : 2 * p;
         bot_out[o] = swap ? 2 * p : 2 * p + 1;
@@ -470,7 +482,7 @@ struct GametesOp {
     }
 };
 
-using Gametes = GametesOp<false>;
+using Gametes = GametesOp<kTotalsCopy>;  // the batched launches (SelBlockscan scans)
 
 // Parent order of the child rows: a counting sort of the gametes by parent
 // individual (counts from the gamete draws, block-local exclusive scan, scan
@@ -2172,7 +2184,7 @@ CLOTHO_SINGLE(k_word_sort, WordSort)
 // statistics, each pair in one launch
 using DrawCounts = Fused<MutCounts, FitnessStats>;
 using DrawAttrs = Fused<MutAttrs, SelScan>;
-using DrawGametes = Fused<GametesOp<true>, WordSort>;
+using DrawGametes = Fused<GametesOp<kTotalsScan>, WordSort>;
 CLOTHO_SINGLE(k_draw_counts, DrawCounts)
 CLOTHO_SINGLE(k_draw_attrs, DrawAttrs)
 CLOTHO_SINGLE(k_draw_gametes, DrawGametes)
@@ -2249,7 +2261,7 @@ template <int RT, int NT, bool E>
 struct RoundRobin<TraitsI8<RT, NT, E>> {
     static constexpr bool value = true;
 };
-template <bool K>
+template <int K>
 struct RoundRobin<GametesOp<K>> {
     static constexpr bool value = true;
 };
