@@ -2165,6 +2165,12 @@ template <bool W, int K>
 struct MinWaves<Recombine<W, K>> {
     static constexpr int value = 8;
 };
+// four traits waves per SIMD (<= 128 registers): the batched launch at 136
+// registers held three 256-thread blocks per CU where its LDS allows four
+template <>
+struct MinWaves<TraitsI8<4, 1, false>> {
+    static constexpr int value = 4;
+};
 
 #define CLOTHO_SINGLE(name, Op)                                                                 \
     __global__ void __launch_bounds__(Op::kThreads)                                            \
