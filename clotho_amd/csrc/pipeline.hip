@@ -433,12 +433,16 @@ struct GametesOp {
         const auto bot_out = gbl(B.bottom);
         const auto nev_out = gbl(B.n_events);
         const auto events = gbl(B.events);
-        __shared__ double s_cb[kTotals != kTotalsGlobal ? kTotalsLds : 1];
+        // copied totals up to 1024 (8 KB: eight 256-thread blocks per CU
+        // still fit), scanned ones up to kTotalsLds; beyond, the global search
+        constexpr uint32_t kCap =
+            kTotals == kTotalsScan ? kTotalsLds : kTotals == kTotalsCopy ? 1024u : 1u;
+        __shared__ double s_cb[kCap];
         __shared__ double s_lds[kWaves];
         __shared__ double s_cdf[kMaxCdf];
         const bool constant = uniform_read(&st->fit_constant) != 0;
         const uint32_t nb = parent_blocks(P);
-        const bool staged = kTotals != kTotalsGlobal && nb <= kTotalsLds && !constant;
+        const bool staged = kTotals != kTotalsGlobal && nb <= kCap && !constant;
         for (uint32_t i = threadIdx.x; i < P.n_cdf_rho; i += kThreads) s_cdf[i] = cdf_rho[i];
         if (kTotals == kTotalsCopy && staged)
             for (uint32_t i = threadIdx.x; i < nb; i += kThreads) s_cb[i] = cum_block[i];
@@ -1956,8 +1960,9 @@ __global__ void __launch_bounds__(kBlock) k_sum_rows(const uint32_t* __restrict_
 // ---------------------------------------------------------------------------
 // Fitness: GeneralFitness over individuals, z = genome(2i) + genome(2i+1)
 // ---------------------------------------------------------------------------
-__device__ inline double fitness_of(const Params& P, const double* z) {
-    const double x = P.traits ? z[0] : 0.0;
+// x = the first trait value (0 without traits), ss = the sum of squares of
+// all trait values in trait order (the multivariate kind's only input).
+__device__ inline double fitness_of(const Params& P, double x, double ss) {
     switch (P.fitness_kind) {
         case CLOTHO_FIT_GAUSSIAN: {
             const double ex = (x * x) * P.fit_inv;
@@ -1970,11 +1975,8 @@ __device__ inline double fitness_of(const Params& P, const double* z) {
         }
         case CLOTHO_FIT_GAUSSIAN_FIXED:
             return det_exp(-((x * x) * P.fit_inv));
-        case CLOTHO_FIT_MV_STABILIZING: {
-            double ss = 0.0;
-            for (uint32_t t = 0; t < P.traits; ++t) ss = ss + z[t] * z[t];
+        case CLOTHO_FIT_MV_STABILIZING:
             return det_exp(-(ss * P.fit_inv));
-        }
         default:
             return 1.0;
     }
@@ -1996,17 +1998,22 @@ struct FitnessOp {
         const uint32_t i = blk * kBlock + threadIdx.x;
         if (i >= P.n) return;
         const uint32_t gen = kCurrent ? st->gen - 1 : st->gen;
-        double z[kMaxTraits];
+        // trait values in trait order, kept as the first value and the sum of
+        // squares (no per-thread array: indexed by a loop counter it lives in
+        // scratch memory)
+        double x0 = 0.0, ss = 0.0;
         for (uint32_t t = 0; t < P.traits; ++t) {
             const int64_t g = traits[(size_t)(2 * i) * P.traits + t] +
                               traits[(size_t)(2 * i + 1) * P.traits + t];
-            z[t] = (double)g * P.eff_quantum;
+            double z = (double)g * P.eff_quantum;
             if (P.env_sigma > 0.0) {
                 const Draw4 d = philox_draw(P.seed, gen, S_ENV, P.replicate, i, t);
-                z[t] = z[t] + P.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
+                z = z + P.env_sigma * normal_ppf(uopen(d.x[0], d.x[1]));
             }
+            if (t == 0) x0 = z;
+            ss = ss + z * z;
         }
-        fit[i] = fitness_of(P, z);
+        fit[i] = fitness_of(P, x0, ss);
     }
 };
 using Fitness = FitnessOp<false>;
