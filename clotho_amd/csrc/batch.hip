@@ -26,7 +26,7 @@ struct clotho_batch {
     std::vector<RepView> host_views;  // last uploaded copy
     RepView* d_views = nullptr;
     uint32_t* d_offs = nullptr;
-    uint32_t* d_live_offs = nullptr;  // [R + 1], written on the device each generation
+    uint32_t* d_live_offs = nullptr;  // [kLiveKinds][R + 1], written on the device each generation
     BatchLaunch L;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int cur = 0;
@@ -82,7 +82,7 @@ int sync_views(clotho_batch* b) {
     if (!b->d_views) {
         BATCH_TRY(b, hipMalloc((void**)&b->d_views, sizeof(RepView) * R));
         BATCH_TRY(b, hipMalloc((void**)&b->d_offs, sizeof(uint32_t) * offs.size()));
-        BATCH_TRY(b, hipMalloc((void**)&b->d_live_offs, sizeof(uint32_t) * (R + 1)));
+        BATCH_TRY(b, hipMalloc((void**)&b->d_live_offs, sizeof(uint32_t) * kLiveKinds * (R + 1)));
     }
     BATCH_TRY(b, hipMemcpy(b->d_views, v.data(), sizeof(RepView) * R, hipMemcpyHostToDevice));
     BATCH_TRY(b, hipMemcpy(b->d_offs, offs.data(), sizeof(uint32_t) * offs.size(),

@@ -2245,16 +2245,49 @@ CLOTHO_SINGLE(k_compact_fitness, CompactFitness)
 // enough parent loads in flight.
 constexpr uint32_t kBatchOffsLds = 1024;  // replicate offsets kept in LDS up to this R
 
-// Ops whose batched offsets count live column tiles only (k_batch_live_offsets):
-// the local index u of a replicate maps to tile u mod lt, row chunk u / lt,
-// lt = ceil(wc / 64), i.e. to Op block (u / lt) tiles + u mod lt.
+// Ops whose batched offsets count live blocks only (k_batch_live_offsets).
+// Their block ranges at capacity cover the allele capacity, but the child
+// rows are wc words wide this generation, so only the blocks of the first wc
+// words do any work.  kind indexes the live offsets; block() maps the local
+// live index u of a replicate to the Op's own block number.
 template <class Op>
-struct LiveTiles {
-    static constexpr bool value = false;
+struct LiveBlocks {
+    static constexpr int kind = -1;
 };
 template <bool W, int K>
-struct LiveTiles<Recombine<W, K>> {
-    static constexpr bool value = true;
+struct LiveBlocks<Recombine<W, K>> {
+    // live tiles lt = ceil(wc / 64) x row chunks: u -> tile u mod lt, chunk u / lt
+    static constexpr int kind = kLiveRecombine;
+    __host__ __device__ static uint32_t count(const Params& P, uint32_t wc) {
+        return min((wc + 63) / 64, P.tiles) * P.chunks;
+    }
+    __device__ static uint32_t block(const Params& P, uint32_t wc, uint32_t u) {
+        const uint32_t lt = (wc + 63) / 64;  // >= 1 when u exists
+        return (u / lt) * P.tiles + u % lt;
+    }
+};
+template <>
+struct LiveBlocks<WordSort> {
+    // kWaves words per block: the live blocks are the first ceil(wc / kWaves)
+    static constexpr int kind = kLiveWordSort;
+    __host__ __device__ static uint32_t count(const Params& P, uint32_t wc) {
+        return min((wc + kWaves - 1) / kWaves, WordSort::blocks(P));
+    }
+    __device__ static uint32_t block(const Params&, uint32_t, uint32_t u) { return u; }
+};
+template <>
+struct LiveBlocks<PackI8> {
+    // four blocks per (trait tile, 16-word group); live groups NG = ceil(wc / 16)
+    static constexpr int kind = kLivePack;
+    __host__ __device__ static uint32_t count(const Params& P, uint32_t wc) {
+        if (!P.traits) return 0u;
+        const uint32_t ng = min((wc + i8::kGroupWords - 1) / i8::kGroupWords, PackI8::ng_cap(P));
+        return i8::kernel_tiles(P.traits) * ng * 4u;
+    }
+    __device__ static uint32_t block(const Params& P, uint32_t wc, uint32_t u) {
+        const uint32_t per = ((wc + i8::kGroupWords - 1) / i8::kGroupWords) * 4u;  // >= 4 here
+        return (u / per) * PackI8::ng_cap(P) * 4u + u % per;
+    }
 };
 
 // Batched ops whose persistent blocks take virtual blocks round robin: their
@@ -2295,49 +2328,54 @@ struct RoundRobin<FreeCompact> {
     static constexpr bool value = true;
 };
 
-// Live virtual blocks of the batched recombination.  A replicate's blocks at
-// capacity cover every column tile of its allele capacity, but its child rows
-// are wc words wide this generation, so only ceil(wc / 64) tiles hold words
-// (a block of a tile past wc does nothing).  One block scans live tiles x row
-// chunks over the replicates into out[0 .. R] (exclusive, out[R] = total), so
-// the persistent blocks of the recombination visit live tiles only.
+// Live virtual blocks of the batched recombination, word sort and digit
+// packing (LiveBlocks): one block scans each kind's live block counts over the
+// replicates into out[kind][0 .. R] (exclusive, out[kind][R] = total) from the
+// child widths the draws set, so the persistent blocks of those ops visit live
+// blocks only.
 __global__ void __launch_bounds__(1024) k_batch_live_offsets(const RepView* __restrict__ views,
                                                              uint32_t R, uint32_t* __restrict__ out) {
     __shared__ uint32_t s_wave[16];
     __shared__ uint32_t s_carry;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_carry = 0;
-    __syncthreads();
-    for (uint32_t base = 0; base < R; base += 1024) {
-        const uint32_t r = base + tid;
-        uint32_t n = 0;
-        if (r < R) {
-            const RepView& rv = views[r];
-            const uint32_t wc = gbl(rv.B.state)->wc;
-            n = min((wc + 63) / 64, rv.P.tiles) * rv.P.chunks;
-        }
-        uint32_t x = n;  // inclusive scan within the wave
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) s_wave[wave] = x;
+    for (int kind = 0; kind < kLiveKinds; ++kind) {
+        uint32_t* const o = out + (size_t)kind * (R + 1);
+        if (tid == 0) s_carry = 0;
         __syncthreads();
-        if (wave == 0) {
-            uint32_t t = lane < 16 ? s_wave[lane] : 0u;
-            for (uint32_t d = 1; d < 16; d <<= 1) {
-                const uint32_t y = (uint32_t)__shfl_up((int)t, d, 64);
-                if (lane >= d) t += y;
+        for (uint32_t base = 0; base < R; base += 1024) {
+            const uint32_t r = base + tid;
+            uint32_t n = 0;
+            if (r < R) {
+                const RepView& rv = views[r];
+                const uint32_t wc = gbl(rv.B.state)->wc;
+                n = kind == kLiveRecombine ? LiveBlocks<RecombineNarrow>::count(rv.P, wc)
+                    : kind == kLiveWordSort ? LiveBlocks<WordSort>::count(rv.P, wc)
+                                            : LiveBlocks<PackI8>::count(rv.P, wc);
             }
-            if (lane < 16) s_wave[lane] = t;  // inclusive wave totals
+            uint32_t x = n;  // inclusive scan within the wave
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+                if (lane >= d) x += y;
+            }
+            if (lane == 63) s_wave[wave] = x;
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t t = lane < 16 ? s_wave[lane] : 0u;
+                for (uint32_t d = 1; d < 16; d <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)t, d, 64);
+                    if (lane >= d) t += y;
+                }
+                if (lane < 16) s_wave[lane] = t;  // inclusive wave totals
+            }
+            __syncthreads();
+            if (r < R) o[r] = s_carry + (wave ? s_wave[wave - 1] : 0u) + x - n;
+            __syncthreads();
+            if (tid == 0) s_carry += s_wave[15];
+            __syncthreads();
         }
-        __syncthreads();
-        if (r < R) out[r] = s_carry + (wave ? s_wave[wave - 1] : 0u) + x - n;
-        __syncthreads();
-        if (tid == 0) s_carry += s_wave[15];
+        if (tid == 0) o[R] = s_carry;
         __syncthreads();
     }
-    if (tid == 0) out[R] = s_carry;
 }
 
 template <class Op>
@@ -2385,10 +2423,8 @@ __global__ void __launch_bounds__(Op::kThreads)
         lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: descriptor reads stay scalar
         const RepView& rv = views[lo];
         uint32_t blk = v - off(lo);
-        if constexpr (LiveTiles<Op>::value) {
-            const uint32_t lt = (uniform_read(&gbl(rv.B.state)->wc) + 63) / 64;  // >= 1 here
-            blk = (blk / lt) * rv.P.tiles + blk % lt;
-        }
+        if constexpr (LiveBlocks<Op>::kind >= 0)
+            blk = LiveBlocks<Op>::block(rv.P, uniform_read(&gbl(rv.B.state)->wc), blk);
         Op::run(rv.P, rv.B, sel, blk);
         __syncthreads();
     }
@@ -2570,7 +2606,7 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
     // 7% at twice its resident grid, so its grid stays resident.
     uint32_t per_cu = std::max<uint32_t>(
         1u, (Op::kThreads >= 1024 ? 2u : 8u) * L.grid_scale_num / L.grid_scale_den);
-    if constexpr (LiveTiles<Op>::value) {
+    if constexpr (LiveBlocks<Op>::kind == kLiveRecombine) {
         static const uint32_t resident = [] {
             int n = 0;
             const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -2585,16 +2621,21 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
     // the grid while the rest only meet tiles past the live width.
     uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
     if (grid > 1 && !(grid & 1u)) --grid;
-    const uint32_t* offs = LiveTiles<Op>::value ? L.live_offs : L.offs + (size_t)op * (L.R + 1);
+    const uint32_t* offs = LiveBlocks<Op>::kind >= 0
+                               ? L.live_offs + (size_t)LiveBlocks<Op>::kind * (L.R + 1)
+                               : L.offs + (size_t)op * (L.R + 1);
     hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::kThreads), 0, s, L.views, offs, L.R, sel);
     return hipGetLastError();
 }
 
-// The recombination's live-tile offsets of this generation (child widths are
-// set by the draws), then the recombination.
-hipError_t launch_batched_recombine(const BatchLaunch& L, uint32_t par, hipStream_t s) {
+// The live-block offsets of this generation (child widths are set by the
+// draws' Prepare launch), for the word sort, recombination and digit packing.
+hipError_t launch_live_offsets(const BatchLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_live_offsets, dim3(1), dim3(1024), 0, s, L.views, L.R, L.live_offs);
-    TRY_LAUNCH(hipGetLastError());
+    return hipGetLastError();
+}
+
+hipError_t launch_batched_recombine(const BatchLaunch& L, uint32_t par, hipStream_t s) {
     if (L.wide_events) return launch_batched<BatchRecombineWide>(L, kOpRecombine, par, s);
     return launch_batched<BatchRecombineNarrow>(L, kOpRecombine, par, s);
 }
@@ -2604,6 +2645,7 @@ hipError_t launch_batched_recombine(const BatchLaunch& L, uint32_t par, hipStrea
 // The batch's recombination and traits over the last generation's gametes,
 // parents [par], children into [1 - par] (the autotuner's timed replay).
 hipError_t launch_batch_replay(const BatchLaunch& L, uint32_t par, hipStream_t s) {
+    TRY_LAUNCH(launch_live_offsets(L, s));
     TRY_LAUNCH(launch_batched_recombine(L, par, s));
     if (!L.traits) return hipSuccess;
     const uint32_t chi = 1 - par;
@@ -2627,6 +2669,7 @@ hipError_t launch_batch_generation(const BatchLaunch& L, uint32_t par, hipStream
     TRY_LAUNCH(launch_batched<SelBlockscan>(L, kOpSelBlockscan, par, s));
     TRY_LAUNCH(launch_batched<Gametes>(L, kOpGametes, par, s));
     TRY_LAUNCH(mark(1));
+    TRY_LAUNCH(launch_live_offsets(L, s));
     TRY_LAUNCH(launch_batched<WordSort>(L, kOpWordSort, par, s));
     TRY_LAUNCH(launch_batched_recombine(L, par, s));
     TRY_LAUNCH(mark(2));

@@ -183,6 +183,10 @@ enum BatchOp : int {
     kOpCount
 };
 
+// Batched ops whose virtual blocks are counted from the child width each
+// generation (live blocks), and their rows of BatchLaunch::live_offs.
+constexpr int kLiveRecombine = 0, kLiveWordSort = 1, kLivePack = 2, kLiveKinds = 3;
+
 // Row chunks of a replicate's batched recombination: its own launch shape's
 // chunk count divided by this (two measured best of 1, 2, 4, 8 on the
 // 256-replicate workload at generation 200: 1.12 against 1.20 ms).
@@ -191,7 +195,7 @@ constexpr uint32_t kBatchChunkDiv = 2;
 struct BatchLaunch {
     const RepView* views = nullptr;  // device [R]
     const uint32_t* offs = nullptr;  // device [kOpCount][R + 1]
-    uint32_t* live_offs = nullptr;   // device [R + 1]: the recombination's live blocks
+    uint32_t* live_offs = nullptr;   // device [kLiveKinds][R + 1]: live blocks of three ops
     uint32_t R = 0;
     uint32_t total[kOpCount] = {};   // offs[k][R]
     uint32_t traits = 0;             // trait count shared by every replicate
