@@ -81,11 +81,16 @@ def test_batch_replicates_match_oracle():
     batch.close()
 
 
-def test_batch_multitrait():
-    cfgs = _configs(traits=3, wide=False)
+@pytest.mark.parametrize("traits", [3, 6, 20])
+def test_batch_multitrait(traits):
+    # 1, 2 and 8 digit-plane tiles: the batched packing visits the live
+    # 16-word groups of every tile, and the batched trait kernels of up to 8,
+    # 16 and 32 traits run
+    cfgs = _configs(traits=traits, wide=False)
     batch = api.Batch(cfgs)
     batch.run(6, mode="graph")
     for r, cfg in enumerate(cfgs):
+        print(f"traits {traits} replicate {r}", flush=True)
         compare_state(batch.replicate(r), _lone(cfg, 6), f"replicate {r}")
     batch.close()
 
