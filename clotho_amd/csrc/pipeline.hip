@@ -2205,8 +2205,21 @@ using RecombineNarrow = Recombine<false, kSibsOrdered>;  // at most 64 events pe
 using RecombineWide = Recombine<true, 2>;  // two sets of 128 event keys stay in registers
 using BatchRecombineNarrow = Recombine<false, 1>;
 using BatchRecombineWide = Recombine<true, 1>;
-CLOTHO_SINGLE(k_recombine, RecombineNarrow)
-CLOTHO_SINGLE(k_recombine_wide, RecombineWide)
+// The recombination's blocks are numbered so that each XCD runs a contiguous
+// range of (row chunk, column tile) blocks, i.e. whole row chunks one tile
+// after another: the chunk's gamete records and event keys, which every tile
+// block of the chunk reads, then stay in that XCD's L2 (with the hardware's
+// round-robin order every tile block of a chunk ran on another XCD and
+// fetched them again).
+#define CLOTHO_SINGLE_XCD(name, Op)                                                             \
+    __global__ void __launch_bounds__(Op::kThreads)                                            \
+        __attribute__((amdgpu_waves_per_eu(MinWaves<Op>::value)))                               \
+        name(Params P, Buffers B, uint32_t sel) {                                               \
+        Op::run(P, B, sel, xcd_contiguous(blockIdx.x, gridDim.x));                              \
+    }
+CLOTHO_SINGLE_XCD(k_recombine, RecombineNarrow)
+CLOTHO_SINGLE_XCD(k_recombine_wide, RecombineWide)
+#undef CLOTHO_SINGLE_XCD
 CLOTHO_SINGLE(k_pack_i8_pop, PackI8)
 using Traits4Even = TraitsI8<5, 1, true>;  // T <= 4, grid within one round of resident blocks
 using Traits4 = TraitsI8<4, 1, false>;      // T <= 4
