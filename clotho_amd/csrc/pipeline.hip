@@ -409,15 +409,15 @@ __device__ inline uint32_t pick_parent(const Params& P, bool constant, uint64_t 
 // Per offspring gamete o = 2i + s: parent (id0, id1 of offspring i), event
 // count E ~ Poisson(rho), strand swap ~ Bernoulli(bias), E event keys
 // (common_crossover::operator(), fill_events), and the top/bottom parent rows.
-// Selection block totals (kTotals): kTotalsGlobal searches the scanned totals
-// in global memory; kTotalsScan has each block scan the raw totals into LDS
-// itself (up to kTotalsLds of them), in place of a separate SelBlockscan
-// launch; kTotalsCopy copies totals a SelBlockscan launch already scanned into
-// LDS (the batched launches), so the block search is a chain of LDS reads,
-// not of round trips.  The event-count table is read from LDS too: the
-// inversion walks it with one dependent read per step (up to ~10 steps at
-// rho = 5).
-constexpr int kTotalsGlobal = 0, kTotalsScan = 1, kTotalsCopy = 2;
+// Selection block totals (kTotals), held in LDS so that the block search is a
+// chain of LDS reads, not of round trips: kTotalsScan has each block scan the
+// raw totals itself (up to kTotalsLds of them), in place of a separate
+// SelBlockscan launch; kTotalsCopy copies totals a SelBlockscan launch
+// already scanned (the batched launches, up to 1024 of them).  Beyond those
+// counts the search reads the scanned totals in global memory.  The
+// event-count table is read from LDS too: the inversion walks it with one
+// dependent read per step (up to ~10 steps at rho = 5).
+constexpr int kTotalsScan = 1, kTotalsCopy = 2;
 template <int kTotals>
 struct GametesOp {
     static constexpr int kThreads = kBlock;
@@ -435,14 +435,13 @@ struct GametesOp {
         const auto events = gbl(B.events);
         // copied totals up to 1024 (8 KB: eight 256-thread blocks per CU
         // still fit), scanned ones up to kTotalsLds; beyond, the global search
-        constexpr uint32_t kCap =
-            kTotals == kTotalsScan ? kTotalsLds : kTotals == kTotalsCopy ? 1024u : 1u;
+        constexpr uint32_t kCap = kTotals == kTotalsScan ? kTotalsLds : 1024u;
         __shared__ double s_cb[kCap];
         __shared__ double s_lds[kWaves];
         __shared__ double s_cdf[kMaxCdf];
         const bool constant = uniform_read(&st->fit_constant) != 0;
         const uint32_t nb = parent_blocks(P);
-        const bool staged = kTotals != kTotalsGlobal && nb <= kCap && !constant;
+        const bool staged = nb <= kCap && !constant;
         for (uint32_t i = threadIdx.x; i < P.n_cdf_rho; i += kThreads) s_cdf[i] = cdf_rho[i];
         if (kTotals == kTotalsCopy && staged)
             for (uint32_t i = threadIdx.x; i < nb; i += kThreads) s_cb[i] = cum_block[i];
@@ -2628,10 +2627,10 @@ hipError_t launch_batched(const BatchLaunch& L, int op, uint32_t sel, hipStream_
         }();
         if (resident) per_cu = std::min(per_cu, resident);
     }
-    // An odd grid: virtual block v runs on block v mod grid, and replicate
-    // ranges are multiples of the column-tile count (a power of two), so an
-    // even grid would hand every live tile-0 block to the same quarter of
-    // the grid while the rest only meet tiles past the live width.
+    // An odd grid: round-robin virtual block v runs on block v mod grid, and
+    // with replicate ranges that are multiples of a power of two (the column
+    // tiles at capacity, before the live-block offsets) an even grid handed
+    // every tile-0 block to the same quarter of the grid.
     uint32_t grid = std::min<uint32_t>(total, per_cu * L.cus);
     if (grid > 1 && !(grid & 1u)) --grid;
     const uint32_t* offs = LiveBlocks<Op>::kind >= 0
