@@ -799,8 +799,8 @@ __device__ __forceinline__ void recombine_folded(const Params& P, const Buffers&
 
     // Each wave owns a contiguous segment of the chunk's rows (a multiple of
     // NF rows).  The gamete records of 64 rows (event and mutation counts,
-    // top row) are loaded with one vector load, lane q holding row q of the
-    // batch; a step of NF rows reads its rows' records from those registers
+    // top row) are loaded with one load per field, lane q holding row q of
+    // the batch; a step of NF rows reads its rows' records from those registers
     // with shuffles, so only the parent words and event keys of a step are
     // loads of their own.  Steps alternate between two register sets: the
     // loads of step k + 2 are issued as soon as step k is done and stay in
