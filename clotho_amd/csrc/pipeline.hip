@@ -1479,10 +1479,11 @@ __global__ void __launch_bounds__(kBlock) k_compact_alleles(Params P, Buffers B)
 // input words) and the wave packs kCompactRows rows per step, their loads in
 // flight together; otherwise it packs one row at a time, reading the plan
 // per row.  With a fraction f of the columns removed a 64-word tile draws on
-// about 64 / (1 - f) + 1 input words, so any removal sends it down the
-// two-plan path, which compresses a second, nearly idle set of words;
-// narrower tiles stay on one plan but write rows in pieces that do not start
-// on 128-byte lines.  The autotuner picks the width.
+// about 64 / (1 - f) + 1 input words.  The few words past the first 64 (up
+// to 64 / kCompactRows of them) are packed for all the step's rows in one
+// compress, lanes spread over (row, word); more take the two-plan path,
+// which compresses a second set of words per row.  Narrower tiles stay on one
+// plan but write rows in pieces that do not start on 128-byte lines.
 constexpr int kCompactWaves = 4;
 
 // Block x of a grid of nb runs on XCD x mod 8; the returned index is a
@@ -1493,11 +1494,28 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t x, uint32_t nb) {
     return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
 }
 
+// One packed input word into a tile's LDS row: x holds the word's n kept
+// bits, packed, and d is their tile-relative destination bit.
+__device__ __forceinline__ void or_packed(unsigned long long* row, uint64_t x, int d, uint32_t n) {
+    if (d >= 0) {
+        const uint32_t u0 = (uint32_t)d >> 6, s = (uint32_t)d & 63;
+        if (u0 < 64) atomicOr(&row[u0], (unsigned long long)(x << s));
+        if (s && s + n > 64 && u0 + 1 < 64) atomicOr(&row[u0 + 1], (unsigned long long)(x >> (64 - s)));
+    } else if (-d < (int)n) {
+        atomicOr(&row[0], (unsigned long long)(x >> (-d)));
+    }
+}
+
 // The planned path of k_compact_rows: lane l holds the plan (keep mask,
 // compress masks, destination bit) of input words wa + l + 64 i, i < NWL, and
 // the wave packs kCompactRows rows per step from row o_first, `step` rows
-// apart, up to row r1.
-template <int NWL, int kCompactRows, class In, class Out, class Keep, class Dst, class Mv>
+// apart, up to row r1.  With kSpill (NWL = 1) the tile draws on m = wb - wa -
+// 64 more words, m * kCompactRows <= 64, and these are packed for all the
+// step's rows at once: lane l holds the plan of word wa + 64 + l % m and packs
+// it for row l / m of the step, so the m words cost one compress per step
+// instead of one per row.
+template <int NWL, bool kSpill, int kCompactRows, class In, class Out, class Keep, class Dst,
+          class Mv>
 __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out out, Keep keepm,
                                                      Dst dst, Mv mvg,
                                                      unsigned long long (*s_row)[64], uint32_t W,
@@ -1505,6 +1523,7 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
                                                      uint32_t wb, uint32_t lo_bit,
                                                      uint32_t o_first, uint32_t r1,
                                                      uint32_t step) {
+    static_assert(!kSpill || NWL == 1, "spill words follow a single plan");
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t u = lo_bit / 64 + lane;
     const bool writes = lane < tw && u < W2;
@@ -1522,22 +1541,39 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
         for (int j = 0; j < 6; ++j) mv[i][j] = mvg[(size_t)j * P.w_cap + wr[i]];
         d[i] = (int)dst[wr[i]] - (int)lo_bit;  // tile-relative bit of the packed word
     }
+    // spill plan (kSpill): lanes past m * kCompactRows keep no bits; their
+    // row and word are clamped in bounds for the unconditional loads
+    uint32_t sk = 0, sw = wa, sn = 0;
+    uint64_t skeep = 0, smv[6] = {};
+    int sd = 0;
+    if constexpr (kSpill) {
+        const uint32_t m = wb - wa - 64;
+        sk = min(lane / m, (uint32_t)kCompactRows - 1);
+        sw = wa + 64 + lane % m;
+        skeep = lane < m * kCompactRows ? keepm[sw] : 0;
+        sn = (uint32_t)__popcll(skeep);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) smv[j] = mvg[(size_t)j * P.w_cap + sw];
+        sd = (int)dst[sw] - (int)lo_bit;
+    }
     if (o_first >= r1) return;
     // software pipeline: the rows of step s + 1 are loaded (row indices
     // clamped into the chunk, so every load is unconditional and in bounds)
     // before step s is packed, keeping loads in flight through the packing
-    uint64_t xin[kCompactRows][NWL];
+    uint64_t xin[kCompactRows][NWL], xs = 0;
 #pragma unroll
     for (int k = 0; k < kCompactRows; ++k)
 #pragma unroll
         for (int i = 0; i < NWL; ++i) xin[k][i] = in[(size_t)min(o_first + k, r1 - 1) * W + wr[i]];
+    if constexpr (kSpill) xs = in[(size_t)min(o_first + sk, r1 - 1) * W + sw];
     for (uint32_t o0 = o_first; o0 < r1; o0 += step) {
-        uint64_t xnx[kCompactRows][NWL];
+        uint64_t xnx[kCompactRows][NWL], xsn = 0;
         const uint32_t o1 = min(o0 + step, r1 - 1);
 #pragma unroll
         for (int k = 0; k < kCompactRows; ++k)
 #pragma unroll
             for (int i = 0; i < NWL; ++i) xnx[k][i] = in[(size_t)min(o1 + k, r1 - 1) * W + wr[i]];
+        if constexpr (kSpill) xsn = in[(size_t)min(o1 + sk, r1 - 1) * W + sw];
 #pragma unroll
         for (int k = 0; k < kCompactRows; ++k) s_row[k][lane] = 0;
         __builtin_amdgcn_wave_barrier();
@@ -1546,17 +1582,12 @@ __device__ __forceinline__ void compact_rows_planned(const Params& P, In in, Out
         for (int i = 0; i < NWL; ++i) {
             if (!n[i]) continue;
 #pragma unroll
-            for (int k = 0; k < kCompactRows; ++k) {
-                const uint64_t x = compress_bits(xin[k][i] & keep[i], mv[i]);
-                if (d[i] >= 0) {
-                    const uint32_t u0 = (uint32_t)d[i] >> 6, s = (uint32_t)d[i] & 63;
-                    if (u0 < 64) atomicOr(&s_row[k][u0], (unsigned long long)(x << s));
-                    if (s && s + n[i] > 64 && u0 + 1 < 64)
-                        atomicOr(&s_row[k][u0 + 1], (unsigned long long)(x >> (64 - s)));
-                } else if (-d[i] < (int)n[i]) {
-                    atomicOr(&s_row[k][0], (unsigned long long)(x >> (-d[i])));
-                }
-            }
+            for (int k = 0; k < kCompactRows; ++k)
+                or_packed(s_row[k], compress_bits(xin[k][i] & keep[i], mv[i]), d[i], n[i]);
+        }
+        if constexpr (kSpill) {
+            if (sn) or_packed(s_row[sk], compress_bits(xs & skeep, smv), sd, sn);
+            xs = xsn;
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1626,13 +1657,18 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
     const uint32_t r0 = chunk * rows_per_chunk, r1 = min(P.h, r0 + rows_per_chunk);
     const uint32_t u = tile * tw + lane;
     if (wb - wa <= 64) {
-        compact_rows_planned<1, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, tw,
-                                              wa, wb, lo_bit, r0, r1, kCompactRows);
+        compact_rows_planned<1, false, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W,
+                                                     W2, tw, wa, wb, lo_bit, r0, r1, kCompactRows);
+        return;
+    }
+    if ((wb - wa - 64) * kCompactRows <= 64) {
+        compact_rows_planned<1, true, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W,
+                                                    W2, tw, wa, wb, lo_bit, r0, r1, kCompactRows);
         return;
     }
     if (wb - wa <= 128) {
-        compact_rows_planned<2, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W, W2, tw,
-                                              wa, wb, lo_bit, r0, r1, kCompactRows);
+        compact_rows_planned<2, false, kCompactRows>(P, in, out, keepm, dst, mvg, s_out[wave], W,
+                                                     W2, tw, wa, wb, lo_bit, r0, r1, kCompactRows);
         return;
     }
     for (uint32_t o = r0; o < r1; ++o) {
@@ -1648,16 +1684,9 @@ __global__ void __launch_bounds__(64 * kCompactWaves) k_compact_rows(Params P, B
             uint64_t mv[6];
 #pragma unroll
             for (int i = 0; i < 6; ++i) mv[i] = mvg[(size_t)i * P.w_cap + w];
-            const uint64_t x = compress_bits(in[(size_t)o * W + w] & keep, mv);
-            const int d = (int)dst[w] - (int)lo_bit;  // tile-relative bit of the packed word
-            if (d >= 0) {
-                const uint32_t u0 = (uint32_t)d >> 6, s = (uint32_t)d & 63;
-                if (u0 < 64) atomicOr(&s_out[wave][0][u0], (unsigned long long)(x << s));
-                if (s && s + n > 64 && u0 + 1 < 64)
-                    atomicOr(&s_out[wave][0][u0 + 1], (unsigned long long)(x >> (64 - s)));
-            } else if (-d < (int)n) {
-                atomicOr(&s_out[wave][0][0], (unsigned long long)(x >> (-d)));
-            }
+            // tile-relative destination bit of the packed word
+            or_packed(s_out[wave][0], compress_bits(in[(size_t)o * W + w] & keep, mv),
+                      (int)dst[w] - (int)lo_bit, n);
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

@@ -101,14 +101,17 @@ def test_compaction_invariant_at_bench_shape():
     assert a.rows().shape[0] == h
 
 
+@pytest.mark.parametrize("rows_per_step", [4, 8, 16])
 @pytest.mark.parametrize("tile_words", [64, 60, 56, 48])
 @pytest.mark.parametrize("removed", [0.01, 0.04, 0.5])
-def test_compact_row_paths(removed, tile_words):
-    """Four tiles of input words.  With 1% of the columns removed an output
-    tile of 64 words draws on at most 64 input words (plan in registers, rows
-    in groups), with 4% on up to 128 (two plans) unless the tile is narrower;
-    with half removed the tiles draw on about 128 or more (plan read per
-    row).  Every tile width gives the oracle's rows."""
+def test_compact_row_paths(removed, tile_words, rows_per_step):
+    """Four tiles of input words.  With 1% or 4% of the columns removed an
+    output tile of 48-60 words draws on at most 64 input words (plan in
+    registers, rows in groups) and a tile of 64 words on 1-4 more, packed for
+    all of a step's rows at once; with half removed the narrower tiles draw
+    on 97-122 input words (two plans) and 64-word tiles on 127-130 (two
+    plans, or the plan read per row).  Every tile width and step gives the
+    oracle's rows."""
     rng = np.random.default_rng(int(removed * 100) + 11)
     n, alleles = 150, 16000
     words = ((alleles + 511) // 512) * 8
@@ -127,8 +130,9 @@ def test_compact_row_paths(removed, tile_words):
     sim, orc = api.Simulation(cfg), oracle.OracleSim(cfg)
     sim.set_population(rows, pos, eff)
     orc.set_population(rows, pos, eff)
-    sim.set_launch_shape(compact_tile_words=tile_words)
+    sim.set_launch_shape(compact_tile_words=tile_words, compact_rows=rows_per_step)
     assert sim.launch_shape()["compact_tile_words"] == tile_words
+    assert sim.launch_shape()["compact_rows"] == rows_per_step
     sim.compact()
     orc.compact()
     compare_state(sim, orc, f"after compact ({removed:.0%} removed, {tile_words}-word tiles)")
