@@ -66,10 +66,12 @@ def compaction_spans(api, slots, generations):
     """Where a compaction after each generation removes columns, and the
     input words each output tile of the compaction's row pass draws on: a
     tile of tw words whose kept bits come from at most 64 input words takes
-    the one-plan path, up to 128 the two-plan path, beyond that the per-row
-    path.  The generations run without compact_every and are compacted
-    explicitly after their free-space lists are read, which is the same
-    sequence of states."""
+    the one-plan path, and 64 + m words with m * rows_per_step <= 64 the
+    same path with the m extra words packed for a step's rows in one pass
+    (counted per rows per step); otherwise up to 128 words take the two-plan
+    path and more the per-row path.  The generations run without
+    compact_every and are compacted explicitly after their free-space lists
+    are read, which is the same sequence of states."""
     sim = api.Simulation(config(api, slots + 8 * 4096, compact=False))
     freqs, positions, effects = standing_variation(slots)
     sim.set_synthetic(freqs, positions, effects, seed=42)
@@ -90,13 +92,17 @@ def compaction_spans(api, slots, generations):
                [round(float(x.mean()), 4) for x in np.array_split(~keep[:s.alleles], 8)]}
         for tw in (64, 60, 56, 48):
             paths = [0, 0, 0]
+            spill = {4: 0, 8: 0, 16: 0}
             widest = 0
             for t in range((w2 + tw - 1) // tw):
                 wa = int(np.searchsorted(dst, t * tw * 64, side="right")) - 1
                 wb = int(np.searchsorted(dst, (t + 1) * tw * 64, side="left"))
                 widest = max(widest, wb - wa)
                 paths[0 if wb - wa <= 64 else 1 if wb - wa <= 128 else 2] += 1
-            rec[f"tiles_{tw}"] = {"one_plan": paths[0], "two_plans": paths[1],
+                for rows in spill:
+                    spill[rows] += int(0 < wb - wa - 64 and (wb - wa - 64) * rows <= 64)
+            rec[f"tiles_{tw}"] = {"one_plan": paths[0], "over_64": paths[1],
+                                  "one_pass_extra_by_rows_per_step": spill,
                                   "per_row": paths[2], "widest_span": widest}
         print(json.dumps(rec), flush=True)
         sim.compact()
