@@ -110,6 +110,24 @@ __device__ inline WaveTiles wave_tiles_even(uint32_t block, uint32_t wave, uint3
     return WaveTiles{t0, (uint32_t)((i + 1) * tiles / waves) - t0};
 }
 
+// End of a stage: wait until at most N vector-memory loads are in flight,
+// then until the wave's LDS accesses are done, then meet the block.  The
+// builtin wait tells the compiler which loads have landed (so that it adds
+// no wait of its own before the next stage's LDS reads); the asm statement
+// repeats it with the barrier, and its memory clobber keeps the compiler
+// from moving a memory access across.  __syncthreads instead waits for every
+// load in flight (its fences cover global memory, and a fence restricted to
+// LDS counts direct-to-LDS loads as pending LDS writes), so loads of the next
+// group could not stay in flight across it.  The caller pins its loads with
+// compiler barriers right after issuing them, so that none is issued
+// between the wait and the barrier.
+template <int N>
+__device__ __forceinline__ void stage_barrier() {
+    static_assert(N >= 0 && N < 16, "vmcnt immediate");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
 // Exact traits: one wave = up to RT row tiles of 16 rows over all K; NT N tiles.
 // The block stages the B fragments in LDS, shared by its four waves, in
 // stages of 16 KB: stage s of group G holds q-blocks s QS .. s QS + QS - 1
@@ -170,6 +188,24 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
             }
         }
     };
+    // With several stages per group (NT >= 2) the next group's rows are
+    // loaded after the first stage's B loads (a compiler barrier keeps that
+    // order), all 2 RT loads issued in every group (the group and word
+    // clamped in bounds: W is a multiple of 8) and the invalid ones zeroed
+    // when the group is taken over.  The wait that ends the first stage then
+    // leaves exactly these loads in flight, so they have two stages of
+    // products to land in instead of one (loads complete in issue order: a
+    // wait for a later B load is a wait for every earlier row load).
+    constexpr bool kLateRows = NT == 2 || NT == 4;  // eight stages: one more in 8 not worth a wave
+    auto load_rows_all = [&](uint32_t G, u32x4 (&x)[RT][2]) {
+        const uint32_t w = min(kGroupWords * G + 4 * g, W - 4);
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const g_u32x4* src = (const g_u32x4*)(rp[r] + w);
+            x[r][0] = src[0];
+            x[r][1] = src[1];
+        }
+    };
     // stage k = (group k / NT, part k mod NT) into buffer k & 1: piece i =
     // (tile i / QS, q-block part QS + i mod QS), 1 KB each, four per wave
     const uint32_t n_stages = NG * NT;
@@ -178,25 +214,27 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
     // divergent, of unknown length, leaves it unable to count the loads in
     // flight, and every later wait for a load becomes a wait for all of them
     const uint32_t wave_s = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave);
-    auto stage_b = [&](uint32_t k) {
-        if (k < n_stages) {
-            const uint32_t G = k / NT, q0 = (k % NT) * QS;
+    auto stage_b_at = [&](uint32_t k) {  // k < n_stages
+        const uint32_t G = k / NT, q0 = (k % NT) * QS;
 #pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-                const uint32_t i = wave_s + 4 * j;
-                const uint32_t nt = i / QS, q = q0 + i % QS;
-                const g_u32x4* src =
-                    (const g_u32x4*)(packed + (((size_t)nt * NG + G) * 16 + q) * 64) + lane;
-                __builtin_amdgcn_global_load_lds(
-                    (const __attribute__((address_space(1))) void*)src,
-                    (__attribute__((address_space(3))) void*)(((k & 1) ? s_b1 : s_b0) + i * 64),
-                    16, 0, 0);
-            }
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = wave_s + 4 * j;
+            const uint32_t nt = i / QS, q = q0 + i % QS;
+            const g_u32x4* src =
+                (const g_u32x4*)(packed + (((size_t)nt * NG + G) * 16 + q) * 64) + lane;
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void*)src,
+                (__attribute__((address_space(3))) void*)(((k & 1) ? s_b1 : s_b0) + i * 64), 16,
+                0, 0);
         }
+    };
+    auto stage_b = [&](uint32_t k) {
+        if (k < n_stages) stage_b_at(k);
     };
     // vmcnt(0) only (expcnt and lgkmcnt at their maxima): the direct-to-LDS
     // loads and the next group's rows have landed
     auto wait_loads = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
+
     u32x4 x[RT][2];
     if (NG) load_rows(0, x);
     stage_b(0);
@@ -204,11 +242,19 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
     __syncthreads();
     for (uint32_t G = 0; G < NG; ++G) {
         u32x4 xn[RT][2];
-        if (G + 1 < NG) load_rows(G + 1, xn);
+        if (!kLateRows && G + 1 < NG) load_rows(G + 1, xn);
 #pragma unroll
         for (int s = 0; s < NT; ++s) {
             const uint32_t k = G * NT + s;
-            stage_b(k + 1);
+            if (kLateRows && s == 0) {
+                stage_b_at(k + 1);  // G NT + 1 < NG NT
+                asm volatile("" ::: "memory");
+                load_rows_all(min(G + 1, NG - 1), xn);
+                asm volatile("" ::: "memory");
+            } else {
+                stage_b(k + 1);
+                if constexpr (kLateRows) asm volatile("" ::: "memory");
+            }
 #pragma unroll
             for (int qq = 0; qq < QS; ++qq) {
                 const int q = s * QS + qq;
@@ -233,13 +279,31 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
                             __builtin_amdgcn_mfma_i32_16x16x64_i8(a[r], bv, acc[r][nt], 0, 0, 0);
                 }
             }
-            wait_loads();
-            __syncthreads();  // stage k + 1 complete; buffer k & 1 free
+            // stage k + 1 complete; buffer k & 1 free
+            if constexpr (kLateRows) {
+                if (s == 0)
+                    stage_barrier<2 * RT>();  // the next group's rows stay in flight
+                else
+                    stage_barrier<0>();
+            } else {
+                wait_loads();
+                __syncthreads();
+            }
         }
+        if constexpr (kLateRows) {
+            const bool wok = kGroupWords * (G + 1) + 4 * g < W;
 #pragma unroll
-        for (int r = 0; r < RT; ++r) {
-            x[r][0] = xn[r][0];
-            x[r][1] = xn[r][1];
+            for (int r = 0; r < RT; ++r) {
+                const bool ok = rv[r] && wok;
+                x[r][0] = ok ? xn[r][0] : u32x4{0, 0, 0, 0};
+                x[r][1] = ok ? xn[r][1] : u32x4{0, 0, 0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                x[r][0] = xn[r][0];
+                x[r][1] = xn[r][1];
+            }
         }
     }
     // recombine the digit planes: column n = 4 t_local + p held by lane m = n
