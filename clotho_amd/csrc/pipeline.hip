@@ -476,10 +476,15 @@ struct GametesOp {
         if (ranked)
             rank = __hip_atomic_fetch_add(gbl(B.parent_cnt) + p, 1u, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
+        // one 16-byte store per draw of four keys: the keys past ne in the
+        // last group are never read (every reader stops at the event count),
+        // and e + 3 < ev_cap since e < ne <= ev_cap and both e and ev_cap
+        // are multiples of 4 (so is the row offset: the stores are aligned)
         const auto ev = events + (size_t)o * P.ev_cap;
         for (uint32_t e = 0; e < ne; e += 4) {
             const Draw4 d = philox_draw(P.seed, gen, S_GAMETE, P.replicate, o, 1 + e / 4);
-            for (uint32_t q = 0; q < 4 && e + q < ne; ++q) ev[e + q] = d.x[q];
+            *(__attribute__((address_space(1))) i8::u32x4*)(ev + e) =
+                i8::u32x4{d.x[0], d.x[1], d.x[2], d.x[3]};
         }
         if (ranked) gbl(B.prank)[o] = rank;
     }
