@@ -196,7 +196,9 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
     // leaves exactly these loads in flight, so they have two stages of
     // products to land in instead of one (loads complete in issue order: a
     // wait for a later B load is a wait for every earlier row load).
-    constexpr bool kLateRows = NT == 2 || NT == 4;  // eight stages: one more in 8 not worth a wave
+    // (With eight stages per group the rows would gain one stage in eight,
+    // and the 32-trait kernel would lose a wave per SIMD to the registers.)
+    constexpr bool kLateRows = NT == 2 || NT == 4;
     auto load_rows_all = [&](uint32_t G, u32x4 (&x)[RT][2]) {
         const uint32_t w = min(kGroupWords * G + 4 * g, W - 4);
 #pragma unroll
@@ -234,7 +236,6 @@ __device__ __forceinline__ void traits_i8_wave(const uint64_t* __restrict__ rows
     // vmcnt(0) only (expcnt and lgkmcnt at their maxima): the direct-to-LDS
     // loads and the next group's rows have landed
     auto wait_loads = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };
-
     u32x4 x[RT][2];
     if (NG) load_rows(0, x);
     stage_b(0);
